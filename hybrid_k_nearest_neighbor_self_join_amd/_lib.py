@@ -1,0 +1,224 @@
+"""ctypes binding of libgpuknnselfjoin.so (the C ABI declared in include/gpuknnselfjoin.h).
+
+The shared library is built in-tree by `build.py`; importing this module never falls back to
+another implementation: a missing library is built once with hipcc, and a library that cannot
+be loaded raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import threading
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libgpuknnselfjoin.so")
+
+KNN_OK, KNN_ERR_ARG, KNN_ERR_TOO_FEW, KNN_ERR_EPS_ZERO, KNN_ERR_NO_PROGRESS = 0, 1, 2, 3, 4
+KNN_ERR_HIP, KNN_ERR_NOMEM, KNN_ERR_NO_DEVICE, KNN_ERR_MODEL = 5, 6, 7, 8
+KNN_FLOAT32, KNN_FLOAT64 = 0, 1
+KNN_MODE_BETA, KNN_MODE_LEARNED = 0, 1
+
+STATUS_NAMES = {
+    0: "KNN_OK", 1: "KNN_ERR_ARG", 2: "KNN_ERR_TOO_FEW", 3: "KNN_ERR_EPS_ZERO",
+    4: "KNN_ERR_NO_PROGRESS", 5: "KNN_ERR_HIP", 6: "KNN_ERR_NOMEM", 7: "KNN_ERR_NO_DEVICE",
+    8: "KNN_ERR_MODEL",
+}
+
+# every symbol include/gpuknnselfjoin.h declares
+EXPORTED_SYMBOLS = [
+    "KNNJoinPy", "KNNJoinPyDouble", "knnjoin_ex", "knnjoin_default_params",
+    "knnjoin_last_status", "knnjoin_last_error", "knnjoin_release", "knnjoin_stage_reorder",
+    "knnjoin_stage_estimate_eps", "knnjoin_stage_index", "knnjoin_stage_join_pass",
+]
+
+
+class KnnParams(ctypes.Structure):
+    _fields_ = [
+        ("num_indexed_dims", ctypes.c_uint32),
+        ("beta", ctypes.c_double),
+        ("mode", ctypes.c_int32),
+        ("reorder", ctypes.c_int32),
+        ("verbose", ctypes.c_int32),
+        ("device_ptrs", ctypes.c_int32),
+        ("stream", ctypes.c_void_p),
+    ]
+
+
+class KnnStats(ctypes.Structure):
+    _fields_ = [
+        ("perm", ctypes.c_uint32 * 16),
+        ("eps0", ctypes.c_float),
+        ("eps_final", ctypes.c_float),
+        ("iterations", ctypes.c_uint32),
+        ("reindexes", ctypes.c_uint32),
+        ("kernel_launches", ctypes.c_uint32),
+        ("failed_final", ctypes.c_uint32),
+        ("candidates", ctypes.c_uint64),
+        ("algo_bytes", ctypes.c_uint64),
+        ("checksum", ctypes.c_double),
+        ("ms_total", ctypes.c_double),
+        ("ms_reorder", ctypes.c_double),
+        ("ms_eps", ctypes.c_double),
+        ("ms_index", ctypes.c_double),
+        ("ms_join", ctypes.c_double),
+        ("ms_other", ctypes.c_double),
+        ("ms_range_kernel", ctypes.c_double),
+        ("ms_range_kernel_max", ctypes.c_double),
+        ("algo_bytes_max", ctypes.c_uint64),
+    ]
+
+    def as_dict(self) -> dict:
+        d = {}
+        for name, _ in self._fields_:
+            v = getattr(self, name)
+            d[name] = list(v) if name == "perm" else v
+        return d
+
+
+class KnnError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{STATUS_NAMES.get(code, code)}: {msg}")
+        self.code = code
+
+
+_lib = None
+_lock = threading.Lock()
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    with _lock:
+        if _lib is None:
+            if not os.path.exists(LIB_PATH):
+                from . import build as _build
+                _build.build()
+            L = ctypes.CDLL(LIB_PATH)
+            _setup(L)
+            _lib = L
+    return _lib
+
+
+def _setup(L):
+    vp, u32, u64, i32, dbl, flt = (ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint64,
+                                   ctypes.c_int, ctypes.c_double, ctypes.c_float)
+    L.KNNJoinPy.argtypes = [vp, ctypes.c_uint, ctypes.c_uint, ctypes.c_uint, vp, vp]
+    L.KNNJoinPy.restype = None
+    L.KNNJoinPyDouble.argtypes = [vp, ctypes.c_uint, ctypes.c_uint, ctypes.c_uint, vp, vp]
+    L.KNNJoinPyDouble.restype = None
+    L.knnjoin_ex.argtypes = [vp, i32, u32, u32, u32, ctypes.POINTER(KnnParams), vp, vp,
+                             ctypes.POINTER(KnnStats)]
+    L.knnjoin_ex.restype = i32
+    L.knnjoin_default_params.argtypes = [ctypes.POINTER(KnnParams)]
+    L.knnjoin_default_params.restype = None
+    L.knnjoin_last_status.argtypes = []
+    L.knnjoin_last_status.restype = i32
+    L.knnjoin_last_error.argtypes = []
+    L.knnjoin_last_error.restype = ctypes.c_char_p
+    L.knnjoin_release.argtypes = []
+    L.knnjoin_release.restype = None
+    L.knnjoin_stage_reorder.argtypes = [vp, u32, u32, vp, vp]
+    L.knnjoin_stage_reorder.restype = i32
+    L.knnjoin_stage_estimate_eps.argtypes = [vp, u32, u32, u32, dbl, vp, vp, vp, vp, vp]
+    L.knnjoin_stage_estimate_eps.restype = i32
+    L.knnjoin_stage_index.argtypes = [vp, u32, u32, u32, flt, vp, vp, vp, vp, vp, vp, vp]
+    L.knnjoin_stage_index.restype = i32
+    L.knnjoin_stage_join_pass.argtypes = [vp, u32, u32, u32, u32, flt, vp, u32, vp, vp, vp, vp]
+    L.knnjoin_stage_join_pass.restype = i32
+    for name in ("knnjoin_learned_train", "knnjoin_learned_set_model", "knnjoin_batched"):
+        if hasattr(L, name):
+            pass
+
+
+def p(a) -> ctypes.c_void_p | None:
+    if a is None:
+        return None
+    if isinstance(a, int):
+        return ctypes.c_void_p(a)
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def check(rc: int):
+    if rc != KNN_OK:
+        msg = lib().knnjoin_last_error()
+        raise KnnError(rc, msg.decode() if msg else "")
+
+
+def default_params(**kw) -> KnnParams:
+    prm = KnnParams()
+    lib().knnjoin_default_params(ctypes.byref(prm))
+    for k, v in kw.items():
+        setattr(prm, k, v)
+    return prm
+
+
+# ------------------------------------------------------------------ convenience wrappers
+def knnjoin_ex(points: np.ndarray, k: int, *, num_indexed_dims: int = 0, beta: float = 0.05,
+               mode: int = KNN_MODE_BETA, reorder: int = 1, verbose: int = 0):
+    """Whole join through knnjoin_ex on host arrays.  Returns (ids[N,K+1], dists[N,K+1], stats)."""
+    pts = np.ascontiguousarray(points)
+    if pts.dtype not in (np.float32, np.float64):
+        pts = pts.astype(np.float32)
+    n, d = pts.shape
+    dt = KNN_FLOAT32 if pts.dtype == np.float32 else KNN_FLOAT64
+    ids = np.full((n, k + 1), -1, np.int32)
+    dists = np.zeros((n, k + 1), pts.dtype)
+    prm = default_params(num_indexed_dims=num_indexed_dims, beta=beta, mode=mode,
+                         reorder=reorder, verbose=verbose)
+    st = KnnStats()
+    rc = lib().knnjoin_ex(p(pts), dt, n, d, k, ctypes.byref(prm), p(ids), p(dists),
+                          ctypes.byref(st))
+    check(rc)
+    return ids, dists, st
+
+
+def stage_reorder(points: np.ndarray):
+    pts = np.ascontiguousarray(points, np.float32)
+    n, d = pts.shape
+    perm = np.zeros(d, np.uint32)
+    var = np.zeros(d, np.float32)
+    check(lib().knnjoin_stage_reorder(p(pts), n, d, p(perm), p(var)))
+    return perm, var
+
+
+def stage_estimate_eps(points: np.ndarray, k: int, beta: float):
+    pts = np.ascontiguousarray(points, np.float32)
+    n, d = pts.shape
+    eps = np.zeros(1, np.float32)
+    avg = np.zeros(1, np.float64)
+    cnt = np.zeros(1, np.uint64)
+    bucket = np.zeros(1, np.uint32)
+    hist = np.zeros(10000, np.uint32)
+    check(lib().knnjoin_stage_estimate_eps(p(pts), n, d, k, beta, p(eps), p(avg), p(cnt),
+                                           p(bucket), p(hist)))
+    return float(eps[0]), float(avg[0]), int(cnt[0]), int(bucket[0]), hist
+
+
+def stage_index(points: np.ndarray, nd: int, eps: float):
+    pts = np.ascontiguousarray(points, np.float32)
+    n, d = pts.shape
+    minArr = np.zeros(nd, np.float32)
+    nCells = np.zeros(nd, np.uint32)
+    tot = np.zeros(1, np.uint64)
+    sortedIds = np.zeros(n, np.uint32)
+    cellKeys = np.zeros(n, np.uint64)
+    cellStart = np.zeros(n + 1, np.uint32)
+    ncell = np.zeros(1, np.uint32)
+    check(lib().knnjoin_stage_index(p(pts), n, d, nd, eps, p(minArr), p(nCells), p(tot),
+                                    p(sortedIds), p(cellKeys), p(cellStart), p(ncell)))
+    c = int(ncell[0])
+    return minArr, nCells, int(tot[0]), sortedIds, cellKeys[:c].copy(), cellStart[:c + 1].copy()
+
+
+def stage_join_pass(points: np.ndarray, nd: int, k: int, eps: float, queries: np.ndarray):
+    pts = np.ascontiguousarray(points, np.float32)
+    n, d = pts.shape
+    q = np.ascontiguousarray(queries, np.uint32)
+    ids = np.full((n, k + 1), -1, np.int32)
+    dists = np.zeros((n, k + 1), np.float32)
+    failed = np.zeros(n, np.uint8)
+    nf = np.zeros(1, np.uint32)
+    check(lib().knnjoin_stage_join_pass(p(pts), n, d, nd, k, eps, p(q), len(q), p(ids), p(dists),
+                                        p(failed), p(nf)))
+    return ids, dists, failed, int(nf[0])

@@ -1,0 +1,847 @@
+// api.hip -- host orchestration and the C ABI of libgpuknnselfjoin.so.
+//
+// Pipeline of one join (main.cu:109-510 / KNNJoinPy main.cu:513-908), all on one HIP stream:
+//   points (host -> HBM, or already in HBM)
+//   -> variance per dim, dim permutation (main.cu:2521-2615)       estimator.hip
+//   -> BETA epsilon estimate (GPU.cu:3088-3566)                     estimator.hip
+//   -> min/max, grid dims (main.cu:2425-2488)                       prims.hip + host
+//   -> index: cell keys, radix sort, cell table (GPU.cu:3715-4067)  join.hip + prims.hip
+//   -> loop (main.cu:305-430): active tiles -> range top-(K+1) kernel -> failure count;
+//      eps += eps0/2 and re-index when more than 25% failed.
+// The host only reads back scalars (variances, the 40 KB k-distance histogram, cell and tile
+// counts, the failure count); the table stays in HBM until the caller's copy.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "gpuknnselfjoin.h"
+#include "knn_launch.h"
+
+namespace knn {
+
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+#define HIP_CHECK(expr)                                                                     \
+  do {                                                                                      \
+    hipError_t e_ = (expr);                                                                 \
+    if (e_ != hipSuccess)                                                                   \
+      throw ::knn::Error(e_ == hipErrorOutOfMemory ? KNN_ERR_NOMEM : KNN_ERR_HIP,           \
+                         std::string(#expr) + ": " + hipGetErrorString(e_));                \
+  } while (0)
+
+// ------------------------------------------------------------------ buffer cache
+enum Slot {
+  S_PTS, S_X, S_XS, S_KEYS, S_KEYS_TMP, S_IDS, S_IDS_TMP, S_CELLKEYS, S_CELLSTART, S_FLAGS, S_POS,
+  S_ACT, S_NTILES, S_TILEOFF, S_TILES, S_FAILED, S_ROWSUM, S_OUTIDS, S_OUTDISTS, S_SCRATCH,
+  S_HEAP, S_SMALL, S_PART, S_CTR, S_CSUM, S_HOSTFLAGS, S_NSLOTS
+};
+
+struct Cache {
+  void *p[S_NSLOTS] = {};
+  size_t cap[S_NSLOTS] = {};
+  void *get(Slot s, size_t bytes) {
+    if (bytes == 0) bytes = 16;
+    if (cap[s] < bytes) {
+      if (p[s]) HIP_CHECK(hipFree(p[s]));
+      p[s] = nullptr;
+      cap[s] = 0;
+      size_t b = (bytes + 4095) & ~size_t(4095);
+      HIP_CHECK(hipMalloc(&p[s], b));
+      cap[s] = b;
+    }
+    return p[s];
+  }
+  void release() {
+    for (int i = 0; i < S_NSLOTS; i++) {
+      if (p[i]) (void)hipFree(p[i]);
+      p[i] = nullptr;
+      cap[i] = 0;
+    }
+  }
+};
+
+static std::mutex g_mu;
+static Cache g_cache;
+static hipStream_t g_stream = nullptr;
+static int g_status = KNN_OK;
+static std::string g_err;
+
+static hipStream_t lib_stream() {
+  if (!g_stream) {
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+      throw Error(KNN_ERR_NO_DEVICE, "no HIP device visible");
+    HIP_CHECK(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
+  }
+  return g_stream;
+}
+
+// ------------------------------------------------------------------ host-side numerics
+// main.cu:330-339: eps_est += (eps_est_initial*0.5), DTYPE eps, double literal.
+static inline float grow_eps(float eps, float eps0) { return (float)((double)eps + (double)eps0 * 0.5); }
+static inline double grow_eps(double eps, double eps0) { return eps + eps0 * 0.5; }
+
+template <typename T>
+static void grid_dims(const T *mn, const T *mx, uint32_t nd, T eps, typename GridOf<T>::type &g,
+                      uint64_t &totalCells) {
+  // main.cu:2465-2486
+  g.eps = eps;
+  g.nd = nd;
+  uint64_t tot = 1;
+  for (uint32_t j = 0; j < nd; j++) {
+    T lo = mn[j] - eps;
+    T hi = mx[j] + eps;
+    g.minArr[j] = lo;
+    T q = (hi - lo) / eps;
+    T c = std::ceil(q);
+    if (!(c >= (T)1) || c > (T)4294967295.0)
+      throw Error(KNN_ERR_ARG, "grid: cell count per dimension out of range (eps too small)");
+    g.nCells[j] = (uint32_t)c;
+    g.mult[j] = tot;
+    if (tot > UINT64_MAX / g.nCells[j]) throw Error(KNN_ERR_ARG, "grid: more than 2^64 cells");
+    tot *= g.nCells[j];
+  }
+  for (uint32_t j = nd; j < KNN_MAX_INDEXED; j++) {
+    g.minArr[j] = 0;
+    g.nCells[j] = 1;
+    g.mult[j] = tot;
+  }
+  totalCells = tot;
+}
+
+// eps0 when the BETA estimate is 0.  The reference divides by zero then (N <= 1000: its
+// distance sample is point 0 against itself; or all sampled pairs coincide).  Any eps > 0
+// gives the same exact rows; take the radius holding K+1 points of a uniform set over the
+// bounding box: largest indexed extent * (K1/N)^(1/nd), or 1 when all points coincide.
+template <typename T>
+static T fallback_eps(const T *mn, const T *mx, uint32_t nd, uint32_t n, uint32_t k1) {
+  T ext = 0;
+  for (uint32_t j = 0; j < nd; j++) {
+    T e = mx[j] - mn[j];
+    if (e > ext) ext = e;
+  }
+  T eps = (T)((double)ext * std::pow((double)k1 / (double)n, 1.0 / (double)nd));
+  return eps > (T)0 ? eps : (T)1;
+}
+
+static int bits_for(uint64_t totalCells) {
+  uint64_t m = totalCells ? totalCells - 1 : 0;
+  int b = 0;
+  while (m) {
+    b++;
+    m >>= 1;
+  }
+  return b;
+}
+
+// ------------------------------------------------------------------ pipeline
+struct PassResult {
+  uint32_t nfail, ntiles;
+  double kernel_ms;
+  unsigned long long bytes_total;
+};
+
+struct Timer;
+
+template <typename T>
+struct Pipeline {
+  using G = typename GridOf<T>::type;
+  hipStream_t s;
+  uint32_t n, d, nd, k, K1;
+  T *x = nullptr;  // reordered dim-major
+  T *xs = nullptr; // cell-sorted dim-major
+  uint32_t *sortedIds = nullptr;
+  uint64_t *cellKeys = nullptr;
+  uint32_t *cellStart = nullptr;
+  uint32_t ncell = 0;
+  G g{};
+  uint64_t totalCells = 0;
+  uint32_t *scratch = nullptr;  // u32 words
+  size_t scratchWords = 0;
+  bool verbose = false;
+
+  template <typename V>
+  V *buf(Slot sl, size_t count) { return reinterpret_cast<V *>(g_cache.get(sl, sizeof(V) * count)); }
+
+  void sync() { HIP_CHECK(hipStreamSynchronize(s)); }
+
+  // populateNDGridIndexAndLookupArrayGPU (GPU.cu:3715-4067), all in HBM
+  void build_index(const T *mn, const T *mx, T eps) {
+    grid_dims<T>(mn, mx, nd, eps, g, totalCells);
+    const int bits = bits_for(totalCells);
+    uint32_t *ids = buf<uint32_t>(S_IDS, n);
+    uint32_t *ids_tmp = buf<uint32_t>(S_IDS_TMP, n);
+    if (bits <= 32) {
+      uint32_t *keys = buf<uint32_t>(S_KEYS, n);
+      uint32_t *kt = buf<uint32_t>(S_KEYS_TMP, n);
+      cell_keys_u32<T>(x, n, g, keys, ids, s);
+      radix_sort_pairs_u32(keys, ids, kt, ids_tmp, n, bits, scratch, s);
+      uint32_t *flags = buf<uint32_t>(S_FLAGS, n + 1);
+      uint32_t *pos = buf<uint32_t>(S_POS, n + 1);
+      boundary_flags_u32(keys, n, flags, s);
+      scan_exclusive_u32(flags, pos, n, scratch, s);
+      cellKeys = buf<uint64_t>(S_CELLKEYS, n);
+      cellStart = buf<uint32_t>(S_CELLSTART, n + 1);
+      compact_cells_u32(keys, flags, pos, n, cellKeys, cellStart, s);
+      HIP_CHECK(hipMemcpyAsync(&ncell, pos + n, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    } else {
+      uint64_t *keys = buf<uint64_t>(S_KEYS, n);
+      uint64_t *kt = buf<uint64_t>(S_KEYS_TMP, n);
+      cell_keys_u64<T>(x, n, g, keys, ids, s);
+      radix_sort_pairs_u64(keys, ids, kt, ids_tmp, n, bits, scratch, s);
+      uint32_t *flags = buf<uint32_t>(S_FLAGS, n + 1);
+      uint32_t *pos = buf<uint32_t>(S_POS, n + 1);
+      boundary_flags_u64(keys, n, flags, s);
+      scan_exclusive_u32(flags, pos, n, scratch, s);
+      cellKeys = buf<uint64_t>(S_CELLKEYS, n);
+      cellStart = buf<uint32_t>(S_CELLSTART, n + 1);
+      compact_cells_u64(keys, flags, pos, n, cellKeys, cellStart, s);
+      HIP_CHECK(hipMemcpyAsync(&ncell, pos + n, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    }
+    sortedIds = ids;
+    xs = buf<T>(S_XS, (size_t)n * d);
+    gather_soa<T>(x, sortedIds, n, d, xs, s);
+    sync();
+  }
+
+  // One range pass (kernel.cu:61-225 + GPU.cu:3610-3704 + main.cu:1192-1223): queries with
+  // activeFailed[q] != 0 (all when null), grouped by cell into tiles of <= 64.
+  PassResult pass(const uint8_t *activeFailed, uint8_t *failed, int32_t *dIds, T *dDists,
+                  double *rowSum, unsigned long long *ctr, uint32_t *failCount, Timer &tK);
+};
+
+template <typename T>
+static void sort_perm_desc(const T *var, uint32_t d, uint32_t *perm) {
+  // std::sort with compareByDimVariance (main.cu:44-47, 2580) on <= 16 elements is
+  // libstdc++'s insertion sort: stable, descending
+  for (uint32_t i = 0; i < d; i++) perm[i] = i;
+  for (uint32_t i = 1; i < d; i++) {
+    uint32_t p = perm[i];
+    T key = var[p];
+    int j = (int)i - 1;
+    while (j >= 0 && key > var[perm[j]]) {
+      perm[j + 1] = perm[j];
+      j--;
+    }
+    perm[j + 1] = p;
+  }
+}
+
+struct EpsResult {
+  double avg;
+  uint64_t cnt;
+  double bw;
+  uint32_t offset, nsample, bucket;
+  double best;
+};
+
+// sampleNeighborsBruteForce GPU.cu:3088-3566 on the reordered dim-major points
+template <typename T>
+static EpsResult estimate_eps(const T *x, uint32_t n, uint32_t d, uint32_t k, double beta,
+                              uint32_t *hist_out, hipStream_t s) {
+  EpsResult r{};
+  const size_t m = avgdist_threads(n);
+  double *part = reinterpret_cast<double *>(g_cache.get(S_PART, m * 16 + 64));
+  uint64_t *cnts = reinterpret_cast<uint64_t *>(part + m);
+  double *tot = reinterpret_cast<double *>(cnts + m);
+  uint64_t *cnt = reinterpret_cast<uint64_t *>(tot + 1);
+  avg_dist_soa<T>(x, n, d, part, cnts, tot, cnt, s);
+  double h_tot = 0;
+  uint64_t h_cnt = 0;
+  HIP_CHECK(hipMemcpyAsync(&h_tot, tot, sizeof(double), hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipMemcpyAsync(&h_cnt, cnt, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipStreamSynchronize(s));
+  r.avg = h_cnt ? h_tot / (h_cnt * 1.0) : 0.0;  // GPU.cu:3334
+  r.cnt = h_cnt;
+  const uint32_t nb = 10000;
+  r.bw = r.avg / (nb * 1.0);
+  double offsetRate = 0.01;
+  if (n >= 50000) {
+    double frac = (50000.0 / n) * offsetRate;
+    r.offset = (uint32_t)(1.0 / frac);
+  } else {
+    r.offset = (uint32_t)(1.0 / offsetRate);
+  }
+  r.nsample = n / r.offset;
+  uint32_t *hist = reinterpret_cast<uint32_t *>(g_cache.get(S_SMALL, 64 * 1024));
+  kdist_hist_soa<T>(x, n, d, r.offset, r.nsample, r.avg, r.bw, hist, s);
+  std::vector<uint32_t> h(nb);
+  HIP_CHECK(hipMemcpyAsync(h.data(), hist, nb * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipStreamSynchronize(s));
+  // GPU.cu:3484-3527
+  uint32_t cumulative = 0;
+  int found = -1;
+  double best = 0;
+  unsigned long min_pts = (unsigned long)(k * 1.0);
+  unsigned long max_pts = (unsigned long)(k * 100.0);
+  unsigned long thr = (unsigned long)(min_pts + (max_pts - min_pts) * beta);
+  for (uint32_t i = 0; i < nb; i++) {
+    cumulative += h[i];
+    double per_point = cumulative / (1.0 * r.nsample);
+    if (per_point > thr && found == -1) {
+      best = r.bw * (i + 1.0);
+      found = (int)i;
+    }
+    if (i == nb - 1 && found == -1) {
+      best = r.bw * (i + 1.0);
+      found = (int)i;
+    }
+  }
+  r.bucket = (uint32_t)found;
+  r.best = best;
+  if (hist_out) std::memcpy(hist_out, h.data(), nb * sizeof(uint32_t));
+  return r;
+}
+
+struct Timer {
+  hipEvent_t a, b;
+  hipStream_t s;
+  explicit Timer(hipStream_t st) : s(st) {
+    HIP_CHECK(hipEventCreate(&a));
+    HIP_CHECK(hipEventCreate(&b));
+  }
+  ~Timer() {
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+  }
+  void start() { HIP_CHECK(hipEventRecord(a, s)); }
+  double stop() {
+    HIP_CHECK(hipEventRecord(b, s));
+    HIP_CHECK(hipEventSynchronize(b));
+    float ms = 0;
+    HIP_CHECK(hipEventElapsedTime(&ms, a, b));
+    return ms;
+  }
+};
+
+template <typename T>
+PassResult Pipeline<T>::pass(const uint8_t *activeFailed, uint8_t *failed, int32_t *dIds,
+                             T *dDists, double *rowSum, unsigned long long *ctr,
+                             uint32_t *failCount, Timer &tK) {
+  uint32_t *flags = buf<uint32_t>(S_FLAGS, n + 1);
+  uint32_t *pos = buf<uint32_t>(S_POS, n + 1);
+  active_flags(sortedIds, activeFailed, n, flags, s);
+  scan_exclusive_u32(flags, pos, n, scratch, s);
+  uint32_t *act;
+  if (!activeFailed) {
+    act = sortedIds;
+  } else {
+    act = buf<uint32_t>(S_ACT, n);
+    compact_active(sortedIds, flags, pos, n, act, s);
+  }
+  uint32_t *ntiles = buf<uint32_t>(S_NTILES, (size_t)ncell + 1);
+  uint32_t *tileOff = buf<uint32_t>(S_TILEOFF, (size_t)ncell + 1);
+  cell_tile_counts(cellStart, pos, ncell, ntiles, s);
+  scan_exclusive_u32(ntiles, tileOff, ncell, scratch, s);
+  uint32_t NT = 0;
+  HIP_CHECK(hipMemcpyAsync(&NT, tileOff + ncell, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipStreamSynchronize(s));
+  uint4 *tiles = buf<uint4>(S_TILES, NT ? NT : 1);
+  write_tiles(cellStart, pos, tileOff, ncell, tiles, s);
+  fill_u32(failCount, 0u, 1, s);
+
+  JoinArgs a{};
+  a.x = x;
+  a.xs = xs;
+  a.sortedIds = sortedIds;
+  a.cellKeys = cellKeys;
+  a.cellStart = cellStart;
+  a.ncell = ncell;
+  a.act = act;
+  a.tiles = tiles;
+  a.ntiles = NT;
+  a.n = n;
+  a.d = d;
+  a.k1 = K1;
+  a.outIds = dIds;
+  a.outDists = dDists;
+  a.rowSum = rowSum;
+  a.failed = failed;
+  a.failCount = failCount;
+  a.candCount = ctr;
+  size_t hs = range_heap_scratch_bytes<T>(NT, K1);
+  a.heapScratch = hs ? g_cache.get(S_HEAP, hs) : nullptr;
+  tK.start();
+  range_topk<T>(a, g, s);
+  PassResult r{};
+  r.kernel_ms = tK.stop();
+  r.ntiles = NT;
+  unsigned long long hctr[2];
+  HIP_CHECK(hipMemcpyAsync(&r.nfail, failCount, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipMemcpyAsync(hctr, ctr, sizeof(hctr), hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipStreamSynchronize(s));
+  r.bytes_total = hctr[1];
+  return r;
+}
+
+namespace {
+__global__ void k_fill_failed_rows(const uint8_t *failed, uint32_t n, uint32_t k1, int32_t *ids,
+                                   int all) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (uint64_t)n * k1) return;
+  const uint32_t q = (uint32_t)(i / k1);
+  if (all || failed[q]) ids[i] = -1;
+}
+}  // namespace
+
+template <typename T>
+static int run_join(const T *pts_in, uint32_t n, uint32_t d, uint32_t k, const knnjoin_params &p,
+                    int32_t *outIds, T *outDists, knnjoin_stats *st) {
+  knnjoin_stats stats{};
+  if (n == 0) {
+    if (st) *st = stats;
+    return KNN_OK;
+  }
+  if (!pts_in || !outIds || !outDists) throw Error(KNN_ERR_ARG, "null pointer argument");
+  if (d == 0 || d > KNN_MAXDIM) throw Error(KNN_ERR_ARG, "NDIM must be in 1..16");
+  uint32_t nd = p.num_indexed_dims ? p.num_indexed_dims : std::min<uint32_t>(d, 6);
+  if (nd > d || nd > KNN_MAX_INDEXED) throw Error(KNN_ERR_ARG, "NUMINDEXEDDIM must be <= NDIM and <= 8");
+  if ((uint64_t)n < (uint64_t)k + 1)
+    throw Error(KNN_ERR_TOO_FEW, "N < K+1: no point has K+1 neighbours (the reference never terminates)");
+  const double beta = p.beta < 0 ? 0.05 : p.beta;
+  const uint32_t K1 = k + 1;
+  const bool devptr = p.device_ptrs != 0;
+  hipStream_t s = (devptr && p.stream) ? reinterpret_cast<hipStream_t>(p.stream) : lib_stream();
+
+  Pipeline<T> P;
+  P.s = s;
+  P.n = n;
+  P.d = d;
+  P.nd = nd;
+  P.k = k;
+  P.K1 = K1;
+  P.verbose = p.verbose != 0;
+  P.scratchWords = std::max(scan_scratch_words(n + 1), radix_scratch_words(n)) + 1024;
+  P.scratch = P.template buf<uint32_t>(S_SCRATCH, P.scratchWords);
+
+  Timer tAll(s), tPh(s), tK(s);
+  auto wall0 = std::chrono::steady_clock::now();
+
+  // ---- inputs
+  const T *dpts;
+  int32_t *dIds;
+  T *dDists;
+  if (devptr) {
+    dpts = pts_in;
+    dIds = outIds;
+    dDists = outDists;
+  } else {
+    T *b = P.template buf<T>(S_PTS, (size_t)n * d);
+    HIP_CHECK(hipMemcpyAsync(b, pts_in, sizeof(T) * (size_t)n * d, hipMemcpyHostToDevice, s));
+    dpts = b;
+    dIds = P.template buf<int32_t>(S_OUTIDS, (size_t)n * K1);
+    dDists = P.template buf<T>(S_OUTDISTS, (size_t)n * K1);
+  }
+  uint8_t *failed = P.template buf<uint8_t>(S_FAILED, n);
+  double *rowSum = P.template buf<double>(S_ROWSUM, n);
+  double h2d_ms = 0;
+  if (!devptr) {
+    HIP_CHECK(hipStreamSynchronize(s));
+    h2d_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - wall0).count();
+  }
+  tAll.start();
+
+  // ---- dimension reorder (main.cu:2521-2615)
+  tPh.start();
+  uint32_t perm[KNN_MAXDIM];
+  for (uint32_t j = 0; j < KNN_MAXDIM; j++) perm[j] = j;
+  if (p.reorder) {
+    T *var = P.template buf<T>(S_SMALL, 64 * 1024 / sizeof(T));
+    variance_rows<T>(dpts, n, d, var, s);
+    T hv[KNN_MAXDIM];
+    HIP_CHECK(hipMemcpyAsync(hv, var, sizeof(T) * d, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    sort_perm_desc<T>(hv, d, perm);
+  }
+  P.x = P.template buf<T>(S_X, (size_t)n * d);
+  permute_to_soa<T>(dpts, n, d, perm, P.x, s);
+  stats.ms_reorder = tPh.stop();
+  for (uint32_t j = 0; j < 16; j++) stats.perm[j] = j < d ? perm[j] : 0;
+
+  auto fail_all = [&](int code, const std::string &msg) {
+    const size_t tot = (size_t)n * K1;
+    k_fill_failed_rows<<<(unsigned)((tot + 255) / 256), 256, 0, s>>>(failed, n, K1, dIds, 1);
+    throw Error(code, msg);
+  };
+
+  // ---- epsilon estimate (GPU.cu:3088-3566)
+  tPh.start();
+  EpsResult er = estimate_eps<T>(P.x, n, d, k, beta, nullptr, s);
+  T eps0 = (T)er.best;
+  stats.ms_eps = tPh.stop();
+
+  // ---- min/max of the indexed dims (main.cu:2439-2456)
+  T *mm = P.template buf<T>(S_PART, 2 * KNN_MAX_INDEXED + minmax_scratch_elems(n, nd));
+  minmax_soa<T>(P.x, n, nd, mm, mm + KNN_MAX_INDEXED, mm + 2 * KNN_MAX_INDEXED, s);
+  T hmn[KNN_MAX_INDEXED], hmx[KNN_MAX_INDEXED];
+  HIP_CHECK(hipMemcpyAsync(hmn, mm, sizeof(T) * nd, hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipMemcpyAsync(hmx, mm + KNN_MAX_INDEXED, sizeof(T) * nd, hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipStreamSynchronize(s));
+  if (eps0 == (T)0) eps0 = fallback_eps<T>(hmn, hmx, nd, n, K1);
+  stats.eps0 = (float)eps0;
+
+  // ---- the join loop (main.cu:305-430)
+  tPh.start();
+  T eps = eps0;
+  P.build_index(hmn, hmx, eps);
+  stats.ms_index = tPh.stop();
+
+  unsigned long long *ctr = P.template buf<unsigned long long>(S_CTR, 64);
+  uint32_t *failCount = reinterpret_cast<uint32_t *>(ctr + 4);
+  fill_u32(reinterpret_cast<uint32_t *>(ctr), 0u, 16, s);
+  uint32_t nq = n, numIter = 0, reindexes = 0, launches = 0;
+  double frac = 0;
+  bool first = true;
+  double joinMs = 0, kMs = 0, kMax = 0;
+  uint64_t bytesAtMax = 0, prevBytes = 0;
+  while (nq != 0) {
+    if (numIter != 0) {
+      if (frac > 0.25) {  // main.cu:330-356
+        T grown = grow_eps(eps, eps0);
+        if (grown == (T)0) fail_all(KNN_ERR_EPS_ZERO, "epsilon grew to 0");
+        if (grown == eps) {
+          k_fill_failed_rows<<<(unsigned)(((size_t)n * K1 + 255) / 256), 256, 0, s>>>(failed, n, K1, dIds, 0);
+          throw Error(KNN_ERR_NO_PROGRESS, "eps + eps0/2 == eps: epsilon cannot grow");
+        }
+        eps = grown;
+        tPh.start();
+        P.build_index(hmn, hmx, eps);
+        stats.ms_index += tPh.stop();
+        reindexes++;
+      } else {
+        // The reference reruns the failed queries at the same eps (main.cu:330 is false);
+        // the pass is deterministic, so every one of them fails again (fraction 1.0).
+        numIter++;
+        frac = 1.0;
+        continue;
+      }
+    }
+    tPh.start();
+    PassResult pr = P.pass(first ? nullptr : failed, failed, dIds, dDists, rowSum, ctr, failCount, tK);
+    launches++;
+    kMs += pr.kernel_ms;
+    if (pr.kernel_ms > kMax) {
+      kMax = pr.kernel_ms;
+      bytesAtMax = pr.bytes_total - prevBytes;
+    }
+    prevBytes = pr.bytes_total;
+    const uint32_t nfail = pr.nfail;
+    joinMs += tPh.stop();
+    if (P.verbose)
+      printf("[knn] iter %u eps %.9g queries %u failed %u tiles %u cells %u kernel %.3f ms\n",
+             numIter, (double)eps, nq, nfail, pr.ntiles, P.ncell, pr.kernel_ms);
+    frac = (nfail * 1.0) / (nq * 1.0);  // main.cu:402
+    nq = nfail;
+    numIter++;
+    first = false;
+  }
+  stats.ms_join = joinMs;
+  stats.ms_range_kernel = kMs;
+  stats.ms_range_kernel_max = kMax;
+  stats.algo_bytes_max = bytesAtMax;
+
+  // ---- checksum (main.cu:452)
+  double *part = P.template buf<double>(S_CSUM, 64 + (n + 4095) / 4096 + 16);
+  double *csum = part + (n + 4095) / 4096 + 8;
+  sum_rows(rowSum, n, part, csum, s);
+  double hcs = 0;
+  unsigned long long hctr[2];
+  HIP_CHECK(hipMemcpyAsync(&hcs, csum, sizeof(double), hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipMemcpyAsync(hctr, ctr, sizeof(hctr), hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipStreamSynchronize(s));
+  stats.ms_total = tAll.stop();
+  stats.checksum = hcs;
+  stats.candidates = hctr[0];
+  stats.algo_bytes = hctr[1];
+  stats.eps_final = (float)eps;
+  stats.iterations = numIter;
+  stats.reindexes = reindexes;
+  stats.kernel_launches = launches;
+  stats.failed_final = 0;
+
+  if (!devptr) {
+    auto t0 = std::chrono::steady_clock::now();
+    HIP_CHECK(hipMemcpyAsync(outIds, dIds, sizeof(int32_t) * (size_t)n * K1, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipMemcpyAsync(outDists, dDists, sizeof(T) * (size_t)n * K1, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    stats.ms_other = h2d_ms + std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  }
+  if (P.verbose)
+    printf("[knn] done: iterations %u reindexes %u eps0 %.9g eps %.9g checksum %.6f total %.3f ms\n",
+           numIter, reindexes, (double)eps0, (double)eps, hcs, stats.ms_total);
+  if (st) *st = stats;
+  return KNN_OK;
+}
+
+template <typename T>
+static int guarded_join(const T *pts, uint32_t n, uint32_t d, uint32_t k, const knnjoin_params &p,
+                        int32_t *ids, T *dists, knnjoin_stats *st) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  try {
+    int rc = run_join<T>(pts, n, d, k, p, ids, dists, st);
+    g_status = rc;
+    g_err.clear();
+    return rc;
+  } catch (const Error &e) {
+    g_status = e.code;
+    g_err = e.what();
+    if (p.device_ptrs == 0 && e.code != KNN_ERR_ARG && ids && n && k + 1 > 0 &&
+        e.code != KNN_ERR_HIP && e.code != KNN_ERR_NOMEM && e.code != KNN_ERR_NO_DEVICE) {
+      // copy back the -1 rows of the unresolved queries (main.cu:260-268)
+      int32_t *dIds = reinterpret_cast<int32_t *>(g_cache.p[S_OUTIDS]);
+      if (dIds && g_stream) {
+        (void)hipMemcpyAsync(ids, dIds, sizeof(int32_t) * (size_t)n * (k + 1), hipMemcpyDeviceToHost, g_stream);
+        (void)hipStreamSynchronize(g_stream);
+      }
+    }
+    if (e.code == KNN_ERR_TOO_FEW && ids && p.device_ptrs == 0) {
+      for (size_t i = 0; i < (size_t)n * (k + 1); i++) ids[i] = -1;
+    }
+    return e.code;
+  } catch (const std::exception &e) {
+    g_status = KNN_ERR_HIP;
+    g_err = e.what();
+    return KNN_ERR_HIP;
+  }
+}
+
+}  // namespace knn
+
+// ================================================================== C ABI
+extern "C" {
+
+void knnjoin_default_params(knnjoin_params *p) {
+  if (!p) return;
+  std::memset(p, 0, sizeof(*p));
+  p->num_indexed_dims = 0;
+  p->beta = 0.05;  // params.h:58
+  p->mode = KNN_MODE_BETA;
+  p->reorder = 1;  // params.h:35
+}
+
+int knnjoin_ex(const void *points, int dtype, uint32_t n, uint32_t ndim, uint32_t k,
+               const knnjoin_params *params, int32_t *out_ids, void *out_dists,
+               knnjoin_stats *stats) {
+  knnjoin_params p;
+  if (params) p = *params; else knnjoin_default_params(&p);
+  if (dtype == KNN_FLOAT32)
+    return knn::guarded_join<float>(reinterpret_cast<const float *>(points), n, ndim, k, p, out_ids,
+                                    reinterpret_cast<float *>(out_dists), stats);
+  if (dtype == KNN_FLOAT64)
+    return knn::guarded_join<double>(reinterpret_cast<const double *>(points), n, ndim, k, p, out_ids,
+                                     reinterpret_cast<double *>(out_dists), stats);
+  std::lock_guard<std::mutex> lk(knn::g_mu);
+  knn::g_status = KNN_ERR_ARG;
+  knn::g_err = "dtype must be KNN_FLOAT32 or KNN_FLOAT64";
+  return KNN_ERR_ARG;
+}
+
+void KNNJoinPy(float *dataset, unsigned int NUMPOINTS, unsigned int kNN, unsigned int NDIM,
+               int *outNearestNeighborTable, float *outNearestNeighborTableDistances) {
+  knnjoin_params p;
+  knnjoin_default_params(&p);
+  p.num_indexed_dims = NDIM <= 6 ? NDIM : 6;
+  (void)knnjoin_ex(dataset, KNN_FLOAT32, NUMPOINTS, NDIM, kNN, &p, outNearestNeighborTable,
+                   outNearestNeighborTableDistances, nullptr);
+}
+
+void KNNJoinPyDouble(double *dataset, unsigned int NUMPOINTS, unsigned int kNN, unsigned int NDIM,
+                     int *outNearestNeighborTable, double *outNearestNeighborTableDistances) {
+  knnjoin_params p;
+  knnjoin_default_params(&p);
+  p.num_indexed_dims = NDIM <= 6 ? NDIM : 6;
+  (void)knnjoin_ex(dataset, KNN_FLOAT64, NUMPOINTS, NDIM, kNN, &p, outNearestNeighborTable,
+                   outNearestNeighborTableDistances, nullptr);
+}
+
+int knnjoin_last_status(void) {
+  std::lock_guard<std::mutex> lk(knn::g_mu);
+  return knn::g_status;
+}
+
+const char *knnjoin_last_error(void) {
+  std::lock_guard<std::mutex> lk(knn::g_mu);
+  return knn::g_err.c_str();
+}
+
+void knnjoin_release(void) {
+  std::lock_guard<std::mutex> lk(knn::g_mu);
+  knn::g_cache.release();
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ stage entry points
+namespace knn {
+
+template <typename F>
+static int guarded(F &&f) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  try {
+    int rc = f();
+    g_status = rc;
+    g_err.clear();
+    return rc;
+  } catch (const Error &e) {
+    g_status = e.code;
+    g_err = e.what();
+    return e.code;
+  } catch (const std::exception &e) {
+    g_status = KNN_ERR_HIP;
+    g_err = e.what();
+    return KNN_ERR_HIP;
+  }
+}
+
+// uploads reordered row-major float points and builds the dim-major copy (identity order)
+static void stage_setup(Pipeline<float> &P, const float *points, uint32_t n, uint32_t d,
+                        uint32_t nd, uint32_t k) {
+  if (!points || n == 0 || d == 0 || d > KNN_MAXDIM || nd == 0 || nd > d || nd > KNN_MAX_INDEXED)
+    throw Error(KNN_ERR_ARG, "bad stage arguments");
+  P.s = lib_stream();
+  P.n = n;
+  P.d = d;
+  P.nd = nd;
+  P.k = k;
+  P.K1 = k + 1;
+  P.scratchWords = std::max(scan_scratch_words(n + 1), radix_scratch_words(n)) + 1024;
+  P.scratch = P.buf<uint32_t>(S_SCRATCH, P.scratchWords);
+  float *b = P.buf<float>(S_PTS, (size_t)n * d);
+  HIP_CHECK(hipMemcpyAsync(b, points, sizeof(float) * (size_t)n * d, hipMemcpyHostToDevice, P.s));
+  uint32_t ident[KNN_MAXDIM];
+  for (uint32_t j = 0; j < KNN_MAXDIM; j++) ident[j] = j;
+  P.x = P.buf<float>(S_X, (size_t)n * d);
+  permute_to_soa<float>(b, n, d, ident, P.x, P.s);
+}
+
+static void stage_minmax(Pipeline<float> &P, float *hmn, float *hmx) {
+  float *mm = P.buf<float>(S_PART, 2 * KNN_MAX_INDEXED + minmax_scratch_elems(P.n, P.nd));
+  minmax_soa<float>(P.x, P.n, P.nd, mm, mm + KNN_MAX_INDEXED, mm + 2 * KNN_MAX_INDEXED, P.s);
+  HIP_CHECK(hipMemcpyAsync(hmn, mm, sizeof(float) * P.nd, hipMemcpyDeviceToHost, P.s));
+  HIP_CHECK(hipMemcpyAsync(hmx, mm + KNN_MAX_INDEXED, sizeof(float) * P.nd, hipMemcpyDeviceToHost, P.s));
+  HIP_CHECK(hipStreamSynchronize(P.s));
+}
+
+}  // namespace knn
+
+extern "C" {
+
+int knnjoin_stage_reorder(const float *points, uint32_t n, uint32_t ndim, uint32_t *perm,
+                          float *variance) {
+  return knn::guarded([&]() -> int {
+    using namespace knn;
+    if (!points || !perm || n == 0 || ndim == 0 || ndim > KNN_MAXDIM)
+      throw Error(KNN_ERR_ARG, "bad reorder arguments");
+    hipStream_t s = lib_stream();
+    float *b = reinterpret_cast<float *>(g_cache.get(S_PTS, sizeof(float) * (size_t)n * ndim));
+    HIP_CHECK(hipMemcpyAsync(b, points, sizeof(float) * (size_t)n * ndim, hipMemcpyHostToDevice, s));
+    float *var = reinterpret_cast<float *>(g_cache.get(S_SMALL, 64 * 1024));
+    variance_rows<float>(b, n, ndim, var, s);
+    float hv[KNN_MAXDIM];
+    HIP_CHECK(hipMemcpyAsync(hv, var, sizeof(float) * ndim, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    sort_perm_desc<float>(hv, ndim, perm);
+    if (variance)
+      for (uint32_t j = 0; j < ndim; j++) variance[j] = hv[perm[j]];
+    return KNN_OK;
+  });
+}
+
+int knnjoin_stage_estimate_eps(const float *points, uint32_t n, uint32_t ndim, uint32_t k,
+                               double beta, float *eps, double *avg_dist, uint64_t *pair_count,
+                               uint32_t *bucket, uint32_t *hist) {
+  return knn::guarded([&]() -> int {
+    using namespace knn;
+    Pipeline<float> P;
+    stage_setup(P, points, n, ndim, 1, k);
+    EpsResult er = estimate_eps<float>(P.x, n, ndim, k, beta, hist, P.s);
+    if (eps) *eps = (float)er.best;
+    if (avg_dist) *avg_dist = er.avg;
+    if (pair_count) *pair_count = er.cnt;
+    if (bucket) *bucket = er.bucket;
+    return ((float)er.best == 0.0f) ? KNN_ERR_EPS_ZERO : KNN_OK;
+  });
+}
+
+int knnjoin_stage_index(const float *points, uint32_t n, uint32_t ndim, uint32_t nd, float eps,
+                        float *min_arr, uint32_t *n_cells, uint64_t *total_cells,
+                        uint32_t *sorted_ids, uint64_t *cell_keys, uint32_t *cell_start,
+                        uint32_t *num_cells) {
+  return knn::guarded([&]() -> int {
+    using namespace knn;
+    if (!(eps > 0.0f)) throw Error(KNN_ERR_EPS_ZERO, "eps must be > 0");
+    Pipeline<float> P;
+    stage_setup(P, points, n, ndim, nd, 0);
+    float hmn[KNN_MAX_INDEXED], hmx[KNN_MAX_INDEXED];
+    stage_minmax(P, hmn, hmx);
+    P.build_index(hmn, hmx, eps);
+    if (min_arr) std::memcpy(min_arr, P.g.minArr, sizeof(float) * nd);
+    if (n_cells) std::memcpy(n_cells, P.g.nCells, sizeof(uint32_t) * nd);
+    if (total_cells) *total_cells = P.totalCells;
+    if (num_cells) *num_cells = P.ncell;
+    if (sorted_ids)
+      HIP_CHECK(hipMemcpyAsync(sorted_ids, P.sortedIds, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, P.s));
+    if (cell_keys)
+      HIP_CHECK(hipMemcpyAsync(cell_keys, P.cellKeys, sizeof(uint64_t) * P.ncell, hipMemcpyDeviceToHost, P.s));
+    if (cell_start)
+      HIP_CHECK(hipMemcpyAsync(cell_start, P.cellStart, sizeof(uint32_t) * ((size_t)P.ncell + 1),
+                               hipMemcpyDeviceToHost, P.s));
+    HIP_CHECK(hipStreamSynchronize(P.s));
+    return KNN_OK;
+  });
+}
+
+int knnjoin_stage_join_pass(const float *points, uint32_t n, uint32_t ndim, uint32_t nd,
+                            uint32_t k, float eps, const uint32_t *queries, uint32_t nq,
+                            int32_t *out_ids, float *out_dists, uint8_t *failed,
+                            uint32_t *num_failed) {
+  return knn::guarded([&]() -> int {
+    using namespace knn;
+    if (!(eps > 0.0f)) throw Error(KNN_ERR_EPS_ZERO, "eps must be > 0");
+    if (!out_ids || !out_dists || !failed || (nq && !queries))
+      throw Error(KNN_ERR_ARG, "null output");
+    Pipeline<float> P;
+    stage_setup(P, points, n, ndim, nd, k);
+    float hmn[KNN_MAX_INDEXED], hmx[KNN_MAX_INDEXED];
+    stage_minmax(P, hmn, hmx);
+    P.build_index(hmn, hmx, eps);
+    const uint32_t K1 = k + 1;
+    std::vector<uint8_t> act(n, 0);
+    for (uint32_t i = 0; i < nq; i++) {
+      if (queries[i] >= n) throw Error(KNN_ERR_ARG, "query id out of range");
+      act[queries[i]] = 1;
+    }
+    uint8_t *dAct = P.buf<uint8_t>(S_HOSTFLAGS, n);
+    uint8_t *dFailed = P.buf<uint8_t>(S_FAILED, n);
+    int32_t *dIds = P.buf<int32_t>(S_OUTIDS, (size_t)n * K1);
+    float *dD = P.buf<float>(S_OUTDISTS, (size_t)n * K1);
+    double *rowSum = P.buf<double>(S_ROWSUM, n);
+    HIP_CHECK(hipMemcpyAsync(dAct, act.data(), n, hipMemcpyHostToDevice, P.s));
+    HIP_CHECK(hipMemcpyAsync(dFailed, failed, n, hipMemcpyHostToDevice, P.s));
+    HIP_CHECK(hipMemcpyAsync(dIds, out_ids, sizeof(int32_t) * (size_t)n * K1, hipMemcpyHostToDevice, P.s));
+    HIP_CHECK(hipMemcpyAsync(dD, out_dists, sizeof(float) * (size_t)n * K1, hipMemcpyHostToDevice, P.s));
+    unsigned long long *ctr = P.buf<unsigned long long>(S_CTR, 64);
+    fill_u32(reinterpret_cast<uint32_t *>(ctr), 0u, 16, P.s);
+    Timer tK(P.s);
+    PassResult pr = P.pass(dAct, dFailed, dIds, dD, rowSum, ctr, reinterpret_cast<uint32_t *>(ctr + 4), tK);
+    HIP_CHECK(hipMemcpyAsync(out_ids, dIds, sizeof(int32_t) * (size_t)n * K1, hipMemcpyDeviceToHost, P.s));
+    HIP_CHECK(hipMemcpyAsync(out_dists, dD, sizeof(float) * (size_t)n * K1, hipMemcpyDeviceToHost, P.s));
+    HIP_CHECK(hipMemcpyAsync(failed, dFailed, n, hipMemcpyDeviceToHost, P.s));
+    HIP_CHECK(hipStreamSynchronize(P.s));
+    if (num_failed) *num_failed = pr.nfail;
+    return KNN_OK;
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,180 @@
+// estimator.hip -- dimension reorder by variance and the BETA-driven epsilon estimate.
+//
+//  * variance_rows: main.cu:2521-2615 in T arithmetic with sequential sums over rows
+//    0,100,200,...  The order of those float additions decides the variances' last bits, so
+//    each dim is summed by one thread, in row order.
+//  * avg_dist_soa: kernel.cu:343-381.  The reference adds every sampled distance to one
+//    double with atomicAdd (order not reproducible); here each thread sums its own pairs
+//    in order and the per-thread sums are added in thread order by one lane.
+//  * kdist_hist_soa: kernel.cu:450-523 with a per-block LDS histogram (10000 x u32) merged
+//    into the global one by integer atomics (order-free).  Points past N are skipped where
+//    the reference reads beyond the array (kernel.cu:486-492); bucket == 10000 is dropped
+//    where the reference writes one past the histogram.
+#include "knn_launch.h"
+
+namespace knn {
+
+namespace {
+
+template <typename T>
+__global__ void k_variance(const T *__restrict__ pts, uint32_t n, uint32_t d, T *var) {
+  const uint32_t i = threadIdx.x;
+  if (i >= d) return;
+  const T inv = (T)(1.0 / (100 * 1.0));
+  const T denom = (T)n * inv;
+  T sum = 0;
+#pragma unroll 16
+  for (uint64_t j = 0; j < n; j += 100) sum += pts[j * d + i];
+  const T avg = sum / denom;
+  sum = 0;
+#pragma unroll 16
+  for (uint64_t j = 0; j < n; j += 100) {
+    T df = pts[j * d + i] - avg;
+    T sq = df * df;
+    sum += sq;
+  }
+  var[i] = sum / denom;
+}
+
+struct Perm {
+  uint32_t p[KNN_MAXDIM];
+};
+
+template <typename T>
+__global__ void k_permute_to_soa(const T *__restrict__ pts, uint32_t n, uint32_t d, Perm perm,
+                                 T *__restrict__ out) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const T *row = pts + (size_t)i * d;
+  for (uint32_t j = 0; j < d; j++) out[(size_t)j * n + i] = row[perm.p[j]];
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_avg_dist(const T *__restrict__ x, uint32_t n, uint32_t d,
+                                                  double *partials, uint64_t *counts) {
+  const uint32_t g = blockIdx.x * 256 + threadIdx.x;
+  const uint64_t tid = (uint64_t)g * 1000;
+  double part = 0;
+  uint64_t c = 0;
+  if (tid < n) {
+    T q[KNN_MAXDIM];
+#pragma unroll
+    for (uint32_t j = 0; j < KNN_MAXDIM; j++) q[j] = j < d ? x[(size_t)j * n + tid] : (T)0;
+    for (uint64_t i = threadIdx.x; i < n; i += 1000) {
+      double run = 0;
+#pragma unroll
+      for (uint32_t j = 0; j < KNN_MAXDIM; j++) {
+        if (j < d) {
+          T df = x[(size_t)j * n + i] - q[j];
+          T sq = df * df;
+          run += (double)sq;
+        }
+      }
+      part += sqrt(run);
+      c++;
+    }
+  }
+  partials[g] = part;
+  counts[g] = c;
+}
+
+__global__ void k_sum_partials(const double *partials, const uint64_t *counts, uint32_t m,
+                               double *total, uint64_t *cnt) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double t = 0;
+  uint64_t c = 0;
+  for (uint32_t g = 0; g < m; g++) {
+    t += partials[g];
+    c += counts[g];
+  }
+  *total = t;
+  *cnt = c;
+}
+
+constexpr uint32_t NBUCKETS = 10000;
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_kdist(const T *__restrict__ x, uint32_t n, uint32_t d,
+                                               uint32_t offset, double avg, double bw,
+                                               uint32_t *hist) {
+  __shared__ uint32_t lh[NBUCKETS];
+  for (uint32_t b = threadIdx.x; b < NBUCKETS; b += 256) lh[b] = 0;
+  __syncthreads();
+  const uint64_t tid = (uint64_t)blockIdx.x * offset;
+  if (tid < n) {
+    T q[KNN_MAXDIM];
+#pragma unroll
+    for (uint32_t j = 0; j < KNN_MAXDIM; j++) q[j] = j < d ? x[(size_t)j * n + tid] : (T)0;
+    for (uint64_t i = 0; i < n; i += 256 * 8) {
+      const uint64_t p = i + threadIdx.x;
+      if (p >= n) break;
+      double run = 0;
+#pragma unroll
+      for (uint32_t j = 0; j < KNN_MAXDIM; j++) {
+        if (j < d) {
+          T df = x[(size_t)j * n + p] - q[j];
+          T sq = df * df;
+          run += (double)sq;
+        }
+      }
+      run = sqrt(run);
+      if (run < avg && p != tid) {
+        uint32_t bucket = (uint32_t)(run / bw);
+        if (bucket < NBUCKETS) atomicAdd(&lh[bucket], 8u);
+      }
+    }
+  }
+  __syncthreads();
+  for (uint32_t b = threadIdx.x; b < NBUCKETS; b += 256)
+    if (lh[b]) atomicAdd(&hist[b], lh[b]);
+}
+
+}  // namespace
+
+template <typename T>
+void variance_rows(const T *pts, uint32_t n, uint32_t d, T *var, hipStream_t s) {
+  k_variance<T><<<1, 64, 0, s>>>(pts, n, d, var);
+}
+
+template <typename T>
+void permute_to_soa(const T *pts, uint32_t n, uint32_t d, const uint32_t *perm_host, T *out,
+                    hipStream_t s) {
+  Perm p;
+  for (uint32_t j = 0; j < KNN_MAXDIM; j++) p.p[j] = j < d ? perm_host[j] : 0;
+  if (n) k_permute_to_soa<T><<<(n + 255) / 256, 256, 0, s>>>(pts, n, d, p, out);
+}
+
+size_t avgdist_threads(uint32_t n) {
+  int total_blocks = (int)ceil((0.001 * n) / (1.0 * 256));  // GPU.cu:3276
+  if (total_blocks < 1) total_blocks = 1;
+  return (size_t)total_blocks * 256;
+}
+
+template <typename T>
+void avg_dist_soa(const T *x, uint32_t n, uint32_t d, double *partials, uint64_t *counts,
+                  double *total, uint64_t *cnt, hipStream_t s) {
+  const uint32_t m = (uint32_t)avgdist_threads(n);
+  k_avg_dist<T><<<m / 256, 256, 0, s>>>(x, n, d, partials, counts);
+  k_sum_partials<<<1, 64, 0, s>>>(partials, counts, m, total, cnt);
+}
+
+template <typename T>
+void kdist_hist_soa(const T *x, uint32_t n, uint32_t d, uint32_t offset, uint32_t nsample,
+                    double avg, double bw, uint32_t *hist, hipStream_t s) {
+  fill_u32(hist, 0u, NBUCKETS, s);
+  if (nsample) k_kdist<T><<<nsample, 256, 0, s>>>(x, n, d, offset, avg, bw, hist);
+}
+
+#define INST(T)                                                                            \
+  template void variance_rows<T>(const T *, uint32_t, uint32_t, T *, hipStream_t);          \
+  template void permute_to_soa<T>(const T *, uint32_t, uint32_t, const uint32_t *, T *,     \
+                                  hipStream_t);                                             \
+  template void avg_dist_soa<T>(const T *, uint32_t, uint32_t, double *, uint64_t *,        \
+                                double *, uint64_t *, hipStream_t);                         \
+  template void kdist_hist_soa<T>(const T *, uint32_t, uint32_t, uint32_t, uint32_t, double, \
+                                  double, uint32_t *, hipStream_t);
+INST(float)
+INST(double)
+#undef INST
+
+}  // namespace knn

@@ -1,0 +1,107 @@
+// knn_device.h -- shared device helpers of the gfx950 KNN self-join.
+//
+// Numerics (all device code is built with -ffp-contract=off and correctly rounded FP32
+// divide/sqrt, so each expression below is one IEEE operation):
+//   cell coordinate  (unsigned)((x - min_j) / eps)                    kernel.cu:1685-1687
+//   linear cell id   sum_j c_j * prod_{i<j} nCells_i, dim 0 fastest   kernel.cu:31-47
+//   FP64 key         sum_j ((double)c_j - (double)q_j)^2              ranks neighbours
+//   FP32 distance    sqrtf(sum_j (c_j - q_j)^2)                       kernel.cu:127-144
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define KNN_MAXDIM 16
+#define KNN_MAX_INDEXED 8
+#define KNN_WAVE 64
+
+namespace knn {
+
+struct GridParams {
+  float minArr[KNN_MAX_INDEXED];
+  uint32_t nCells[KNN_MAX_INDEXED];
+  uint64_t mult[KNN_MAX_INDEXED];  // prod_{i<j} nCells_i
+  float eps;
+  uint32_t nd;                      // indexed dims
+};
+
+struct GridParamsD {
+  double minArr[KNN_MAX_INDEXED];
+  uint32_t nCells[KNN_MAX_INDEXED];
+  uint64_t mult[KNN_MAX_INDEXED];
+  double eps;
+  uint32_t nd;
+};
+
+template <typename T> struct GridOf;
+template <> struct GridOf<float> { using type = GridParams; };
+template <> struct GridOf<double> { using type = GridParamsD; };
+
+// kernel.cu:1685-1687 with clamping to [0, ncell-1]; the clamp never binds for data points
+// (min = datamin - eps, nCells = ceil((max-min)/eps)), only for query range bounds.
+template <typename T>
+__device__ __forceinline__ uint32_t cell_coord(T x, T mn, T eps, uint32_t ncell) {
+  T t = (x - mn) / eps;
+  if (!(t > (T)0)) return 0u;
+  if (t >= (T)4294967295.0) return ncell - 1;
+  uint32_t c = (uint32_t)t;
+  return c >= ncell ? ncell - 1 : c;
+}
+
+// IEEE successor / predecessor of a finite value (nextafter toward +-inf); used to round the
+// query's [x-eps, x+eps] outward before mapping it to cell coordinates.
+__device__ __forceinline__ float next_up(float x) {
+  if (x != x || x == __builtin_inff()) return x;
+  if (x == 0.0f) return __int_as_float(1);
+  int b = __float_as_int(x);
+  return __int_as_float(x > 0.0f ? b + 1 : b - 1);
+}
+__device__ __forceinline__ float next_down(float x) { return -next_up(-x); }
+__device__ __forceinline__ double next_up(double x) {
+  if (x != x || x == __builtin_inf()) return x;
+  if (x == 0.0) return __longlong_as_double(1ll);
+  long long b = __double_as_longlong(x);
+  return __longlong_as_double(x > 0.0 ? b + 1 : b - 1);
+}
+__device__ __forceinline__ double next_down(double x) { return -next_up(-x); }
+
+__device__ __forceinline__ uint64_t lanemask_lt() {
+  uint32_t lane = __lane_id();
+  return lane == 0 ? 0ull : (~0ull >> (64 - lane));
+}
+
+template <typename T>
+__device__ __forceinline__ T readlane(T v, int lane);
+
+template <>
+__device__ __forceinline__ float readlane<float>(float v, int lane) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+template <>
+__device__ __forceinline__ uint32_t readlane<uint32_t>(uint32_t v, int lane) {
+  return (uint32_t)__builtin_amdgcn_readlane((int)v, lane);
+}
+template <>
+__device__ __forceinline__ double readlane<double>(double v, int lane) {
+  uint64_t b = (uint64_t)__double_as_longlong(v);
+  uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)b, lane);
+  uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(b >> 32), lane);
+  return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+
+__device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) {
+  for (int o = 32; o > 0; o >>= 1) {
+    uint32_t w = __shfl_xor(v, o, 64);
+    v = w < v ? w : v;
+  }
+  return v;
+}
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
+  for (int o = 32; o > 0; o >>= 1) {
+    uint32_t w = __shfl_xor(v, o, 64);
+    v = w > v ? w : v;
+  }
+  return v;
+}
+
+}  // namespace knn

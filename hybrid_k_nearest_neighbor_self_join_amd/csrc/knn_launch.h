@@ -1,0 +1,118 @@
+// knn_launch.h -- host-side launchers of the device kernels (all asynchronous on `s`).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "knn_device.h"
+
+namespace knn {
+
+// ---------------------------------------------------------------- primitives (prims.hip)
+// Exclusive scan of n u32 values into out[0..n); out[n] = total.  `scratch` must hold
+// scan_scratch_words(n) u32 words.  in may alias out.
+size_t scan_scratch_words(uint32_t n);
+void scan_exclusive_u32(const uint32_t *in, uint32_t *out, uint32_t n, uint32_t *scratch,
+                        hipStream_t s);
+
+// Stable LSD radix sort of (key, value) pairs on the low `bits` bits of the key.  Result
+// ends in (keys, vals); (keys_tmp, vals_tmp) are ping-pong buffers of n elements; `scratch`
+// must hold radix_scratch_words(n) u32 words.
+size_t radix_scratch_words(uint32_t n);
+void radix_sort_pairs_u32(uint32_t *keys, uint32_t *vals, uint32_t *keys_tmp, uint32_t *vals_tmp,
+                          uint32_t n, int bits, uint32_t *scratch, hipStream_t s);
+void radix_sort_pairs_u64(uint64_t *keys, uint32_t *vals, uint64_t *keys_tmp, uint32_t *vals_tmp,
+                          uint32_t n, int bits, uint32_t *scratch, hipStream_t s);
+
+// Per-dimension min/max of the first nd rows of a dim-major (SoA) array x[j*n + i].
+// scratch: minmax_scratch_elems(n, nd) elements of T.  Results (device) in mn[nd], mx[nd].
+size_t minmax_scratch_elems(uint32_t n, uint32_t nd);
+template <typename T>
+void minmax_soa(const T *x, uint32_t n, uint32_t nd, T *mn, T *mx, T *scratch, hipStream_t s);
+
+void iota_u32(uint32_t *a, uint32_t n, hipStream_t s);
+void fill_u8(uint8_t *a, uint8_t v, size_t n, hipStream_t s);
+void fill_u32(uint32_t *a, uint32_t v, size_t n, hipStream_t s);
+
+// --------------------------------------------------------------- estimator (estimator.hip)
+// main.cu:2521-2615: exact sequential variance (T arithmetic) per dim over rows 0,100,...
+// of the row-major input.  var[d] (device).
+template <typename T>
+void variance_rows(const T *pts, uint32_t n, uint32_t d, T *var, hipStream_t s);
+// out[j*n + i] = pts[i*d + perm[j]]  (row-major -> reordered dim-major)
+template <typename T>
+void permute_to_soa(const T *pts, uint32_t n, uint32_t d, const uint32_t *perm_host, T *out,
+                    hipStream_t s);
+// kernel.cu:343-381: per-thread double partial sums (partials[g], counts[g]) then a fixed-
+// order total in sums[0] (double) and counts total in cnt[0].
+size_t avgdist_threads(uint32_t n);
+template <typename T>
+void avg_dist_soa(const T *x, uint32_t n, uint32_t d, double *partials, uint64_t *counts,
+                  double *total, uint64_t *cnt, hipStream_t s);
+// kernel.cu:450-523: k-distance histogram (10000 buckets, +8 per hit) of nsample sample
+// points (stride offset) against every 8th 256-point block.  avg/bw are host values.
+template <typename T>
+void kdist_hist_soa(const T *x, uint32_t n, uint32_t d, uint32_t offset, uint32_t nsample,
+                    double avg, double bw, uint32_t *hist, hipStream_t s);
+
+// ------------------------------------------------------------------- index (join.hip)
+template <typename T>
+void cell_keys_u32(const T *x, uint32_t n, const typename GridOf<T>::type &g, uint32_t *keys,
+                   uint32_t *ids, hipStream_t s);
+template <typename T>
+void cell_keys_u64(const T *x, uint32_t n, const typename GridOf<T>::type &g, uint64_t *keys,
+                   uint32_t *ids, hipStream_t s);
+// flag[p] = (p == 0 || key[p] != key[p-1])
+void boundary_flags_u32(const uint32_t *keys, uint32_t n, uint32_t *flags, hipStream_t s);
+void boundary_flags_u64(const uint64_t *keys, uint32_t n, uint32_t *flags, hipStream_t s);
+// cellKeys[pos[p]] = key[p], cellStart[pos[p]] = p for boundary p; cellStart[C] = n.
+// Keys are widened to u64 in cellKeys.
+void compact_cells_u32(const uint32_t *keys, const uint32_t *flags, const uint32_t *pos,
+                       uint32_t n, uint64_t *cellKeys, uint32_t *cellStart, hipStream_t s);
+void compact_cells_u64(const uint64_t *keys, const uint32_t *flags, const uint32_t *pos,
+                       uint32_t n, uint64_t *cellKeys, uint32_t *cellStart, hipStream_t s);
+// xs[j*n + p] = x[j*n + ids[p]]
+template <typename T>
+void gather_soa(const T *x, const uint32_t *ids, uint32_t n, uint32_t d, T *xs, hipStream_t s);
+
+// active-query flags per sorted position (all = 1 when failed == NULL)
+void active_flags(const uint32_t *sortedIds, const uint8_t *failed, uint32_t n, uint32_t *flags,
+                  hipStream_t s);
+void compact_active(const uint32_t *sortedIds, const uint32_t *flags, const uint32_t *pos,
+                    uint32_t n, uint32_t *act, hipStream_t s);
+// per cell: tiles of up to 64 active queries (pos is the exclusive scan of active flags,
+// with pos[n] = total)
+void cell_tile_counts(const uint32_t *cellStart, const uint32_t *pos, uint32_t ncell,
+                      uint32_t *ntiles, hipStream_t s);
+void write_tiles(const uint32_t *cellStart, const uint32_t *pos, const uint32_t *tileOff,
+                 uint32_t ncell, uint4 *tiles, hipStream_t s);
+
+struct JoinArgs {
+  const void *x;         // reordered dim-major points x[j*n + i]
+  const void *xs;        // cell-sorted copy xs[j*n + p]
+  const uint32_t *sortedIds;
+  const uint64_t *cellKeys;
+  const uint32_t *cellStart;
+  uint32_t ncell;
+  const uint32_t *act;   // active query ids, grouped by cell
+  const uint4 *tiles;    // {cell, first active position, count, 0}
+  uint32_t ntiles;
+  uint32_t n, d, k1;
+  int32_t *outIds;
+  void *outDists;
+  double *rowSum;        // per-row sum of (T)(dist*dist), for the checksum
+  uint8_t *failed;       // per point: 1 = failed this pass
+  uint32_t *failCount;   // device counter (added to)
+  unsigned long long *candCount;  // device counter of (query, candidate) pairs (added to)
+  void *heapScratch;     // global heap storage when K+1 does not fit in LDS
+};
+template <typename T>
+void range_topk(const JoinArgs &a, const typename GridOf<T>::type &g, hipStream_t s);
+// bytes of heap scratch needed in global memory for `ntiles` tiles (0 if LDS is used)
+template <typename T>
+size_t range_heap_scratch_bytes(uint32_t ntiles, uint32_t k1);
+
+// deterministic sum of rowSum over the rows (fixed tree): result in out[0]
+void sum_rows(const double *rowSum, uint32_t n, double *partial, double *out, hipStream_t s);
+
+}  // namespace knn

@@ -1,0 +1,331 @@
+// prims.hip -- scan, stable LSD radix sort, min/max reductions and fills for gfx950.
+//
+// These replace the Thrust calls of the reference's index build (thrust::sort + unique
+// GPU.cu:3884-3886, thrust::sort_by_key GPU.cu:3959).  Wave64 throughout: a block is four
+// wavefronts; per-wave work uses 64-bit ballots and shuffles.
+#include "knn_launch.h"
+
+namespace knn {
+
+namespace {
+
+constexpr int SCAN_BLOCK = 256;
+constexpr int SCAN_ITEMS = 8;
+constexpr uint32_t SCAN_TILE = SCAN_BLOCK * SCAN_ITEMS;  // 2048
+
+// inclusive scan of v over the 256 threads of a block; returns the exclusive prefix and
+// sets *total.  Uses __shfl_up inside a wave and 4 LDS slots across waves.
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t *total) {
+  __shared__ uint32_t s_w[SCAN_BLOCK / 64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  uint32_t inc = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    uint32_t u = __shfl_up(inc, o, 64);
+    if (lane >= o) inc += u;
+  }
+  if (lane == 63) s_w[w] = inc;
+  __syncthreads();
+  uint32_t woff = 0, tot = 0;
+#pragma unroll
+  for (int i = 0; i < SCAN_BLOCK / 64; i++) {
+    uint32_t x = s_w[i];
+    woff += (i < w) ? x : 0u;
+    tot += x;
+  }
+  __syncthreads();
+  *total = tot;
+  return woff + inc - v;
+}
+
+__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_reduce(const uint32_t *in, uint32_t n,
+                                                             uint32_t *sums) {
+  const uint32_t base = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
+  uint32_t acc = 0;
+#pragma unroll
+  for (int i = 0; i < SCAN_ITEMS; i++) {
+    uint32_t idx = base + i;
+    acc += idx < n ? in[idx] : 0u;
+  }
+  uint32_t tot;
+  (void)block_exclusive_scan(acc, &tot);
+  if (threadIdx.x == 0) sums[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_down(const uint32_t *in, uint32_t *out,
+                                                           uint32_t n, const uint32_t *offs) {
+  const uint32_t base = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
+  uint32_t v[SCAN_ITEMS];
+  uint32_t acc = 0;
+#pragma unroll
+  for (int i = 0; i < SCAN_ITEMS; i++) {
+    uint32_t idx = base + i;
+    v[i] = idx < n ? in[idx] : 0u;
+    acc += v[i];
+  }
+  uint32_t tot;
+  uint32_t ex = block_exclusive_scan(acc, &tot);
+  uint32_t off = (offs ? offs[blockIdx.x] : 0u) + ex;
+#pragma unroll
+  for (int i = 0; i < SCAN_ITEMS; i++) {
+    uint32_t idx = base + i;
+    if (idx < n) out[idx] = off;
+    off += v[i];
+  }
+  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == SCAN_BLOCK - 1) out[n] = off;
+}
+
+}  // namespace
+
+size_t scan_scratch_words(uint32_t n) {
+  size_t words = 0;
+  uint32_t m = n;
+  while (m > SCAN_TILE) {
+    uint32_t nb = (m + SCAN_TILE - 1) / SCAN_TILE;
+    words += (size_t)nb + 1;
+    m = nb;
+  }
+  return words + 16;
+}
+
+void scan_exclusive_u32(const uint32_t *in, uint32_t *out, uint32_t n, uint32_t *scratch,
+                        hipStream_t s) {
+  if (n == 0) {
+    fill_u32(out, 0u, 1, s);
+    return;
+  }
+  uint32_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;
+  if (nb == 1) {
+    k_scan_down<<<1, SCAN_BLOCK, 0, s>>>(in, out, n, nullptr);
+    return;
+  }
+  uint32_t *sums = scratch;  // nb + 1 words
+  k_scan_reduce<<<nb, SCAN_BLOCK, 0, s>>>(in, n, sums);
+  scan_exclusive_u32(sums, sums, nb, scratch + nb + 1, s);
+  k_scan_down<<<nb, SCAN_BLOCK, 0, s>>>(in, out, n, sums);
+}
+
+// ------------------------------------------------------------------------- radix sort
+namespace {
+
+constexpr int RADIX_BLOCK = 256;
+constexpr int RADIX_ROUNDS = 16;
+constexpr uint32_t RADIX_TILE = RADIX_BLOCK * RADIX_ROUNDS;  // 4096 keys per block
+
+template <typename K>
+__global__ __launch_bounds__(RADIX_BLOCK) void k_radix_hist(const K *keys, uint32_t n,
+                                                             uint32_t ntiles, int shift,
+                                                             uint32_t *counts) {
+  __shared__ uint32_t h[256];
+  const int t = threadIdx.x;
+  h[t] = 0;
+  __syncthreads();
+  const uint32_t base = blockIdx.x * RADIX_TILE;
+#pragma unroll 4
+  for (int r = 0; r < RADIX_ROUNDS; r++) {
+    uint32_t idx = base + r * RADIX_BLOCK + t;
+    if (idx < n) atomicAdd(&h[(uint32_t)(keys[idx] >> shift) & 255u], 1u);
+  }
+  __syncthreads();
+  counts[(size_t)t * ntiles + blockIdx.x] = h[t];
+}
+
+// Stable scatter: round r of a tile covers keys base + r*256 + [0,256), thread t takes key
+// t of the round, so tile order = key order.  Inside a round each wave ranks its lanes by
+// the 8-bit match mask (8 ballots); earlier waves' counts come from LDS.
+template <typename K>
+__global__ __launch_bounds__(RADIX_BLOCK) void k_radix_scatter(
+    const K *__restrict__ kin, const uint32_t *__restrict__ vin, K *__restrict__ kout,
+    uint32_t *__restrict__ vout, const uint32_t *__restrict__ offsets, uint32_t n,
+    uint32_t ntiles, int shift) {
+  __shared__ uint32_t s_off[256];
+  __shared__ uint32_t s_wc[4][256];
+  const int t = threadIdx.x, w = t >> 6;
+  s_off[t] = offsets[(size_t)t * ntiles + blockIdx.x];
+  s_wc[0][t] = 0;
+  s_wc[1][t] = 0;
+  s_wc[2][t] = 0;
+  s_wc[3][t] = 0;
+  __syncthreads();
+  const uint32_t base = blockIdx.x * RADIX_TILE;
+  const uint64_t lt = lanemask_lt();
+  for (int r = 0; r < RADIX_ROUNDS; r++) {
+    const uint32_t rb = base + r * RADIX_BLOCK;
+    if (rb >= n) break;  // uniform across the block
+    const uint32_t idx = rb + t;
+    const bool valid = idx < n;
+    K key = valid ? kin[idx] : (K)0;
+    uint32_t val = valid ? vin[idx] : 0u;
+    uint32_t digit = (uint32_t)(key >> shift) & 255u;
+    uint64_t m = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < 8; b++) {
+      bool bit = (digit >> b) & 1u;
+      uint64_t bb = __ballot(bit);
+      m &= bit ? bb : ~bb;
+    }
+    uint32_t rank = __popcll(m & lt);
+    if (valid && rank == 0) s_wc[w][digit] = __popcll(m);
+    __syncthreads();
+    if (valid) {
+      uint32_t off = s_off[digit] + rank;
+      for (int ww = 0; ww < w; ww++) off += s_wc[ww][digit];
+      kout[off] = key;
+      vout[off] = val;
+    }
+    __syncthreads();
+    s_off[t] += s_wc[0][t] + s_wc[1][t] + s_wc[2][t] + s_wc[3][t];
+    s_wc[0][t] = 0;
+    s_wc[1][t] = 0;
+    s_wc[2][t] = 0;
+    s_wc[3][t] = 0;
+    __syncthreads();
+  }
+}
+
+template <typename K>
+void radix_sort_impl(K *keys, uint32_t *vals, K *keys_tmp, uint32_t *vals_tmp, uint32_t n,
+                     int bits, uint32_t *scratch, hipStream_t s) {
+  if (n <= 1 || bits <= 0) return;
+  const uint32_t ntiles = (n + RADIX_TILE - 1) / RADIX_TILE;
+  const uint32_t ncounts = 256u * ntiles;
+  uint32_t *counts = scratch;                   // ncounts + 1
+  uint32_t *scan_scr = scratch + ncounts + 1;   // scan scratch
+  K *ka = keys, *kb = keys_tmp;
+  uint32_t *va = vals, *vb = vals_tmp;
+  const int passes = (bits + 7) / 8;
+  for (int p = 0; p < passes; p++) {
+    const int shift = 8 * p;
+    k_radix_hist<K><<<ntiles, RADIX_BLOCK, 0, s>>>(ka, n, ntiles, shift, counts);
+    scan_exclusive_u32(counts, counts, ncounts, scan_scr, s);
+    k_radix_scatter<K><<<ntiles, RADIX_BLOCK, 0, s>>>(ka, va, kb, vb, counts, n, ntiles, shift);
+    K *tk = ka;
+    ka = kb;
+    kb = tk;
+    uint32_t *tv = va;
+    va = vb;
+    vb = tv;
+  }
+  if (ka != keys) {
+    hipMemcpyAsync(keys, ka, sizeof(K) * (size_t)n, hipMemcpyDeviceToDevice, s);
+    hipMemcpyAsync(vals, va, sizeof(uint32_t) * (size_t)n, hipMemcpyDeviceToDevice, s);
+  }
+}
+
+}  // namespace
+
+size_t radix_scratch_words(uint32_t n) {
+  const uint32_t ntiles = (n + RADIX_TILE - 1) / RADIX_TILE;
+  const uint32_t ncounts = 256u * (ntiles ? ntiles : 1);
+  return (size_t)ncounts + 1 + scan_scratch_words(ncounts);
+}
+
+void radix_sort_pairs_u32(uint32_t *keys, uint32_t *vals, uint32_t *keys_tmp, uint32_t *vals_tmp,
+                          uint32_t n, int bits, uint32_t *scratch, hipStream_t s) {
+  radix_sort_impl<uint32_t>(keys, vals, keys_tmp, vals_tmp, n, bits, scratch, s);
+}
+
+void radix_sort_pairs_u64(uint64_t *keys, uint32_t *vals, uint64_t *keys_tmp, uint32_t *vals_tmp,
+                          uint32_t n, int bits, uint32_t *scratch, hipStream_t s) {
+  radix_sort_impl<uint64_t>(keys, vals, keys_tmp, vals_tmp, n, bits, scratch, s);
+}
+
+// ------------------------------------------------------------------------- min / max
+namespace {
+constexpr int MM_BLOCK = 256;
+constexpr uint32_t MM_MAXBLOCKS = 1024;
+
+template <typename T>
+__device__ __forceinline__ T tmin(T a, T b) { return b < a ? b : a; }
+template <typename T>
+__device__ __forceinline__ T tmax(T a, T b) { return b > a ? b : a; }
+
+// block b reduces rows [b*chunk, (b+1)*chunk) of dim blockIdx.y
+template <typename T>
+__global__ __launch_bounds__(MM_BLOCK) void k_minmax_part(const T *x, uint32_t n, uint32_t chunk,
+                                                          T *pmin, T *pmax) {
+  __shared__ T smin[MM_BLOCK], smax[MM_BLOCK];
+  const uint32_t j = blockIdx.y;
+  const T *xj = x + (size_t)j * n;
+  uint32_t lo = blockIdx.x * chunk, hi = lo + chunk < n ? lo + chunk : n;
+  T mn = xj[lo < n ? lo : 0], mx = mn;
+  for (uint32_t i = lo + threadIdx.x; i < hi; i += MM_BLOCK) {
+    T v = xj[i];
+    mn = tmin(mn, v);
+    mx = tmax(mx, v);
+  }
+  smin[threadIdx.x] = mn;
+  smax[threadIdx.x] = mx;
+  __syncthreads();
+  for (int o = MM_BLOCK / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) {
+      smin[threadIdx.x] = tmin(smin[threadIdx.x], smin[threadIdx.x + o]);
+      smax[threadIdx.x] = tmax(smax[threadIdx.x], smax[threadIdx.x + o]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    pmin[(size_t)j * gridDim.x + blockIdx.x] = smin[0];
+    pmax[(size_t)j * gridDim.x + blockIdx.x] = smax[0];
+  }
+}
+
+template <typename T>
+__global__ void k_minmax_final(const T *pmin, const T *pmax, uint32_t nb, uint32_t nd, T *mn,
+                               T *mx) {
+  const uint32_t j = threadIdx.x;
+  if (j >= nd) return;
+  T a = pmin[(size_t)j * nb], b = pmax[(size_t)j * nb];
+  for (uint32_t i = 1; i < nb; i++) {
+    a = tmin(a, pmin[(size_t)j * nb + i]);
+    b = tmax(b, pmax[(size_t)j * nb + i]);
+  }
+  mn[j] = a;
+  mx[j] = b;
+}
+}  // namespace
+
+size_t minmax_scratch_elems(uint32_t n, uint32_t nd) { return 2ull * MM_MAXBLOCKS * nd; }
+
+template <typename T>
+void minmax_soa(const T *x, uint32_t n, uint32_t nd, T *mn, T *mx, T *scratch, hipStream_t s) {
+  uint32_t nb = (n + 8191) / 8192;
+  if (nb > MM_MAXBLOCKS) nb = MM_MAXBLOCKS;
+  if (nb == 0) nb = 1;
+  uint32_t chunk = (n + nb - 1) / nb;
+  T *pmin = scratch, *pmax = scratch + (size_t)MM_MAXBLOCKS * nd;
+  k_minmax_part<T><<<dim3(nb, nd), MM_BLOCK, 0, s>>>(x, n, chunk, pmin, pmax);
+  k_minmax_final<T><<<1, 64, 0, s>>>(pmin, pmax, nb, nd, mn, mx);
+}
+template void minmax_soa<float>(const float *, uint32_t, uint32_t, float *, float *, float *,
+                                hipStream_t);
+template void minmax_soa<double>(const double *, uint32_t, uint32_t, double *, double *, double *,
+                                 hipStream_t);
+
+// ------------------------------------------------------------------------- fills
+namespace {
+__global__ void k_iota(uint32_t *a, uint32_t n) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = i;
+}
+__global__ void k_fill_u8(uint8_t *a, uint8_t v, size_t n) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = v;
+}
+__global__ void k_fill_u32(uint32_t *a, uint32_t v, size_t n) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = v;
+}
+}  // namespace
+
+void iota_u32(uint32_t *a, uint32_t n, hipStream_t s) {
+  if (n) k_iota<<<(n + 255) / 256, 256, 0, s>>>(a, n);
+}
+void fill_u8(uint8_t *a, uint8_t v, size_t n, hipStream_t s) {
+  if (n) k_fill_u8<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(a, v, n);
+}
+void fill_u32(uint32_t *a, uint32_t v, size_t n, hipStream_t s) {
+  if (n) k_fill_u32<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(a, v, n);
+}
+
+}  // namespace knn

@@ -1,0 +1,147 @@
+/*
+ * gpuknnselfjoin.h -- C ABI of libgpuknnselfjoin.so, an exact K-nearest-neighbour
+ * self-join for one AMD MI355X (gfx950), written in HIP.
+ *
+ * Contract of every join entry point (same as the reference's KNNJoinPy):
+ *   input   row-major N x NDIM points (never modified);
+ *   output  row i = point i's K+1 nearest points, self included, ascending by distance;
+ *           ids int32, Euclidean distances (not squared) in the input's type.
+ * Ordering: neighbours are ranked by the FP64 squared distance
+ *   key = sum_j ((double)c_j - (double)q_j)^2  (no FMA, dims in descending-variance order)
+ * and ties by point id, so rows equal an FP64 brute-force KNN.  Reported float distances are
+ *   sqrtf(sum_j (c_j - q_j)^2) in FP32, same dim order, no FMA contraction.
+ * Calls are serialised internally (one call at a time per process); buffers reused across
+ * calls stay allocated until knnjoin_release() or process exit.
+ */
+#ifndef GPUKNNSELFJOIN_H
+#define GPUKNNSELFJOIN_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes (the reference returns void and prints; see knnjoin_last_status) ---- */
+enum {
+  KNN_OK = 0,
+  KNN_ERR_ARG = 1,         /* bad pointer / size / dims (NDIM != expected, NUMINDEXEDDIM > NDIM) */
+  KNN_ERR_TOO_FEW = 2,     /* N < K+1: no row can be filled (the reference never terminates) */
+  KNN_ERR_EPS_ZERO = 3,    /* epsilon estimate or growth gives 0 (main.cu:334-338) */
+  KNN_ERR_NO_PROGRESS = 4, /* eps + eps0/2 == eps in floating point: the loop cannot advance */
+  KNN_ERR_HIP = 5,         /* a HIP runtime call failed (message in knnjoin_last_error) */
+  KNN_ERR_NOMEM = 6,       /* device allocation failed */
+  KNN_ERR_NO_DEVICE = 7,   /* no HIP device visible */
+  KNN_ERR_MODEL = 8        /* learned-radius model missing or of the wrong shape */
+};
+
+/* dtype tags */
+enum { KNN_FLOAT32 = 0, KNN_FLOAT64 = 1 };
+
+/* radius schedule */
+enum {
+  KNN_MODE_BETA = 0,    /* plain method: BETA-driven eps0 + failed-query loop (main.cu:305-430) */
+  KNN_MODE_LEARNED = 1  /* learned per-query initial radius, same loop as fallback */
+};
+
+typedef struct {
+  uint32_t num_indexed_dims; /* NUMINDEXEDDIM (params.h:9); 0 = min(NDIM, 6) */
+  double beta;               /* BETA (params.h:58); < 0 = 0.05, the reference's default */
+  int32_t mode;              /* KNN_MODE_* */
+  int32_t reorder;           /* REORDER (params.h:35); 1 = reorder dims by variance */
+  int32_t verbose;           /* 1 = print phase timings to stdout */
+  int32_t device_ptrs;       /* 1 = points/outputs are device pointers (HBM-resident) */
+  void *stream;              /* hipStream_t for device_ptrs mode; NULL = library stream */
+} knnjoin_params;
+
+typedef struct {
+  uint32_t perm[16];        /* dim order used (column j = input column perm[j]) */
+  float eps0;               /* initial epsilon (BETA estimate; 0 in learned mode) */
+  float eps_final;          /* epsilon of the last iteration */
+  uint32_t iterations;      /* numIter of main.cu:312-430, counting the same-eps reruns */
+  uint32_t reindexes;       /* epsilon growths (grid rebuilds) */
+  uint32_t kernel_launches; /* range-kernel launches */
+  uint32_t failed_final;    /* queries left unfilled (0 on success) */
+  uint64_t candidates;      /* (query, candidate) pairs evaluated by the range kernel */
+  uint64_t algo_bytes;      /* compulsory HBM bytes of the range kernel launches */
+  double checksum;          /* sum over rows of sum_j (float)(d_j*d_j): main.cu:452 */
+  double ms_total;          /* device pipeline: points resident -> table resident */
+  double ms_reorder, ms_eps, ms_index, ms_join, ms_other;
+  double ms_range_kernel;   /* sum of range-kernel durations (HIP events) */
+  double ms_range_kernel_max;
+  uint64_t algo_bytes_max;  /* algorithmic bytes of the longest range-kernel launch */
+} knnjoin_stats;
+
+/*
+ * Replaces main.cu:514 `extern "C" void KNNJoinPy(DTYPE*, unsigned, unsigned, unsigned, int*, DTYPE*)`
+ * with DTYPE=float, GPUNUMDIM=NUMINDEXEDDIM=NDIM runtime values (the reference fixes them at
+ * compile time, params.h:5-9, Makefile:61-71), BETA=0.05 (params.h:58), REORDER=1.
+ * Host pointers.  Void like the reference; errors are reported by knnjoin_last_status().
+ * Rows of points never resolved (error exits) are left as -1 (main.cu:260-268).
+ */
+void KNNJoinPy(float *dataset, unsigned int NUMPOINTS, unsigned int kNN, unsigned int NDIM,
+               int *outNearestNeighborTable, float *outNearestNeighborTableDistances);
+
+/* Same as KNNJoinPy for DTYPE=double (the wrapper's DTYPE="double", knnjoingpu.py:213-217). */
+void KNNJoinPyDouble(double *dataset, unsigned int NUMPOINTS, unsigned int kNN, unsigned int NDIM,
+                     int *outNearestNeighborTable, double *outNearestNeighborTableDistances);
+
+/*
+ * Extended entry point: the whole pipeline of main.cu:109-510 / 513-908 with runtime
+ * NUMINDEXEDDIM, BETA, REORDER and radius mode, host or device pointers, and a status code.
+ * dtype: KNN_FLOAT32 / KNN_FLOAT64.  params/stats may be NULL.
+ */
+int knnjoin_ex(const void *points, int dtype, uint32_t n, uint32_t ndim, uint32_t k,
+               const knnjoin_params *params, int32_t *out_ids, void *out_dists,
+               knnjoin_stats *stats);
+
+/* Default parameters (reference defaults: NUMINDEXEDDIM=min(NDIM,6), BETA=0.05, REORDER=1). */
+void knnjoin_default_params(knnjoin_params *p);
+
+/* Status / message of the last call on this process (KNNJoinPy and friends return void). */
+int knnjoin_last_status(void);
+const char *knnjoin_last_error(void);
+
+/* Frees the cached device buffers (the reference keeps its pinned buffers for the life of
+ * the process, GPU.cu:51-59; cleanUpPinned GPU.cu:3570). */
+void knnjoin_release(void);
+
+/* ---------------------------------------------------------------------------------------
+ * Stage entry points: the hot path's intermediate results, exposed so that each stage can be
+ * checked against the CPU oracle.  All take host pointers, float32 points, row-major.
+ * ------------------------------------------------------------------------------------- */
+
+/* ReorderByDimension, main.cu:2521-2615: perm[j] = input column placed at column j. */
+int knnjoin_stage_reorder(const float *points, uint32_t n, uint32_t ndim, uint32_t *perm,
+                          float *variance);
+
+/* sampleNeighborsBruteForce, GPU.cu:3088-3566 (points already reordered).
+ * hist: 10000 buckets or NULL.  Returns eps0 through *eps. */
+int knnjoin_stage_estimate_eps(const float *points, uint32_t n, uint32_t ndim, uint32_t k,
+                               double beta, float *eps, double *avg_dist, uint64_t *pair_count,
+                               uint32_t *bucket, uint32_t *hist);
+
+/* generateNDGridDimensions main.cu:2425-2488 + populateNDGridIndexAndLookupArrayGPU
+ * GPU.cu:3715-4067 (points already reordered).  Outputs (caller-sized: n, n, n+1):
+ *   sorted_ids  point ids grouped by cell, ascending id within a cell (indexLookupArr),
+ *   cell_keys   linear ids of the non-empty cells, ascending (gridCellLookup.gridLinearID),
+ *   cell_start  first sorted position of each cell, plus n at the end (grid[c].indexmin).
+ * min_arr/n_cells: NUMINDEXEDDIM values.  Returns the non-empty cell count via *num_cells. */
+int knnjoin_stage_index(const float *points, uint32_t n, uint32_t ndim, uint32_t nd, float eps,
+                        float *min_arr, uint32_t *n_cells, uint64_t *total_cells,
+                        uint32_t *sorted_ids, uint64_t *cell_keys, uint32_t *cell_start,
+                        uint32_t *num_cells);
+
+/* One range-kernel pass (kernel.cu:61-225 + GPU.cu:3610-3704 + main.cu:1192-1223) at a fixed
+ * eps over the query ids given (points already reordered): rows of succeeding queries are
+ * written, failed[q] set to 1 for the others.  Returns the failure count via *num_failed. */
+int knnjoin_stage_join_pass(const float *points, uint32_t n, uint32_t ndim, uint32_t nd,
+                            uint32_t k, float eps, const uint32_t *queries, uint32_t nq,
+                            int32_t *out_ids, float *out_dists, uint8_t *failed,
+                            uint32_t *num_failed);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,31 @@
+"""CPU tests of the C-ABI library: it loads and exports every declared symbol (no GPU calls)."""
+import ctypes
+import os
+import re
+
+from hybrid_k_nearest_neighbor_self_join_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    text = open(os.path.join(ROOT, "include", "gpuknnselfjoin.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = re.findall(r"\b([A-Za-z_][A-Za-z0-9_]*)\s*\(", text)
+    return sorted({n for n in names if n.startswith(("knnjoin_", "KNNJoin"))})
+
+
+def test_header_symbols_listed():
+    assert declared_symbols() == sorted(_lib.EXPORTED_SYMBOLS)
+
+
+def test_library_loads_and_exports_everything():
+    L = ctypes.CDLL(_lib.LIB_PATH)
+    for name in declared_symbols():
+        assert hasattr(L, name), name
+
+
+def test_default_params_match_reference_defaults():
+    prm = _lib.default_params()
+    assert prm.beta == 0.05 and prm.reorder == 1 and prm.num_indexed_dims == 0
+    assert prm.mode == _lib.KNN_MODE_BETA
