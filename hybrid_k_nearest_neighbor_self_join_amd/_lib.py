@@ -67,6 +67,7 @@ class KnnStats(ctypes.Structure):
         ("ms_range_kernel", ctypes.c_double),
         ("ms_range_kernel_max", ctypes.c_double),
         ("algo_bytes_max", ctypes.c_uint64),
+        ("overflow_queries", ctypes.c_uint64),
     ]
 
     def as_dict(self) -> dict:

@@ -44,7 +44,8 @@ struct Error : std::runtime_error {
 enum Slot {
   S_PTS, S_X, S_XS, S_KEYS, S_KEYS_TMP, S_IDS, S_IDS_TMP, S_CELLKEYS, S_CELLSTART, S_FLAGS, S_POS,
   S_ACT, S_NTILES, S_TILEOFF, S_TILES, S_FAILED, S_ROWSUM, S_OUTIDS, S_OUTDISTS, S_SCRATCH,
-  S_HEAP, S_SMALL, S_PART, S_CTR, S_CSUM, S_HOSTFLAGS, S_NSLOTS
+  S_HEAP, S_SMALL, S_PART, S_CTR, S_CSUM, S_HOSTFLAGS, S_OVERFLOW, S_STAGE, S_DENSE, S_BFLAGS,
+  S_BPOS, S_BSTART, S_NSLOTS
 };
 
 struct Cache {
@@ -147,7 +148,7 @@ static int bits_for(uint64_t totalCells) {
 
 // ------------------------------------------------------------------ pipeline
 struct PassResult {
-  uint32_t nfail, ntiles;
+  uint32_t nfail, ntiles, noverflow;
   double kernel_ms;
   unsigned long long bytes_total;
 };
@@ -213,12 +214,33 @@ struct Pipeline {
     xs = buf<T>(S_XS, (size_t)n * d);
     gather_soa<T>(x, sortedIds, n, d, xs, s);
     sync();
+    // dense cell-start table: O(1) row lookups when the grid is small enough
+    denseStart = nullptr;
+    if (totalCells + 1 <= kDenseMaxCells) {
+      const uint32_t tc = (uint32_t)totalCells;
+      const size_t need = scan_scratch_words(tc + 1) + 1024;
+      if (need > scratchWords) {
+        scratchWords = need;
+        scratch = buf<uint32_t>(S_SCRATCH, scratchWords);
+      }
+      uint32_t *dense = buf<uint32_t>(S_DENSE, (size_t)tc + 1);
+      fill_u32(dense, 0u, (size_t)tc + 1, s);
+      dense_cell_counts(cellKeys, cellStart, ncell, dense, s);
+      scan_exclusive_u32(dense, dense, tc, scratch, s);
+      denseStart = dense;
+    }
   }
+
+  static constexpr uint64_t kDenseMaxCells = 1ull << 27;
+  const uint32_t *denseStart = nullptr;
 
   // One range pass (kernel.cu:61-225 + GPU.cu:3610-3704 + main.cu:1192-1223): queries with
   // activeFailed[q] != 0 (all when null), grouped by cell into tiles of <= 64.
   PassResult pass(const uint8_t *activeFailed, uint8_t *failed, int32_t *dIds, T *dDists,
                   double *rowSum, unsigned long long *ctr, uint32_t *failCount, Timer &tK);
+  uint32_t make_tiles(const uint8_t *activeFailed, uint32_t *&act, uint4 *&tiles);
+  const T *radius = nullptr;  // per-query radii (learned schedule), null = eps
+  uint32_t lastSpan = 1;
 };
 
 template <typename T>
@@ -325,31 +347,79 @@ struct Timer {
   }
 };
 
+// Builds tiles of <= 64 active queries per cell: activeFailed[q] != 0 selects q (all when
+// null).  Returns the tile count; act/tiles point into the cache.
 template <typename T>
-PassResult Pipeline<T>::pass(const uint8_t *activeFailed, uint8_t *failed, int32_t *dIds,
-                             T *dDists, double *rowSum, unsigned long long *ctr,
-                             uint32_t *failCount, Timer &tK) {
+uint32_t Pipeline<T>::make_tiles(const uint8_t *activeFailed, uint32_t *&act, uint4 *&tiles) {
   uint32_t *flags = buf<uint32_t>(S_FLAGS, n + 1);
   uint32_t *pos = buf<uint32_t>(S_POS, n + 1);
   active_flags(sortedIds, activeFailed, n, flags, s);
   scan_exclusive_u32(flags, pos, n, scratch, s);
-  uint32_t *act;
   if (!activeFailed) {
     act = sortedIds;
   } else {
     act = buf<uint32_t>(S_ACT, n);
     compact_active(sortedIds, flags, pos, n, act, s);
   }
-  uint32_t *ntiles = buf<uint32_t>(S_NTILES, (size_t)ncell + 1);
-  uint32_t *tileOff = buf<uint32_t>(S_TILEOFF, (size_t)ncell + 1);
-  cell_tile_counts(cellStart, pos, ncell, ntiles, s);
-  scan_exclusive_u32(ntiles, tileOff, ncell, scratch, s);
-  uint32_t NT = 0;
-  HIP_CHECK(hipMemcpyAsync(&NT, tileOff + ncell, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  // tile blocks: windows of `span` cells along dim 0 within one row.  A tile scans the union
+  // of its lanes' blocks, about (span + 2) cells per row, for min(64, span * rho_active)
+  // queries (rho_active = active queries per non-empty cell), so the cost per query is about
+  // (span + 2) / max(1, min(64, span * rho_active)): wide windows pay off only when active
+  // queries are dense; sparse passes use one cell per window.
+  uint32_t A = 0;
+  HIP_CHECK(hipMemcpyAsync(&A, pos + n, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
   HIP_CHECK(hipStreamSynchronize(s));
-  uint4 *tiles = buf<uint4>(S_TILES, NT ? NT : 1);
-  write_tiles(cellStart, pos, tileOff, ncell, tiles, s);
-  fill_u32(failCount, 0u, 1, s);
+  const uint32_t spanMax = nd == 1 ? 4096u : (nd == 2 ? 256u : 8u);
+  uint32_t span = 1;
+  if (A && ncell) {
+    const double rhoA = (double)A / (double)ncell;
+    double best = 1e300;
+    for (uint32_t w = 1; w <= spanMax; w *= 2) {
+      const double q = std::max(1.0, std::min(64.0, w * rhoA));
+      const double cost = (w + 2.0) / q;
+      if (cost < best * 0.97) {
+        best = cost;
+        span = w;
+      }
+    }
+  }
+  lastSpan = span;
+  uint32_t *bflags = buf<uint32_t>(S_BFLAGS, (size_t)ncell + 1);
+  uint32_t *bpos = buf<uint32_t>(S_BPOS, (size_t)ncell + 1);
+  uint32_t *bstart = buf<uint32_t>(S_BSTART, (size_t)ncell + 1);
+  block_flags(cellKeys, ncell, g.nCells[0], span, bflags, s);
+  scan_exclusive_u32(bflags, bpos, ncell, scratch, s);
+  compact_blocks(cellStart, bflags, bpos, ncell, n, bstart, s);
+  uint32_t B = 0;
+  HIP_CHECK(hipMemcpyAsync(&B, bpos + ncell, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipStreamSynchronize(s));
+  uint32_t *ntiles = buf<uint32_t>(S_NTILES, (size_t)B + 1);
+  uint32_t *tileOff = buf<uint32_t>(S_TILEOFF, (size_t)B + 1);
+  cell_tile_counts(bstart, pos, B, ntiles, s);
+  scan_exclusive_u32(ntiles, tileOff, B, scratch, s);
+  uint32_t NT = 0;
+  HIP_CHECK(hipMemcpyAsync(&NT, tileOff + B, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipStreamSynchronize(s));
+  tiles = buf<uint4>(S_TILES, NT ? NT : 1);
+  write_tiles(bstart, pos, tileOff, B, tiles, s);
+  return NT;
+}
+
+template <typename T>
+PassResult Pipeline<T>::pass(const uint8_t *activeFailed, uint8_t *failed, int32_t *dIds,
+                             T *dDists, double *rowSum, unsigned long long *ctr,
+                             uint32_t *failCount, Timer &tK) {
+  // ctr layout (u64 words): [0] pairs, [1] algorithmic bytes, [2] fail|overflow counters,
+  // [3] tile queue head
+  uint32_t *counters = reinterpret_cast<uint32_t *>(ctr + 2);
+  uint32_t *overflowCount = counters + 1;
+  uint32_t *tileCounter = counters + 2;
+  uint32_t *act;
+  uint4 *tiles;
+  const uint32_t NT = make_tiles(activeFailed, act, tiles);
+  uint8_t *overflow = buf<uint8_t>(S_OVERFLOW, n);
+  fill_u8(overflow, 0, n, s);
+  fill_u32(counters, 0u, 4, s);
 
   JoinArgs a{};
   a.x = x;
@@ -358,30 +428,71 @@ PassResult Pipeline<T>::pass(const uint8_t *activeFailed, uint8_t *failed, int32
   a.cellKeys = cellKeys;
   a.cellStart = cellStart;
   a.ncell = ncell;
+  a.denseStart = denseStart;
   a.act = act;
   a.tiles = tiles;
   a.ntiles = NT;
   a.n = n;
   a.d = d;
   a.k1 = K1;
+  a.radius = radius;
   a.outIds = dIds;
   a.outDists = dDists;
-  a.rowSum = rowSum;
   a.failed = failed;
-  a.failCount = failCount;
+  a.failCount = counters;
   a.candCount = ctr;
-  size_t hs = range_heap_scratch_bytes<T>(NT, K1);
-  a.heapScratch = hs ? g_cache.get(S_HEAP, hs) : nullptr;
-  tK.start();
-  range_topk<T>(a, g, s);
+  a.tileCounter = tileCounter;
+  // LDS staging slots per query: 2(K+1)+32, at most 512 (8 ranked items per lane) and
+  // within the 64 KB a workgroup may allocate
+  a.smax = std::min<uint32_t>(2 * K1 + 32, 512u);
+  while (a.smax > K1 && range_select_lds_bytes<T>(a.smax) > 65536) a.smax -= 16;
+  a.overflow = overflow;
+  a.overflowCount = overflowCount;
   PassResult r{};
-  r.kernel_ms = tK.stop();
   r.ntiles = NT;
+  if (!(a.smax > K1 && range_select_lds_bytes<T>(a.smax) <= 65536)) {
+    // K+1 too large for LDS staging: the heap kernel takes every query of the pass
+    size_t hs = range_heap_scratch_bytes<T>(NT, K1);
+    a.heapScratch = hs ? g_cache.get(S_HEAP, hs) : nullptr;
+    tK.start();
+    range_topk<T>(a, g, s);
+    r.kernel_ms = tK.stop();
+    r.noverflow = NT ? n : 0;
+    unsigned long long hctr[2];
+    HIP_CHECK(hipMemcpyAsync(&r.nfail, counters, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipMemcpyAsync(hctr, ctr, sizeof(hctr), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    r.bytes_total = hctr[1];
+    return r;
+  }
+  tK.start();
+  range_select<T>(a, g, s);
+  r.kernel_ms = tK.stop();
+  uint32_t hc[2];
+  HIP_CHECK(hipMemcpyAsync(hc, counters, sizeof(hc), hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipStreamSynchronize(s));
+  r.noverflow = hc[1];
+  if (hc[1]) {
+    // queries whose candidates overflow the staging slots: per-lane heap kernel
+    uint32_t *act2;
+    uint4 *tiles2;
+    const uint32_t NT2 = make_tiles(overflow, act2, tiles2);
+    fill_u32(tileCounter, 0u, 1, s);
+    a.act = act2;
+    a.tiles = tiles2;
+    a.ntiles = NT2;
+    size_t hs = range_heap_scratch_bytes<T>(NT2, K1);
+    a.heapScratch = hs ? g_cache.get(S_HEAP, hs) : nullptr;
+    tK.start();
+    range_topk<T>(a, g, s);
+    r.kernel_ms += tK.stop();
+  }
   unsigned long long hctr[2];
-  HIP_CHECK(hipMemcpyAsync(&r.nfail, failCount, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipMemcpyAsync(&r.nfail, counters, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
   HIP_CHECK(hipMemcpyAsync(hctr, ctr, sizeof(hctr), hipMemcpyDeviceToHost, s));
   HIP_CHECK(hipStreamSynchronize(s));
   r.bytes_total = hctr[1];
+  (void)failCount;
   return r;
 }
 
@@ -530,6 +641,7 @@ static int run_join(const T *pts_in, uint32_t n, uint32_t d, uint32_t k, const k
     tPh.start();
     PassResult pr = P.pass(first ? nullptr : failed, failed, dIds, dDists, rowSum, ctr, failCount, tK);
     launches++;
+    stats.overflow_queries += pr.noverflow;
     kMs += pr.kernel_ms;
     if (pr.kernel_ms > kMax) {
       kMax = pr.kernel_ms;
@@ -539,8 +651,8 @@ static int run_join(const T *pts_in, uint32_t n, uint32_t d, uint32_t k, const k
     const uint32_t nfail = pr.nfail;
     joinMs += tPh.stop();
     if (P.verbose)
-      printf("[knn] iter %u eps %.9g queries %u failed %u tiles %u cells %u kernel %.3f ms\n",
-             numIter, (double)eps, nq, nfail, pr.ntiles, P.ncell, pr.kernel_ms);
+      printf("[knn] iter %u eps %.9g queries %u failed %u tiles %u cells %u span %u overflow %u kernel %.3f ms\n",
+             numIter, (double)eps, nq, nfail, pr.ntiles, P.ncell, P.lastSpan, pr.noverflow, pr.kernel_ms);
     frac = (nfail * 1.0) / (nq * 1.0);  // main.cu:402
     nq = nfail;
     numIter++;
@@ -554,6 +666,7 @@ static int run_join(const T *pts_in, uint32_t n, uint32_t d, uint32_t k, const k
   // ---- checksum (main.cu:452)
   double *part = P.template buf<double>(S_CSUM, 64 + (n + 4095) / 4096 + 16);
   double *csum = part + (n + 4095) / 4096 + 8;
+  row_sums<T>(dDists, n, K1, rowSum, s);
   sum_rows(rowSum, n, part, csum, s);
   double hcs = 0;
   unsigned long long hctr[2];

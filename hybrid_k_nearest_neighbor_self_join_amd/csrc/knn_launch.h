@@ -75,6 +75,19 @@ void compact_cells_u64(const uint64_t *keys, const uint32_t *flags, const uint32
 template <typename T>
 void gather_soa(const T *x, const uint32_t *ids, uint32_t n, uint32_t d, T *xs, hipStream_t s);
 
+// Dense cell-start table (totalCells + 1 entries): denseStart[id] = first sorted position of
+// linear cell id (number of points in cells < id).  counts are scattered from the compacted
+// cell table, then scanned in place.
+void dense_cell_counts(const uint64_t *cellKeys, const uint32_t *cellStart, uint32_t ncell,
+                       uint32_t *dense, hipStream_t s);
+// Tile blocks: runs of cells in one row (dims 1.. equal) whose dim-0 coordinate falls in the
+// same window of `span` cells.  blockStart[b] = first sorted position of block b,
+// blockStart[B] = n.  flags/pos are ncell + 1 scratch words.
+void block_flags(const uint64_t *cellKeys, uint32_t ncell, uint32_t ncell0, uint32_t span,
+                 uint32_t *flags, hipStream_t s);
+void compact_blocks(const uint32_t *cellStart, const uint32_t *flags, const uint32_t *pos,
+                    uint32_t ncell, uint32_t n, uint32_t *blockStart, hipStream_t s);
+
 // active-query flags per sorted position (all = 1 when failed == NULL)
 void active_flags(const uint32_t *sortedIds, const uint8_t *failed, uint32_t n, uint32_t *flags,
                   hipStream_t s);
@@ -94,25 +107,41 @@ struct JoinArgs {
   const uint64_t *cellKeys;
   const uint32_t *cellStart;
   uint32_t ncell;
+  const uint32_t *denseStart;  // dense cell-start table, or null (binary search of cellKeys)
   const uint32_t *act;   // active query ids, grouped by cell
-  const uint4 *tiles;    // {cell, first active position, count, 0}
+  const uint4 *tiles;    // {block, first active position, count, first tile of its block}
   uint32_t ntiles;
   uint32_t n, d, k1;
+  const void *radius;    // per-query search radius (T); null = the grid's eps for all
   int32_t *outIds;
   void *outDists;
-  double *rowSum;        // per-row sum of (T)(dist*dist), for the checksum
   uint8_t *failed;       // per point: 1 = failed this pass
   uint32_t *failCount;   // device counter (added to)
-  unsigned long long *candCount;  // device counter of (query, candidate) pairs (added to)
-  void *heapScratch;     // global heap storage when K+1 does not fit in LDS
+  unsigned long long *candCount;  // [0] (query, candidate) pairs, [1] algorithmic bytes (added to)
+  uint32_t *tileCounter; // dynamic tile queue head (zeroed before the launch)
+  void *heapScratch;     // heap kernel: global heap storage when K+1 does not fit in LDS
+  uint32_t smax;         // select kernel: LDS staging slots per query
+  uint8_t *overflow;     // select kernel: per point, 1 = needs the heap kernel
+  uint32_t *overflowCount;
 };
+// Two-pass select kernel (histogram, then bucket-ordered collect); queries whose candidates
+// do not fit in `smax` staging slots are flagged in `overflow` for the heap kernel.
+template <typename T>
+void range_select(const JoinArgs &a, const typename GridOf<T>::type &g, hipStream_t s);
+template <typename T>
+size_t range_select_lds_bytes(uint32_t smax);
+uint32_t range_select_grid(uint32_t ntiles);
+// Heap kernel (per-lane max-heap on (key, id)): handles any input, used for overflow queries.
 template <typename T>
 void range_topk(const JoinArgs &a, const typename GridOf<T>::type &g, hipStream_t s);
 // bytes of heap scratch needed in global memory for `ntiles` tiles (0 if LDS is used)
 template <typename T>
 size_t range_heap_scratch_bytes(uint32_t ntiles, uint32_t k1);
 
-// deterministic sum of rowSum over the rows (fixed tree): result in out[0]
+// per-row sums of (T)(d*d) over the K+1 entries of each row in entry order (main.cu:452)
+template <typename T>
+void row_sums(const T *dists, uint32_t n, uint32_t k1, double *rowSum, hipStream_t s);
+// deterministic sum of rowSum over the rows (chunks of 4096 rows, then chunk sums): out[0]
 void sum_rows(const double *rowSum, uint32_t n, double *partial, double *out, hipStream_t s);
 
 }  // namespace knn

@@ -1,0 +1,700 @@
+// range.hip -- the epsilon-range top-(K+1) kernels for gfx950.
+//
+// Replace kernel.cu:61-225 (range query over the adjacent cells), the pair buffer and the two
+// thrust::stable_sort_by_key of GPU.cu:1629-1634, and the host top-k of GPU.cu:3610-3704.
+//
+// Work unit: one wavefront per tile of <= 64 consecutive active queries of one tile block (a
+// window of cells along dim 0 in one row of the grid); lane = query.  Each lane has a search radius r (the grid's eps, or its own radius) and scans the
+// cells [coord(x - r, rounded down), coord(x + r, rounded up)] per indexed dim; the tile scans
+// the union of its lanes' blocks.  Within one row of that block (dims 1.. fixed) the cells of
+// dim 0 are adjacent linear ids, i.e. one contiguous range of cell-sorted points, streamed
+// 64 at a time (one coalesced load per lane) and broadcast to all lanes.
+//
+// Ranking: FP64 key = sum_j ((double)c_j - (double)q_j)^2, ties by point id.  The FP32 sum
+// s = sum_j (c_j - q_j)^2 (reported distance sqrtf(s)) is within 2^-19 of the key relative,
+// so it can filter: a candidate whose key could be <= X has s <= X (1 + 2^-16).
+//
+// Success (per query): at least K+1 candidates with key <= Tq = r^2 (1 - 2^-20).  Every point
+// with key <= Tq lies within r of the query in each indexed dim, hence inside the scanned
+// block, so a success is the exact global top-(K+1).  The reference's FP32 test
+// (kernel.cu:130-147) differs from this only for candidates within 2^-20 of eps.
+//
+// k_range_select (main kernel), per tile:
+//   pass 1  (lane = query) per-lane histogram of s over [0, F] (F = Tq (1+2^-16)) in NB=64
+//           linear buckets, LDS ds_add without return;
+//   scan    b* = first bucket whose cumulative count reaches K+1; each histogram column
+//           becomes that query's bucket slot offsets (a counting sort);
+//   pass 2  (wave = query, for each query of the tile) the 64 lanes scan the query's own
+//           sub-block, radius sqrt(edge of b*), coalesced; candidates in buckets <= b* (plus a
+//           2^-16 margin past b*'s edge) are placed in LDS staging in bucket order, ranked
+//           exactly by (key, id) against buckets b-1..b+1 only, and the first K+1 are written
+//           to the row (coalesced) when the (K+1)-th key is <= Tq.
+//   A query whose staged list exceeds `smax` LDS slots (e.g. thousands of duplicates in one
+//   bucket) is flagged for k_range_heap.
+// k_range_heap: per-lane max-heap of (key, id), in LDS when K+1 fits, else in global scratch.
+#include "knn_launch.h"
+
+namespace knn {
+
+namespace {
+
+__device__ __forceinline__ bool kless(double ka, uint32_t ia, double kb, uint32_t ib) {
+  return ka < kb || (ka == kb && ia < ib);
+}
+
+__device__ __forceinline__ uint32_t lower_bound_u64(const uint64_t *a, uint32_t n, uint64_t v) {
+  uint32_t lo = 0, hi = n;
+  while (lo < hi) {
+    uint32_t m = (lo + hi) >> 1;
+    if (a[m] < v) lo = m + 1; else hi = m;
+  }
+  return lo;
+}
+__device__ __forceinline__ uint32_t upper_bound_u64(const uint64_t *a, uint32_t n, uint64_t v) {
+  uint32_t lo = 0, hi = n;
+  while (lo < hi) {
+    uint32_t m = (lo + hi) >> 1;
+    if (a[m] <= v) lo = m + 1; else hi = m;
+  }
+  return lo;
+}
+
+// Reported-distance arithmetic of the input type and the filter margin that bounds it
+// against the FP64 key.
+template <typename T> struct Num;
+template <> struct Num<float> {
+  template <int D>
+  __device__ static __forceinline__ float sq(const float *c, const float *q) {
+    float s = 0.f;
+#pragma unroll
+    for (int j = 0; j < D; j++) {
+      float df = c[j] - q[j];
+      float t = df * df;
+      s = s + t;
+    }
+    return s;
+  }
+  __device__ static __forceinline__ float bound(double key) { return (float)(key * (1.0 + 0x1p-16)); }
+  static constexpr float margin = 1.0f + 0x1p-16f;
+};
+template <> struct Num<double> {
+  template <int D>
+  __device__ static __forceinline__ double sq(const double *c, const double *q) {
+    double s = 0.0;
+#pragma unroll
+    for (int j = 0; j < D; j++) {
+      double df = c[j] - q[j];
+      double t = df * df;
+      s = s + t;
+    }
+    return s;
+  }
+  __device__ static __forceinline__ double bound(double key) { return key; }
+  static constexpr double margin = 1.0;
+};
+
+template <int D, typename T>
+__device__ __forceinline__ double fp64_key(const T *c, const double *qd) {
+  double s = 0.0;
+#pragma unroll
+  for (int j = 0; j < D; j++) {
+    double dx = (double)c[j] - qd[j];
+    double t = dx * dx;
+    s = s + t;
+  }
+  return s;
+}
+
+// Per-lane state of one tile.
+template <typename T, int D>
+struct Lane {
+  bool active;
+  uint32_t q;
+  T qc[D];
+  double qd[D];
+  double Tq;  // success threshold on the key
+  T F;        // filter bound on the FP32/T sum
+};
+
+template <typename T, int D, typename G>
+__device__ __forceinline__ void tile_setup(const JoinArgs &a, const G &g, const uint4 &tl,
+                                           uint32_t lane, Lane<T, D> &L, uint32_t *lo,
+                                           uint32_t *hi, uint32_t *ownLo, uint32_t *ownHi,
+                                           uint64_t &cellId) {
+  const T *__restrict__ x = reinterpret_cast<const T *>(a.x);
+  const size_t n = a.n;
+  L.active = lane < tl.z;
+  L.q = a.act[tl.y + (L.active ? lane : 0u)];
+#pragma unroll
+  for (int j = 0; j < D; j++) {
+    L.qc[j] = x[(size_t)j * n + L.q];
+    L.qd[j] = (double)L.qc[j];
+  }
+  const T r = a.radius ? reinterpret_cast<const T *>(a.radius)[L.q] : (T)g.eps;
+  L.Tq = (double)r * (double)r * (1.0 - 0x1p-20);
+  L.F = Num<T>::bound(L.Tq);
+  const T eps = (T)g.eps;
+  cellId = 0;
+  for (uint32_t j = 0; j < g.nd; j++) {
+    const T mn = (T)g.minArr[j];
+    const uint32_t nc = g.nCells[j];
+    T xj = L.qc[0];
+#pragma unroll
+    for (int jj = 1; jj < D; jj++)
+      if ((uint32_t)jj == j) xj = L.qc[jj];
+    cellId += (uint64_t)cell_coord<T>(xj, mn, eps, nc) * g.mult[j];
+    uint32_t l = cell_coord<T>(next_down(xj - r), mn, eps, nc);
+    uint32_t u = cell_coord<T>(next_up(xj + r), mn, eps, nc);
+    ownLo[j] = l;
+    ownHi[j] = u;
+    if (!L.active) {
+      l = 0xffffffffu;
+      u = 0u;
+    }
+    lo[j] = wave_min_u32(l);
+    hi[j] = wave_max_u32(u);
+  }
+}
+
+// Compulsory-bytes bookkeeping: a query cell's block of candidates is counted once, by the
+// cell's first active query in the active list.
+template <typename T, int D, typename G>
+__device__ __forceinline__ uint64_t own_block_bytes(const JoinArgs &a, const G &g,
+                                                    const uint4 &tl, uint32_t lane, bool active,
+                                                    uint64_t cellId, const uint32_t *ownLo,
+                                                    const uint32_t *ownHi) {
+  // previous active query's cell: lane-1, or for lane 0 the query before the tile
+  uint64_t prev = ((uint64_t)(uint32_t)__shfl((int)(uint32_t)(cellId >> 32), (int)lane - 1, 64) << 32) |
+                  (uint32_t)__shfl((int)(uint32_t)cellId, (int)lane - 1, 64);
+  bool first = active;
+  if (lane == 0) {
+    if (tl.y > 0) {
+      const T *__restrict__ x = reinterpret_cast<const T *>(a.x);
+      const uint32_t pq = a.act[tl.y - 1];
+      uint64_t pid = 0;
+      for (uint32_t j = 0; j < g.nd; j++)
+        pid += (uint64_t)cell_coord<T>(x[(size_t)j * a.n + pq], (T)g.minArr[j], (T)g.eps, g.nCells[j]) * g.mult[j];
+      first = first && pid != cellId;
+    }
+  } else {
+    first = first && prev != cellId;
+  }
+  if (!first) return 0;
+  uint64_t cand = 0, rows = 0;
+  uint32_t cur[KNN_MAX_INDEXED];
+  for (uint32_t j = 0; j < g.nd; j++) cur[j] = ownLo[j];
+  for (;;) {
+    uint64_t rowBase = 0;
+    for (uint32_t j = 1; j < g.nd; j++) rowBase += (uint64_t)cur[j] * g.mult[j];
+    uint32_t pBeg, pEnd;
+    if (a.denseStart) {
+      pBeg = a.denseStart[rowBase + ownLo[0]];
+      pEnd = a.denseStart[rowBase + ownHi[0] + 1];
+    } else {
+      pBeg = a.cellStart[lower_bound_u64(a.cellKeys, a.ncell, rowBase + ownLo[0])];
+      pEnd = a.cellStart[upper_bound_u64(a.cellKeys, a.ncell, rowBase + ownHi[0])];
+    }
+    cand += pEnd - pBeg;
+    rows++;
+    uint32_t j = 1;
+    while (j < g.nd) {
+      if (cur[j] < ownHi[j]) {
+        cur[j]++;
+        break;
+      }
+      cur[j] = ownLo[j];
+      j++;
+    }
+    if (j >= g.nd) break;
+  }
+  return cand * (sizeof(T) * D + 4ull) + rows * 8ull;
+}
+
+// Calls f(pBeg, pEnd) for each row of the block [lo, hi] holding points (the arguments and the
+// calls are uniform across the wave when lo/hi are).
+template <typename G, typename F>
+__device__ __forceinline__ void for_each_range(const JoinArgs &a, const G &g, const uint32_t *lo,
+                                               const uint32_t *hi, F &&f) {
+  uint32_t cur[KNN_MAX_INDEXED];
+  for (uint32_t j = 0; j < g.nd; j++) cur[j] = lo[j];
+  for (;;) {
+    uint64_t rowBase = 0;
+    for (uint32_t j = 1; j < g.nd; j++) rowBase += (uint64_t)cur[j] * g.mult[j];
+    uint32_t pBeg, pEnd;
+    if (a.denseStart) {
+      pBeg = a.denseStart[rowBase + lo[0]];
+      pEnd = a.denseStart[rowBase + hi[0] + 1];
+    } else {
+      const uint32_t cbeg = lower_bound_u64(a.cellKeys, a.ncell, rowBase + lo[0]);
+      const uint32_t cend = upper_bound_u64(a.cellKeys, a.ncell, rowBase + hi[0]);
+      pBeg = a.cellStart[cbeg];
+      pEnd = a.cellStart[cend];
+    }
+    if (pEnd > pBeg) f(pBeg, pEnd);
+    uint32_t j = 1;
+    while (j < g.nd) {
+      if (cur[j] < hi[j]) {
+        cur[j]++;
+        break;
+      }
+      cur[j] = lo[j];
+      j++;
+    }
+    if (j >= g.nd) break;
+  }
+}
+
+__device__ __forceinline__ uint32_t next_tile(uint32_t *counter, uint32_t lane) {
+  uint32_t t = 0;
+  if (lane == 0) t = atomicAdd(counter, 1u);
+  return (uint32_t)__shfl((int)t, 0, 64);
+}
+
+template <typename T, int D>
+__device__ __forceinline__ void tile_accounting(const JoinArgs &a, const uint4 &tl, uint32_t lane,
+                                                bool ok, bool failedLane, uint64_t tileCand,
+                                                uint64_t ownBytes, uint32_t K1) {
+  const uint64_t fm = __ballot(failedLane);
+  const uint64_t om = __ballot(ok);
+  // compulsory HBM bytes: each query cell's candidate block once, query ids in / failure
+  // flags out, and the finished rows
+  for (int o = 32; o > 0; o >>= 1) ownBytes += __shfl_xor(ownBytes, o, 64);
+  if (lane == 0) {
+    if (fm) atomicAdd(a.failCount, (uint32_t)__popcll(fm));
+    unsigned long long b = (unsigned long long)tl.z * 8ull +
+                           (unsigned long long)__popcll(om) * K1 * (4ull + sizeof(T)) + ownBytes;
+    atomicAdd(a.candCount + 1, b);
+    atomicAdd(a.candCount, (unsigned long long)tileCand * tl.z);
+  }
+}
+
+// ====================================================================== select kernel
+constexpr int NB = 64;
+constexpr uint32_t SELECT_MAX_GRID = 2048;
+}  // namespace
+template <typename T>
+size_t range_select_lds_bytes(uint32_t smax);
+namespace {
+
+template <typename T>
+__device__ __forceinline__ uint32_t bucket_of(T s, T scale) {
+  const uint32_t b = (uint32_t)(s * scale);
+  return b < (uint32_t)NB ? b : (uint32_t)NB - 1;
+}
+
+// LDS: hist[NB][64] u32 (per-lane histogram columns, then slot cursors) followed by the
+// per-query staging arrays key[SM] (double), id[SM] (u32), s[SM] (T).
+template <typename T, int D, typename G>
+__global__ __launch_bounds__(64) void k_range_select(JoinArgs a, G g) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  uint32_t *hist = reinterpret_cast<uint32_t *>(smem);
+  const uint32_t SM = a.smax;
+  double *skey = reinterpret_cast<double *>(smem + NB * 64 * sizeof(uint32_t));
+  uint32_t *sid = reinterpret_cast<uint32_t *>(skey + SM);
+  T *ss = reinterpret_cast<T *>(sid + SM + (SM & 1));
+  const uint32_t lane = threadIdx.x;
+  const uint32_t K1 = a.k1;
+  const T *__restrict__ xs = reinterpret_cast<const T *>(a.xs);
+  const size_t n = a.n;
+  const T eps = (T)g.eps;
+
+  for (;;) {
+    const uint32_t t = next_tile(a.tileCounter, lane);
+    if (t >= a.ntiles) break;
+    const uint4 tl = a.tiles[t];
+    Lane<T, D> L;
+    uint32_t lo[KNN_MAX_INDEXED], hi[KNN_MAX_INDEXED], ownLo[KNN_MAX_INDEXED], ownHi[KNN_MAX_INDEXED];
+    uint64_t cellId;
+    tile_setup<T, D, G>(a, g, tl, lane, L, lo, hi, ownLo, ownHi, cellId);
+    const uint64_t ownBytes = own_block_bytes<T, D, G>(a, g, tl, lane, L.active, cellId, ownLo, ownHi);
+    const T scale = (T)NB / L.F;
+    __syncthreads();  // the previous tile's readers of hist/staging are done
+#pragma unroll 8
+    for (int b = 0; b < NB; b++) hist[b * 64 + lane] = 0u;
+    uint64_t tileCand = 0;
+
+    // ---- pass 1 (lane = query): histogram of s over [0, F] for all queries of the tile
+    for_each_range(a, g, lo, hi, [&](uint32_t pBeg, uint32_t pEnd) {
+      tileCand += pEnd - pBeg;
+      for (uint32_t base = pBeg; base < pEnd; base += 64) {
+        const uint32_t pp = base + lane < pEnd ? base + lane : pBeg;
+        T cv[D];
+#pragma unroll
+        for (int j = 0; j < D; j++) cv[j] = xs[(size_t)j * n + pp];
+        const uint32_t m = (pEnd - base) < 64u ? (pEnd - base) : 64u;
+        for (uint32_t jj = 0; jj < m; jj++) {
+          T c[D];
+#pragma unroll
+          for (int j = 0; j < D; j++) c[j] = readlane<T>(cv[j], (int)jj);
+          const T s = Num<T>::template sq<D>(c, L.qc);
+          if (L.active && s <= L.F) atomicAdd(&hist[bucket_of<T>(s, scale) * 64 + lane], 1u);
+        }
+      }
+    });
+
+    // ---- scan: b* per query; counts become exclusive slot offsets
+    uint32_t cum = 0, bstar = NB;
+    for (int b = 0; b < NB; b++) {
+      const uint32_t h = hist[b * 64 + lane];
+      hist[b * 64 + lane] = cum;
+      if (bstar == NB && cum + h >= K1) bstar = (uint32_t)b;
+      cum += h;
+    }
+    const bool go = L.active && bstar < (uint32_t)NB;
+    const T edge = go ? ((T)(bstar + 1) / scale) * Num<T>::margin : (T)0;
+    bool okMine = false, ovfMine = false;
+    const uint64_t goMask = __ballot(go);
+    __syncthreads();
+
+    // ---- pass 2 (wave = query): for each query, scan its own sub-block (radius sqrt(edge)),
+    // stage the candidates of buckets <= b* (plus the margin) in LDS in bucket order, rank
+    // them exactly, write the row coalesced
+    for (uint32_t i = 0; i < tl.z; i++) {
+      if (!((goMask >> i) & 1ull)) continue;
+      T qi[D];
+      double qdi[D];
+#pragma unroll
+      for (int j = 0; j < D; j++) {
+        qi[j] = readlane<T>(L.qc[j], (int)i);
+        qdi[j] = (double)qi[j];
+      }
+      const T Fi = readlane<T>(L.F, (int)i);
+      const T sci = readlane<T>(scale, (int)i);
+      const T edi = readlane<T>(edge, (int)i);
+      const double Tqi = readlane<double>(L.Tq, (int)i);
+      const uint32_t bsi = (uint32_t)__builtin_amdgcn_readlane((int)bstar, (int)i);
+      const uint32_t qid = (uint32_t)__builtin_amdgcn_readlane((int)L.q, (int)i);
+      // every staged candidate has key <= edge (1 + 2^-19): per-dim reach sqrt of that
+      const T ri = (T)next_up((T)sqrt((double)edi * (1.0 + 0x1p-16)));
+      uint32_t slo[KNN_MAX_INDEXED], shi[KNN_MAX_INDEXED];
+      for (uint32_t j = 0; j < g.nd; j++) {
+        T xj = qi[0];
+#pragma unroll
+        for (int jj = 1; jj < D; jj++)
+          if ((uint32_t)jj == j) xj = qi[jj];
+        slo[j] = cell_coord<T>(next_down(xj - ri), (T)g.minArr[j], eps, g.nCells[j]);
+        shi[j] = cell_coord<T>(next_up(xj + ri), (T)g.minArr[j], eps, g.nCells[j]);
+      }
+      bool ovf = false;
+      for_each_range(a, g, slo, shi, [&](uint32_t pBeg, uint32_t pEnd) {
+        for (uint32_t base = pBeg; base < pEnd; base += 64) {
+          const uint32_t p = base + lane;
+          if (p < pEnd) {
+            T c[D];
+#pragma unroll
+            for (int j = 0; j < D; j++) c[j] = xs[(size_t)j * n + p];
+            const T s = Num<T>::template sq<D>(c, qi);
+            if (s <= Fi) {
+              uint32_t b = bucket_of<T>(s, sci);
+              if (b <= bsi || s <= edi) {
+                b = b < bsi ? b : bsi;
+                const uint32_t slot = atomicAdd(&hist[b * 64 + i], 1u);
+                if (slot < SM) {
+                  skey[slot] = fp64_key<D, T>(c, qdi);
+                  sid[slot] = a.sortedIds[p];
+                  ss[slot] = s;
+                } else {
+                  ovf = true;
+                }
+              }
+            }
+          }
+        }
+      });
+      __syncthreads();
+      const uint32_t m = hist[bsi * 64 + i];
+      if (__ballot(ovf) || m > SM) {
+        if (lane == i) ovfMine = true;
+        __syncthreads();
+        continue;
+      }
+      // rank: items of buckets <= b-2 all precede an item of bucket b, items of buckets >= b+2
+      // all follow it (a bucket is >= 1/64 of F wide, FP32/FP64 disagree by < 2^-19), so
+      // only buckets b-1..b+1 are compared
+      constexpr int MAXI = 8;
+      uint32_t rk[MAXI];
+      bool okHere = false;
+#pragma unroll
+      for (int u = 0; u < MAXI; u++) {
+        rk[u] = 0xffffffffu;
+        const uint32_t tI = lane + 64u * u;
+        if (tI < m) {
+          const double kt = skey[tI];
+          const uint32_t it = sid[tI];
+          uint32_t b = bucket_of<T>(ss[tI], sci);
+          b = b < bsi ? b : bsi;
+          const uint32_t w0 = b >= 2 ? hist[(b - 2) * 64 + i] : 0u;
+          const uint32_t w1 = b + 1 < bsi ? hist[(b + 1) * 64 + i] : m;
+          uint32_t r = w0;
+          for (uint32_t v = w0; v < w1; v++) r += kless(skey[v], sid[v], kt, it) ? 1u : 0u;
+          rk[u] = r;
+          if (r == K1 - 1 && kt <= Tqi) okHere = true;
+        }
+      }
+      const bool okq = __ballot(okHere) != 0;
+      if (okq) {
+        T *od = reinterpret_cast<T *>(a.outDists);
+#pragma unroll
+        for (int u = 0; u < MAXI; u++) {
+          const uint32_t tI = lane + 64u * u;
+          if (tI < m && rk[u] < K1) {
+            a.outIds[(size_t)qid * K1 + rk[u]] = (int32_t)sid[tI];
+            od[(size_t)qid * K1 + rk[u]] = sqrt(ss[tI]);
+          }
+        }
+      }
+      if (lane == i) okMine = okq;
+      __syncthreads();
+    }
+
+    if (L.active) {
+      if (ovfMine) {
+        a.overflow[L.q] = 1;
+      } else {
+        a.failed[L.q] = okMine ? 0 : 1;
+      }
+    }
+    const uint64_t om = __ballot(L.active && ovfMine);
+    if (lane == 0 && om) atomicAdd(a.overflowCount, (uint32_t)__popcll(om));
+    tile_accounting<T, D>(a, tl, lane, okMine, L.active && !okMine && !ovfMine, tileCand, ownBytes, K1);
+  }
+}
+
+// ====================================================================== heap kernel
+// Per-lane heap with lane-interleaved slots: key[s*64 + lane], id[s*64 + lane].
+struct Heap {
+  double *key;
+  uint32_t *id;
+  uint32_t lane;
+  __device__ __forceinline__ double &K(uint32_t s) const { return key[(size_t)s * 64 + lane]; }
+  __device__ __forceinline__ uint32_t &I(uint32_t s) const { return id[(size_t)s * 64 + lane]; }
+
+  __device__ void push(uint32_t &size, double k, uint32_t kid) const {
+    uint32_t i = size++;
+    while (i > 0) {
+      uint32_t p = (i - 1) >> 1;
+      double pk = K(p);
+      uint32_t pid = I(p);
+      if (!kless(pk, pid, k, kid)) break;
+      K(i) = pk;
+      I(i) = pid;
+      i = p;
+    }
+    K(i) = k;
+    I(i) = kid;
+  }
+  // replace the maximum of a heap of `size` entries by (k, kid) and restore the order
+  __device__ void replace_top(uint32_t size, double k, uint32_t kid) const {
+    uint32_t i = 0;
+    for (;;) {
+      uint32_t l = 2 * i + 1;
+      if (l >= size) break;
+      uint32_t c = l;
+      double ck = K(l);
+      uint32_t cid = I(l);
+      if (l + 1 < size) {
+        double rk = K(l + 1);
+        uint32_t rid = I(l + 1);
+        if (kless(ck, cid, rk, rid)) {
+          c = l + 1;
+          ck = rk;
+          cid = rid;
+        }
+      }
+      if (!kless(k, kid, ck, cid)) break;
+      K(i) = ck;
+      I(i) = cid;
+      i = c;
+    }
+    K(i) = k;
+    I(i) = kid;
+  }
+  // heap sort: ascending (key, id)
+  __device__ void sort(uint32_t size) const {
+    for (uint32_t end = size; end > 1; end--) {
+      double mk = K(0);
+      uint32_t mid = I(0);
+      double lk = K(end - 1);
+      uint32_t lid = I(end - 1);
+      K(end - 1) = mk;
+      I(end - 1) = mid;
+      replace_top(end - 1, lk, lid);
+    }
+  }
+};
+
+template <typename T, int D, bool LDS, typename G>
+__global__ __launch_bounds__(64) void k_range_heap(JoinArgs a, G g) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const uint32_t lane = threadIdx.x;
+  const uint32_t K1 = a.k1;
+  Heap h;
+  h.lane = lane;
+  if (LDS) {
+    h.key = reinterpret_cast<double *>(smem);
+    h.id = reinterpret_cast<uint32_t *>(smem + (size_t)K1 * 64 * sizeof(double));
+  } else {
+    unsigned char *base =
+        reinterpret_cast<unsigned char *>(a.heapScratch) + (size_t)blockIdx.x * K1 * 64 * 12;
+    h.key = reinterpret_cast<double *>(base);
+    h.id = reinterpret_cast<uint32_t *>(base + (size_t)K1 * 64 * sizeof(double));
+  }
+  const T *__restrict__ x = reinterpret_cast<const T *>(a.x);
+  const T *__restrict__ xs = reinterpret_cast<const T *>(a.xs);
+  const size_t n = a.n;
+
+  for (;;) {
+    const uint32_t t = next_tile(a.tileCounter, lane);
+    if (t >= a.ntiles) break;
+    const uint4 tl = a.tiles[t];
+    Lane<T, D> L;
+    uint32_t lo[KNN_MAX_INDEXED], hi[KNN_MAX_INDEXED], ownLo[KNN_MAX_INDEXED], ownHi[KNN_MAX_INDEXED];
+    uint64_t cellId;
+    tile_setup<T, D, G>(a, g, tl, lane, L, lo, hi, ownLo, ownHi, cellId);
+    const uint64_t ownBytes = own_block_bytes<T, D, G>(a, g, tl, lane, L.active, cellId, ownLo, ownHi);
+    uint32_t size = 0;
+    double topKey = L.Tq;
+    uint32_t topId = 0xffffffffu;
+    T filt = L.F;
+    uint64_t tileCand = 0;
+
+    for_each_range(a, g, lo, hi, [&](uint32_t pBeg, uint32_t pEnd) {
+      tileCand += pEnd - pBeg;
+      for (uint32_t base = pBeg; base < pEnd; base += 64) {
+        const uint32_t pp = base + lane < pEnd ? base + lane : pBeg;
+        T cv[D];
+#pragma unroll
+        for (int j = 0; j < D; j++) cv[j] = xs[(size_t)j * n + pp];
+        const uint32_t cidv = a.sortedIds[pp];
+        const uint32_t m = (pEnd - base) < 64u ? (pEnd - base) : 64u;
+        for (uint32_t jj = 0; jj < m; jj++) {
+          T c[D];
+#pragma unroll
+          for (int j = 0; j < D; j++) c[j] = readlane<T>(cv[j], (int)jj);
+          const uint32_t cid = readlane<uint32_t>(cidv, (int)jj);
+          if (!L.active) continue;
+          const T s = Num<T>::template sq<D>(c, L.qc);
+          if (s > filt) continue;
+          const double key = fp64_key<D, T>(c, L.qd);
+          if (size < K1) {
+            if (key <= L.Tq) {
+              h.push(size, key, cid);
+              if (size == K1) {
+                topKey = h.K(0);
+                topId = h.I(0);
+                filt = Num<T>::bound(topKey);
+              }
+            }
+          } else if (kless(key, cid, topKey, topId)) {
+            h.replace_top(K1, key, cid);
+            topKey = h.K(0);
+            topId = h.I(0);
+            filt = Num<T>::bound(topKey);
+          }
+        }
+      }
+    });
+
+    const bool ok = L.active && size == K1;
+    if (ok) {
+      h.sort(K1);
+      T *od = reinterpret_cast<T *>(a.outDists);
+      for (uint32_t e = 0; e < K1; e++) {
+        const uint32_t id = h.I(e);
+        T c[D];
+#pragma unroll
+        for (int j = 0; j < D; j++) c[j] = x[(size_t)j * n + id];
+        a.outIds[(size_t)L.q * K1 + e] = (int32_t)id;
+        od[(size_t)L.q * K1 + e] = sqrt(Num<T>::template sq<D>(c, L.qc));
+      }
+      a.failed[L.q] = 0;
+    } else if (L.active) {
+      a.failed[L.q] = 1;
+    }
+    tile_accounting<T, D>(a, tl, lane, ok, L.active && !ok, tileCand, ownBytes, K1);
+  }
+}
+
+constexpr uint32_t MAX_LDS_BYTES = 65536;
+constexpr uint32_t HEAP_MAX_GRID = 4096;
+
+template <typename T, int D, typename G>
+void launch_heap(const JoinArgs &a, const G &g, hipStream_t s) {
+  if (a.ntiles == 0) return;
+  const size_t heapBytes = (size_t)a.k1 * 64 * 12;
+  const uint32_t grid = a.ntiles < HEAP_MAX_GRID ? a.ntiles : HEAP_MAX_GRID;
+  if (heapBytes <= MAX_LDS_BYTES)
+    k_range_heap<T, D, true, G><<<grid, 64, heapBytes, s>>>(a, g);
+  else
+    k_range_heap<T, D, false, G><<<grid, 64, 0, s>>>(a, g);
+}
+
+template <typename T, int D, typename G>
+void launch_select(const JoinArgs &a, const G &g, hipStream_t s) {
+  if (a.ntiles == 0) return;
+  k_range_select<T, D, G><<<range_select_grid(a.ntiles), 64, range_select_lds_bytes<T>(a.smax), s>>>(a, g);
+}
+
+}  // namespace
+
+uint32_t range_select_grid(uint32_t ntiles) {
+  return ntiles < SELECT_MAX_GRID ? (ntiles ? ntiles : 1u) : SELECT_MAX_GRID;
+}
+
+template <typename T>
+size_t range_select_lds_bytes(uint32_t smax) {
+  return (size_t)NB * 64 * sizeof(uint32_t) + (size_t)smax * 8 + (size_t)(smax + (smax & 1)) * 4 +
+         (size_t)smax * sizeof(T);
+}
+
+#define DISPATCH_D(FN)                              \
+  switch (a.d) {                                    \
+    case 1: FN<T, 1, G>(a, g, s); break;            \
+    case 2: FN<T, 2, G>(a, g, s); break;            \
+    case 3: FN<T, 3, G>(a, g, s); break;            \
+    case 4: FN<T, 4, G>(a, g, s); break;            \
+    case 5: FN<T, 5, G>(a, g, s); break;            \
+    case 6: FN<T, 6, G>(a, g, s); break;            \
+    case 7: FN<T, 7, G>(a, g, s); break;            \
+    case 8: FN<T, 8, G>(a, g, s); break;            \
+    case 9: FN<T, 9, G>(a, g, s); break;            \
+    case 10: FN<T, 10, G>(a, g, s); break;          \
+    case 11: FN<T, 11, G>(a, g, s); break;          \
+    case 12: FN<T, 12, G>(a, g, s); break;          \
+    case 13: FN<T, 13, G>(a, g, s); break;          \
+    case 14: FN<T, 14, G>(a, g, s); break;          \
+    case 15: FN<T, 15, G>(a, g, s); break;          \
+    case 16: FN<T, 16, G>(a, g, s); break;          \
+    default: break;                                 \
+  }
+
+template <typename T>
+void range_select(const JoinArgs &a, const typename GridOf<T>::type &g, hipStream_t s) {
+  using G = typename GridOf<T>::type;
+  DISPATCH_D(launch_select)
+}
+
+template <typename T>
+void range_topk(const JoinArgs &a, const typename GridOf<T>::type &g, hipStream_t s) {
+  using G = typename GridOf<T>::type;
+  DISPATCH_D(launch_heap)
+}
+
+template <typename T>
+size_t range_heap_scratch_bytes(uint32_t ntiles, uint32_t k1) {
+  const size_t heapBytes = (size_t)k1 * 64 * 12;
+  if (heapBytes <= MAX_LDS_BYTES) return 0;
+  const uint32_t grid = ntiles < HEAP_MAX_GRID ? ntiles : HEAP_MAX_GRID;
+  return heapBytes * grid;
+}
+
+#define INST(T)                                                                          \
+  template void range_select<T>(const JoinArgs &, const GridOf<T>::type &, hipStream_t); \
+  template size_t range_select_lds_bytes<T>(uint32_t);                                   \
+  template void range_topk<T>(const JoinArgs &, const GridOf<T>::type &, hipStream_t);   \
+  template size_t range_heap_scratch_bytes<T>(uint32_t, uint32_t);
+INST(float)
+INST(double)
+#undef INST
+
+}  // namespace knn

@@ -71,6 +71,7 @@ typedef struct {
   double ms_range_kernel;   /* sum of range-kernel durations (HIP events) */
   double ms_range_kernel_max;
   uint64_t algo_bytes_max;  /* algorithmic bytes of the longest range-kernel launch */
+  uint64_t overflow_queries; /* queries re-run by the heap kernel (staging overflow) */
 } knnjoin_stats;
 
 /*

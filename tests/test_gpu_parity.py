@@ -149,6 +149,18 @@ def test_errors_and_edges(oracle):
     assert np.array_equal(ids, bi) and np.array_equal(bits(d), bits(bd))
 
 
+def test_join_heavy_duplicates_use_heap_fallback(oracle):
+    # 3000 copies of one point inside a uniform background: those queries' staged lists
+    # overflow and are finished by the heap kernel
+    rng = np.random.default_rng(12)
+    pts = np.concatenate([np.full((3000, 2), 0.5, np.float32),
+                          rng.uniform(0, 1, (20000, 2)).astype(np.float32)])[rng.permutation(23000)]
+    ids, d, st = _lib.knnjoin_ex(pts, 16, beta=0.025)
+    assert st.overflow_queries > 0
+    rc, i2, d2, _ = oracle.knnjoin(pts, 16, nd=2, beta=0.025)
+    assert np.array_equal(ids, i2) and np.array_equal(bits(d), bits(d2))
+
+
 def test_repeated_calls_reuse_buffers(oracle):
     a = synthetic.uniform_box(50000, seed=8)
     b = synthetic.mixed_box(20000, seed=9)
