@@ -45,7 +45,7 @@ enum Slot {
   S_PTS, S_X, S_XS, S_KEYS, S_KEYS_TMP, S_IDS, S_IDS_TMP, S_CELLKEYS, S_CELLSTART, S_FLAGS, S_POS,
   S_ACT, S_NTILES, S_TILEOFF, S_TILES, S_FAILED, S_ROWSUM, S_OUTIDS, S_OUTDISTS, S_SCRATCH,
   S_HEAP, S_SMALL, S_PART, S_CTR, S_CSUM, S_HOSTFLAGS, S_OVERFLOW, S_STAGE, S_DENSE, S_BFLAGS,
-  S_BPOS, S_BSTART, S_NSLOTS
+  S_BPOS, S_BSTART, S_CELLOF, S_CELLMAX, S_NSLOTS
 };
 
 struct Cache {
@@ -195,6 +195,8 @@ struct Pipeline {
       cellKeys = buf<uint64_t>(S_CELLKEYS, n);
       cellStart = buf<uint32_t>(S_CELLSTART, n + 1);
       compact_cells_u32(keys, flags, pos, n, cellKeys, cellStart, s);
+      cellOf = buf<uint32_t>(S_CELLOF, n);
+      cell_of(flags, pos, n, cellOf, s);
       HIP_CHECK(hipMemcpyAsync(&ncell, pos + n, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
     } else {
       uint64_t *keys = buf<uint64_t>(S_KEYS, n);
@@ -208,6 +210,8 @@ struct Pipeline {
       cellKeys = buf<uint64_t>(S_CELLKEYS, n);
       cellStart = buf<uint32_t>(S_CELLSTART, n + 1);
       compact_cells_u64(keys, flags, pos, n, cellKeys, cellStart, s);
+      cellOf = buf<uint32_t>(S_CELLOF, n);
+      cell_of(flags, pos, n, cellOf, s);
       HIP_CHECK(hipMemcpyAsync(&ncell, pos + n, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
     }
     sortedIds = ids;
@@ -233,14 +237,19 @@ struct Pipeline {
 
   static constexpr uint64_t kDenseMaxCells = 1ull << 27;
   const uint32_t *denseStart = nullptr;
+  uint32_t *cellOf = nullptr;  // compacted cell index of each sorted position
 
   // One range pass (kernel.cu:61-225 + GPU.cu:3610-3704 + main.cu:1192-1223): queries with
   // activeFailed[q] != 0 (all when null), grouped by cell into tiles of <= 64.
-  PassResult pass(const uint8_t *activeFailed, uint8_t *failed, int32_t *dIds, T *dDists,
-                  double *rowSum, unsigned long long *ctr, uint32_t *failCount, Timer &tK);
-  uint32_t make_tiles(const uint8_t *activeFailed, uint32_t *&act, uint4 *&tiles);
+  PassResult pass(const uint8_t *activeFailed, uint32_t nq, uint8_t *failed, int32_t *dIds,
+                  T *dDists, unsigned long long *ctr, Timer &tK);
+  uint32_t make_tiles(const uint8_t *activeFailed, uint32_t *&act, uint4 *&tiles, uint8_t *failed,
+                      unsigned long long *cellAcc);
+  void finish(PassResult &r, uint32_t *counters, unsigned long long *ctr, uint32_t prefailed);
+  unsigned long long accBytes = 0;  // accumulated pre-check accounting bytes
   const T *radius = nullptr;  // per-query radii (learned schedule), null = eps
   uint32_t lastSpan = 1;
+  uint32_t lastActive = 0;   // queries left in tiles after the pre-check
 };
 
 template <typename T>
@@ -350,12 +359,24 @@ struct Timer {
 // Builds tiles of <= 64 active queries per cell: activeFailed[q] != 0 selects q (all when
 // null).  Returns the tile count; act/tiles point into the cache.
 template <typename T>
-uint32_t Pipeline<T>::make_tiles(const uint8_t *activeFailed, uint32_t *&act, uint4 *&tiles) {
+uint32_t Pipeline<T>::make_tiles(const uint8_t *activeFailed, uint32_t *&act, uint4 *&tiles,
+                                 uint8_t *failed, unsigned long long *cellAcc) {
   uint32_t *flags = buf<uint32_t>(S_FLAGS, n + 1);
   uint32_t *pos = buf<uint32_t>(S_POS, n + 1);
-  active_flags(sortedIds, activeFailed, n, flags, s);
+  bool all = !activeFailed;
+  if (denseStart && failed) {
+    // own-block pre-check (exact early failures) + per-cell candidate accounting
+    uint32_t *cellMax = buf<uint32_t>(S_CELLMAX, (size_t)ncell + 1);
+    fill_u32(cellMax, 0u, ncell, s);
+    prefilter<T>(x, n, g, radius, denseStart, sortedIds, activeFailed, failed, K1, cellOf, cellMax,
+                 flags, s);
+    if (cellAcc) cell_bytes(cellMax, ncell, cellAcc, s);
+    all = false;
+  } else {
+    active_flags(sortedIds, activeFailed, n, flags, s);
+  }
   scan_exclusive_u32(flags, pos, n, scratch, s);
-  if (!activeFailed) {
+  if (all) {
     act = sortedIds;
   } else {
     act = buf<uint32_t>(S_ACT, n);
@@ -369,6 +390,7 @@ uint32_t Pipeline<T>::make_tiles(const uint8_t *activeFailed, uint32_t *&act, ui
   uint32_t A = 0;
   HIP_CHECK(hipMemcpyAsync(&A, pos + n, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
   HIP_CHECK(hipStreamSynchronize(s));
+  lastActive = A;
   const uint32_t spanMax = nd == 1 ? 4096u : (nd == 2 ? 256u : 8u);
   uint32_t span = 1;
   if (A && ncell) {
@@ -406,17 +428,18 @@ uint32_t Pipeline<T>::make_tiles(const uint8_t *activeFailed, uint32_t *&act, ui
 }
 
 template <typename T>
-PassResult Pipeline<T>::pass(const uint8_t *activeFailed, uint8_t *failed, int32_t *dIds,
-                             T *dDists, double *rowSum, unsigned long long *ctr,
-                             uint32_t *failCount, Timer &tK) {
+PassResult Pipeline<T>::pass(const uint8_t *activeFailed, uint32_t nq, uint8_t *failed,
+                             int32_t *dIds, T *dDists, unsigned long long *ctr, Timer &tK) {
   // ctr layout (u64 words): [0] pairs, [1] algorithmic bytes, [2] fail|overflow counters,
-  // [3] tile queue head
+  // [3] tile queue head, [4..5] pre-check accounting (candidate points, query cells)
   uint32_t *counters = reinterpret_cast<uint32_t *>(ctr + 2);
   uint32_t *overflowCount = counters + 1;
   uint32_t *tileCounter = counters + 2;
   uint32_t *act;
   uint4 *tiles;
-  const uint32_t NT = make_tiles(activeFailed, act, tiles);
+  fill_u32(reinterpret_cast<uint32_t *>(ctr + 4), 0u, 4, s);
+  const uint32_t NT = make_tiles(activeFailed, act, tiles, failed, ctr + 4);
+  const uint32_t prefailed = nq > lastActive ? nq - lastActive : 0u;
   uint8_t *overflow = buf<uint8_t>(S_OVERFLOW, n);
   fill_u8(overflow, 0, n, s);
   fill_u32(counters, 0u, 4, s);
@@ -457,12 +480,8 @@ PassResult Pipeline<T>::pass(const uint8_t *activeFailed, uint8_t *failed, int32
     tK.start();
     range_topk<T>(a, g, s);
     r.kernel_ms = tK.stop();
-    r.noverflow = NT ? n : 0;
-    unsigned long long hctr[2];
-    HIP_CHECK(hipMemcpyAsync(&r.nfail, counters, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-    HIP_CHECK(hipMemcpyAsync(hctr, ctr, sizeof(hctr), hipMemcpyDeviceToHost, s));
-    HIP_CHECK(hipStreamSynchronize(s));
-    r.bytes_total = hctr[1];
+    r.noverflow = NT ? lastActive : 0;
+    finish(r, counters, ctr, prefailed);
     return r;
   }
   tK.start();
@@ -476,7 +495,7 @@ PassResult Pipeline<T>::pass(const uint8_t *activeFailed, uint8_t *failed, int32
     // queries whose candidates overflow the staging slots: per-lane heap kernel
     uint32_t *act2;
     uint4 *tiles2;
-    const uint32_t NT2 = make_tiles(overflow, act2, tiles2);
+    const uint32_t NT2 = make_tiles(overflow, act2, tiles2, nullptr, nullptr);
     fill_u32(tileCounter, 0u, 1, s);
     a.act = act2;
     a.tiles = tiles2;
@@ -487,13 +506,25 @@ PassResult Pipeline<T>::pass(const uint8_t *activeFailed, uint8_t *failed, int32
     range_topk<T>(a, g, s);
     r.kernel_ms += tK.stop();
   }
-  unsigned long long hctr[2];
-  HIP_CHECK(hipMemcpyAsync(&r.nfail, counters, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  finish(r, counters, ctr, prefailed);
+  return r;
+}
+
+// failures = pre-check failures + kernel failures; algorithmic bytes += the pre-check's
+// per-query-cell candidate blocks ((4d+4) bytes per candidate, 8 bytes per scanned cell)
+template <typename T>
+void Pipeline<T>::finish(PassResult &r, uint32_t *counters, unsigned long long *ctr,
+                         uint32_t prefailed) {
+  unsigned long long hctr[6];
+  uint32_t kf = 0;
+  HIP_CHECK(hipMemcpyAsync(&kf, counters, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
   HIP_CHECK(hipMemcpyAsync(hctr, ctr, sizeof(hctr), hipMemcpyDeviceToHost, s));
   HIP_CHECK(hipStreamSynchronize(s));
-  r.bytes_total = hctr[1];
-  (void)failCount;
-  return r;
+  r.nfail = kf + prefailed;
+  uint64_t cellsScanned = 1;
+  for (uint32_t j = 0; j < nd; j++) cellsScanned *= 3;
+  accBytes += hctr[4] * (sizeof(T) * d + 4ull) + hctr[5] * cellsScanned * 8ull;
+  r.bytes_total = hctr[1] + accBytes;
 }
 
 namespace {
@@ -609,7 +640,6 @@ static int run_join(const T *pts_in, uint32_t n, uint32_t d, uint32_t k, const k
   stats.ms_index = tPh.stop();
 
   unsigned long long *ctr = P.template buf<unsigned long long>(S_CTR, 64);
-  uint32_t *failCount = reinterpret_cast<uint32_t *>(ctr + 4);
   fill_u32(reinterpret_cast<uint32_t *>(ctr), 0u, 16, s);
   uint32_t nq = n, numIter = 0, reindexes = 0, launches = 0;
   double frac = 0;
@@ -639,7 +669,7 @@ static int run_join(const T *pts_in, uint32_t n, uint32_t d, uint32_t k, const k
       }
     }
     tPh.start();
-    PassResult pr = P.pass(first ? nullptr : failed, failed, dIds, dDists, rowSum, ctr, failCount, tK);
+    PassResult pr = P.pass(first ? nullptr : failed, nq, failed, dIds, dDists, ctr, tK);
     launches++;
     stats.overflow_queries += pr.noverflow;
     kMs += pr.kernel_ms;
@@ -676,7 +706,7 @@ static int run_join(const T *pts_in, uint32_t n, uint32_t d, uint32_t k, const k
   stats.ms_total = tAll.stop();
   stats.checksum = hcs;
   stats.candidates = hctr[0];
-  stats.algo_bytes = hctr[1];
+  stats.algo_bytes = hctr[1] + P.accBytes;
   stats.eps_final = (float)eps;
   stats.iterations = numIter;
   stats.reindexes = reindexes;
@@ -947,7 +977,7 @@ int knnjoin_stage_join_pass(const float *points, uint32_t n, uint32_t ndim, uint
     unsigned long long *ctr = P.buf<unsigned long long>(S_CTR, 64);
     fill_u32(reinterpret_cast<uint32_t *>(ctr), 0u, 16, P.s);
     Timer tK(P.s);
-    PassResult pr = P.pass(dAct, dFailed, dIds, dD, rowSum, ctr, reinterpret_cast<uint32_t *>(ctr + 4), tK);
+    PassResult pr = P.pass(dAct, nq, dFailed, dIds, dD, ctr, tK);
     HIP_CHECK(hipMemcpyAsync(out_ids, dIds, sizeof(int32_t) * (size_t)n * K1, hipMemcpyDeviceToHost, P.s));
     HIP_CHECK(hipMemcpyAsync(out_dists, dD, sizeof(float) * (size_t)n * K1, hipMemcpyDeviceToHost, P.s));
     HIP_CHECK(hipMemcpyAsync(failed, dFailed, n, hipMemcpyDeviceToHost, P.s));

@@ -81,6 +81,86 @@ __global__ void k_compact_blocks(const uint32_t *cellStart, const uint32_t *flag
   if (c == ncell - 1) blockStart[pos[ncell]] = n;
 }
 
+// Per active query (sorted position p): the number of points in its own cell block (radius
+// r = its own or the grid's eps, cells [coord(x-r) rounded down, coord(x+r) rounded up]) from
+// the dense cell-start table.  Fewer than K+1 points there means the query cannot succeed
+// at this radius: it is marked failed without a distance evaluation.  The per-cell maximum of
+// those counts feeds the algorithmic-bytes accounting (each query cell's block once).
+template <typename T, typename G>
+__global__ void k_prefilter(const T *__restrict__ x, uint32_t n, G g, const T *radius,
+                            const uint32_t *__restrict__ dense, const uint32_t *__restrict__ sortedIds,
+                            const uint8_t *activeSrc, uint8_t *failed, uint32_t k1,
+                            const uint32_t *cellOf, uint32_t *cellMax, uint32_t *flags) {
+  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const uint32_t q = sortedIds[p];
+  if (activeSrc && !activeSrc[q]) {
+    flags[p] = 0;
+    return;
+  }
+  const T r = radius ? radius[q] : (T)g.eps;
+  const T eps = (T)g.eps;
+  uint32_t lo[KNN_MAX_INDEXED], hi[KNN_MAX_INDEXED], cur[KNN_MAX_INDEXED];
+  for (uint32_t j = 0; j < g.nd; j++) {
+    const T xj = x[(size_t)j * n + q];
+    lo[j] = cell_coord<T>(next_down(xj - r), (T)g.minArr[j], eps, g.nCells[j]);
+    hi[j] = cell_coord<T>(next_up(xj + r), (T)g.minArr[j], eps, g.nCells[j]);
+    cur[j] = lo[j];
+  }
+  uint32_t cnt = 0;
+  for (;;) {
+    uint64_t rowBase = 0;
+    for (uint32_t j = 1; j < g.nd; j++) rowBase += (uint64_t)cur[j] * g.mult[j];
+    cnt += dense[rowBase + hi[0] + 1] - dense[rowBase + lo[0]];
+    uint32_t j = 1;
+    while (j < g.nd) {
+      if (cur[j] < hi[j]) {
+        cur[j]++;
+        break;
+      }
+      cur[j] = lo[j];
+      j++;
+    }
+    if (j >= g.nd) break;
+  }
+  if (cellMax) atomicMax(&cellMax[cellOf[p]], cnt);
+  if (cnt < k1) {
+    failed[q] = 1;
+    flags[p] = 0;
+  } else {
+    flags[p] = 1;
+  }
+}
+
+// sum over cells of cellMax (candidate points of each active query cell's block) and the
+// number of active query cells; out[0] += points, out[1] += cells
+__global__ void k_cell_bytes(const uint32_t *cellMax, uint32_t ncell, unsigned long long *out) {
+  __shared__ unsigned long long sp[256], sc[256];
+  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t v = c < ncell ? cellMax[c] : 0u;
+  sp[threadIdx.x] = v;
+  sc[threadIdx.x] = v ? 1ull : 0ull;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) {
+      sp[threadIdx.x] += sp[threadIdx.x + o];
+      sc[threadIdx.x] += sc[threadIdx.x + o];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    atomicAdd(out, sp[0]);
+    atomicAdd(out + 1, sc[0]);
+  }
+}
+
+// cellOf[p] = compacted cell index of sorted position p (inclusive scan of boundary flags - 1)
+__global__ void k_cell_of(const uint32_t *flags, const uint32_t *pos, uint32_t n, uint32_t *cellOf) {
+  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  cellOf[p] = pos[p] + flags[p] - 1;
+}
+
 __global__ void k_active_flags(const uint32_t *sortedIds, const uint8_t *failed, uint32_t n,
                                uint32_t *flags) {
   const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
@@ -189,6 +269,22 @@ void compact_blocks(const uint32_t *cellStart, const uint32_t *flags, const uint
   if (ncell) k_compact_blocks<<<(ncell + 255) / 256, 256, 0, s>>>(cellStart, flags, pos, ncell, n, blockStart);
 }
 
+template <typename T>
+void prefilter(const T *x, uint32_t n, const typename GridOf<T>::type &g, const T *radius,
+               const uint32_t *dense, const uint32_t *sortedIds, const uint8_t *activeSrc,
+               uint8_t *failed, uint32_t k1, const uint32_t *cellOf, uint32_t *cellMax,
+               uint32_t *flags, hipStream_t s) {
+  if (n)
+    k_prefilter<T, typename GridOf<T>::type><<<(n + 255) / 256, 256, 0, s>>>(
+        x, n, g, radius, dense, sortedIds, activeSrc, failed, k1, cellOf, cellMax, flags);
+}
+void cell_bytes(const uint32_t *cellMax, uint32_t ncell, unsigned long long *out, hipStream_t s) {
+  if (ncell) k_cell_bytes<<<(ncell + 255) / 256, 256, 0, s>>>(cellMax, ncell, out);
+}
+void cell_of(const uint32_t *flags, const uint32_t *pos, uint32_t n, uint32_t *cellOf, hipStream_t s) {
+  if (n) k_cell_of<<<(n + 255) / 256, 256, 0, s>>>(flags, pos, n, cellOf);
+}
+
 void active_flags(const uint32_t *sortedIds, const uint8_t *failed, uint32_t n, uint32_t *flags,
                   hipStream_t s) {
   if (n) k_active_flags<<<(n + 255) / 256, 256, 0, s>>>(sortedIds, failed, n, flags);
@@ -223,7 +319,10 @@ void sum_rows(const double *rowSum, uint32_t n, double *partial, double *out, hi
   template void cell_keys_u64<T>(const T *, uint32_t, const GridOf<T>::type &, uint64_t *,      \
                                  uint32_t *, hipStream_t);                                     \
   template void gather_soa<T>(const T *, const uint32_t *, uint32_t, uint32_t, T *, hipStream_t); \
-  template void row_sums<T>(const T *, uint32_t, uint32_t, double *, hipStream_t);
+  template void row_sums<T>(const T *, uint32_t, uint32_t, double *, hipStream_t);          \
+  template void prefilter<T>(const T *, uint32_t, const GridOf<T>::type &, const T *,         \
+                             const uint32_t *, const uint32_t *, const uint8_t *, uint8_t *,  \
+                             uint32_t, const uint32_t *, uint32_t *, uint32_t *, hipStream_t);
 INST(float)
 INST(double)
 #undef INST

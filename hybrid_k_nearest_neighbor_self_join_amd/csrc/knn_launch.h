@@ -88,6 +88,17 @@ void block_flags(const uint64_t *cellKeys, uint32_t ncell, uint32_t ncell0, uint
 void compact_blocks(const uint32_t *cellStart, const uint32_t *flags, const uint32_t *pos,
                     uint32_t ncell, uint32_t n, uint32_t *blockStart, hipStream_t s);
 
+// Active flags per sorted position with the own-block pre-check: queries whose own cell
+// block holds fewer than k1 points get failed[q] = 1 and flag 0.  activeSrc null = all
+// active.  cellOf/cellMax (optional) collect per-cell candidate counts for accounting.
+template <typename T>
+void prefilter(const T *x, uint32_t n, const typename GridOf<T>::type &g, const T *radius,
+               const uint32_t *dense, const uint32_t *sortedIds, const uint8_t *activeSrc,
+               uint8_t *failed, uint32_t k1, const uint32_t *cellOf, uint32_t *cellMax,
+               uint32_t *flags, hipStream_t s);
+void cell_bytes(const uint32_t *cellMax, uint32_t ncell, unsigned long long *out, hipStream_t s);
+void cell_of(const uint32_t *flags, const uint32_t *pos, uint32_t n, uint32_t *cellOf, hipStream_t s);
+
 // active-query flags per sorted position (all = 1 when failed == NULL)
 void active_flags(const uint32_t *sortedIds, const uint8_t *failed, uint32_t n, uint32_t *flags,
                   hipStream_t s);

@@ -156,60 +156,6 @@ __device__ __forceinline__ void tile_setup(const JoinArgs &a, const G &g, const 
   }
 }
 
-// Compulsory-bytes bookkeeping: a query cell's block of candidates is counted once, by the
-// cell's first active query in the active list.
-template <typename T, int D, typename G>
-__device__ __forceinline__ uint64_t own_block_bytes(const JoinArgs &a, const G &g,
-                                                    const uint4 &tl, uint32_t lane, bool active,
-                                                    uint64_t cellId, const uint32_t *ownLo,
-                                                    const uint32_t *ownHi) {
-  // previous active query's cell: lane-1, or for lane 0 the query before the tile
-  uint64_t prev = ((uint64_t)(uint32_t)__shfl((int)(uint32_t)(cellId >> 32), (int)lane - 1, 64) << 32) |
-                  (uint32_t)__shfl((int)(uint32_t)cellId, (int)lane - 1, 64);
-  bool first = active;
-  if (lane == 0) {
-    if (tl.y > 0) {
-      const T *__restrict__ x = reinterpret_cast<const T *>(a.x);
-      const uint32_t pq = a.act[tl.y - 1];
-      uint64_t pid = 0;
-      for (uint32_t j = 0; j < g.nd; j++)
-        pid += (uint64_t)cell_coord<T>(x[(size_t)j * a.n + pq], (T)g.minArr[j], (T)g.eps, g.nCells[j]) * g.mult[j];
-      first = first && pid != cellId;
-    }
-  } else {
-    first = first && prev != cellId;
-  }
-  if (!first) return 0;
-  uint64_t cand = 0, rows = 0;
-  uint32_t cur[KNN_MAX_INDEXED];
-  for (uint32_t j = 0; j < g.nd; j++) cur[j] = ownLo[j];
-  for (;;) {
-    uint64_t rowBase = 0;
-    for (uint32_t j = 1; j < g.nd; j++) rowBase += (uint64_t)cur[j] * g.mult[j];
-    uint32_t pBeg, pEnd;
-    if (a.denseStart) {
-      pBeg = a.denseStart[rowBase + ownLo[0]];
-      pEnd = a.denseStart[rowBase + ownHi[0] + 1];
-    } else {
-      pBeg = a.cellStart[lower_bound_u64(a.cellKeys, a.ncell, rowBase + ownLo[0])];
-      pEnd = a.cellStart[upper_bound_u64(a.cellKeys, a.ncell, rowBase + ownHi[0])];
-    }
-    cand += pEnd - pBeg;
-    rows++;
-    uint32_t j = 1;
-    while (j < g.nd) {
-      if (cur[j] < ownHi[j]) {
-        cur[j]++;
-        break;
-      }
-      cur[j] = ownLo[j];
-      j++;
-    }
-    if (j >= g.nd) break;
-  }
-  return cand * (sizeof(T) * D + 4ull) + rows * 8ull;
-}
-
 // Calls f(pBeg, pEnd) for each row of the block [lo, hi] holding points (the arguments and the
 // calls are uniform across the wave when lo/hi are).
 template <typename G, typename F>
@@ -253,16 +199,15 @@ __device__ __forceinline__ uint32_t next_tile(uint32_t *counter, uint32_t lane) 
 template <typename T, int D>
 __device__ __forceinline__ void tile_accounting(const JoinArgs &a, const uint4 &tl, uint32_t lane,
                                                 bool ok, bool failedLane, uint64_t tileCand,
-                                                uint64_t ownBytes, uint32_t K1) {
+                                                uint32_t K1) {
   const uint64_t fm = __ballot(failedLane);
   const uint64_t om = __ballot(ok);
-  // compulsory HBM bytes: each query cell's candidate block once, query ids in / failure
-  // flags out, and the finished rows
-  for (int o = 32; o > 0; o >>= 1) ownBytes += __shfl_xor(ownBytes, o, 64);
   if (lane == 0) {
     if (fm) atomicAdd(a.failCount, (uint32_t)__popcll(fm));
+    // compulsory HBM bytes of the tile's queries: ids in / failure flags out and the finished
+    // rows (the query cells' candidate blocks are counted by the pre-filter)
     unsigned long long b = (unsigned long long)tl.z * 8ull +
-                           (unsigned long long)__popcll(om) * K1 * (4ull + sizeof(T)) + ownBytes;
+                           (unsigned long long)__popcll(om) * K1 * (4ull + sizeof(T));
     atomicAdd(a.candCount + 1, b);
     atomicAdd(a.candCount, (unsigned long long)tileCand * tl.z);
   }
@@ -282,14 +227,18 @@ __device__ __forceinline__ uint32_t bucket_of(T s, T scale) {
   return b < (uint32_t)NB ? b : (uint32_t)NB - 1;
 }
 
-// LDS: hist[NB][64] u32 (per-lane histogram columns, then slot cursors) followed by the
-// per-query staging arrays key[SM] (double), id[SM] (u32), s[SM] (T).
+// LDS: hist[NB/2][64] u32, two 16-bit bucket counters per word (per-lane histogram
+// columns, then slot cursors), followed by the per-query staging arrays key[SM] (double),
+// id[SM] (u32), s[SM] (T).  A lane with 65536 or more candidates inside F goes to the heap
+// kernel, so no 16-bit counter or cursor can carry into its neighbour.
+__device__ __forceinline__ uint32_t hword(uint32_t b, uint32_t col) { return (b >> 1) * 64 + col; }
+__device__ __forceinline__ uint32_t hshift(uint32_t b) { return (b & 1u) << 4; }
 template <typename T, int D, typename G>
 __global__ __launch_bounds__(64) void k_range_select(JoinArgs a, G g) {
   extern __shared__ __align__(16) unsigned char smem[];
   uint32_t *hist = reinterpret_cast<uint32_t *>(smem);
   const uint32_t SM = a.smax;
-  double *skey = reinterpret_cast<double *>(smem + NB * 64 * sizeof(uint32_t));
+  double *skey = reinterpret_cast<double *>(smem + (NB / 2) * 64 * sizeof(uint32_t));
   uint32_t *sid = reinterpret_cast<uint32_t *>(skey + SM);
   T *ss = reinterpret_cast<T *>(sid + SM + (SM & 1));
   const uint32_t lane = threadIdx.x;
@@ -306,12 +255,12 @@ __global__ __launch_bounds__(64) void k_range_select(JoinArgs a, G g) {
     uint32_t lo[KNN_MAX_INDEXED], hi[KNN_MAX_INDEXED], ownLo[KNN_MAX_INDEXED], ownHi[KNN_MAX_INDEXED];
     uint64_t cellId;
     tile_setup<T, D, G>(a, g, tl, lane, L, lo, hi, ownLo, ownHi, cellId);
-    const uint64_t ownBytes = own_block_bytes<T, D, G>(a, g, tl, lane, L.active, cellId, ownLo, ownHi);
     const T scale = (T)NB / L.F;
     __syncthreads();  // the previous tile's readers of hist/staging are done
 #pragma unroll 8
-    for (int b = 0; b < NB; b++) hist[b * 64 + lane] = 0u;
+    for (int w = 0; w < NB / 2; w++) hist[w * 64 + lane] = 0u;
     uint64_t tileCand = 0;
+    uint32_t inF = 0;
 
     // ---- pass 1 (lane = query): histogram of s over [0, F] for all queries of the tile
     for_each_range(a, g, lo, hi, [&](uint32_t pBeg, uint32_t pEnd) {
@@ -327,20 +276,30 @@ __global__ __launch_bounds__(64) void k_range_select(JoinArgs a, G g) {
 #pragma unroll
           for (int j = 0; j < D; j++) c[j] = readlane<T>(cv[j], (int)jj);
           const T s = Num<T>::template sq<D>(c, L.qc);
-          if (L.active && s <= L.F) atomicAdd(&hist[bucket_of<T>(s, scale) * 64 + lane], 1u);
+          if (L.active && s <= L.F) {
+            const uint32_t b = bucket_of<T>(s, scale);
+            atomicAdd(&hist[hword(b, lane)], 1u << hshift(b));
+            inF++;
+          }
         }
       }
     });
 
-    // ---- scan: b* per query; counts become exclusive slot offsets
+    // ---- scan: b* per query; counts become exclusive slot offsets (saturated past b*)
+    const bool big = inF >= 65536u;
     uint32_t cum = 0, bstar = NB;
-    for (int b = 0; b < NB; b++) {
-      const uint32_t h = hist[b * 64 + lane];
-      hist[b * 64 + lane] = cum;
-      if (bstar == NB && cum + h >= K1) bstar = (uint32_t)b;
-      cum += h;
+    for (int w = 0; w < NB / 2; w++) {
+      const uint32_t hw = hist[w * 64 + lane];
+      const uint32_t h0 = hw & 0xffffu, h1 = hw >> 16;
+      const uint32_t o0 = cum < 0xffffu ? cum : 0xffffu;
+      if (bstar == NB && cum + h0 >= K1) bstar = (uint32_t)(2 * w);
+      cum += h0;
+      const uint32_t o1 = cum < 0xffffu ? cum : 0xffffu;
+      if (bstar == NB && cum + h1 >= K1) bstar = (uint32_t)(2 * w + 1);
+      cum += h1;
+      hist[w * 64 + lane] = o0 | (o1 << 16);
     }
-    const bool go = L.active && bstar < (uint32_t)NB;
+    const bool go = L.active && !big && bstar < (uint32_t)NB;
     const T edge = go ? ((T)(bstar + 1) / scale) * Num<T>::margin : (T)0;
     bool okMine = false, ovfMine = false;
     const uint64_t goMask = __ballot(go);
@@ -388,7 +347,7 @@ __global__ __launch_bounds__(64) void k_range_select(JoinArgs a, G g) {
               uint32_t b = bucket_of<T>(s, sci);
               if (b <= bsi || s <= edi) {
                 b = b < bsi ? b : bsi;
-                const uint32_t slot = atomicAdd(&hist[b * 64 + i], 1u);
+                const uint32_t slot = (atomicAdd(&hist[hword(b, i)], 1u << hshift(b)) >> hshift(b)) & 0xffffu;
                 if (slot < SM) {
                   skey[slot] = fp64_key<D, T>(c, qdi);
                   sid[slot] = a.sortedIds[p];
@@ -402,7 +361,7 @@ __global__ __launch_bounds__(64) void k_range_select(JoinArgs a, G g) {
         }
       });
       __syncthreads();
-      const uint32_t m = hist[bsi * 64 + i];
+      const uint32_t m = (hist[hword(bsi, i)] >> hshift(bsi)) & 0xffffu;
       if (__ballot(ovf) || m > SM) {
         if (lane == i) ovfMine = true;
         __syncthreads();
@@ -423,8 +382,8 @@ __global__ __launch_bounds__(64) void k_range_select(JoinArgs a, G g) {
           const uint32_t it = sid[tI];
           uint32_t b = bucket_of<T>(ss[tI], sci);
           b = b < bsi ? b : bsi;
-          const uint32_t w0 = b >= 2 ? hist[(b - 2) * 64 + i] : 0u;
-          const uint32_t w1 = b + 1 < bsi ? hist[(b + 1) * 64 + i] : m;
+          const uint32_t w0 = b >= 2 ? (hist[hword(b - 2, i)] >> hshift(b - 2)) & 0xffffu : 0u;
+          const uint32_t w1 = b + 1 < bsi ? (hist[hword(b + 1, i)] >> hshift(b + 1)) & 0xffffu : m;
           uint32_t r = w0;
           for (uint32_t v = w0; v < w1; v++) r += kless(skey[v], sid[v], kt, it) ? 1u : 0u;
           rk[u] = r;
@@ -447,6 +406,7 @@ __global__ __launch_bounds__(64) void k_range_select(JoinArgs a, G g) {
       __syncthreads();
     }
 
+    ovfMine = ovfMine || (L.active && big);
     if (L.active) {
       if (ovfMine) {
         a.overflow[L.q] = 1;
@@ -456,7 +416,7 @@ __global__ __launch_bounds__(64) void k_range_select(JoinArgs a, G g) {
     }
     const uint64_t om = __ballot(L.active && ovfMine);
     if (lane == 0 && om) atomicAdd(a.overflowCount, (uint32_t)__popcll(om));
-    tile_accounting<T, D>(a, tl, lane, okMine, L.active && !okMine && !ovfMine, tileCand, ownBytes, K1);
+    tile_accounting<T, D>(a, tl, lane, okMine, L.active && !okMine && !ovfMine, tileCand, K1);
   }
 }
 
@@ -551,7 +511,6 @@ __global__ __launch_bounds__(64) void k_range_heap(JoinArgs a, G g) {
     uint32_t lo[KNN_MAX_INDEXED], hi[KNN_MAX_INDEXED], ownLo[KNN_MAX_INDEXED], ownHi[KNN_MAX_INDEXED];
     uint64_t cellId;
     tile_setup<T, D, G>(a, g, tl, lane, L, lo, hi, ownLo, ownHi, cellId);
-    const uint64_t ownBytes = own_block_bytes<T, D, G>(a, g, tl, lane, L.active, cellId, ownLo, ownHi);
     uint32_t size = 0;
     double topKey = L.Tq;
     uint32_t topId = 0xffffffffu;
@@ -611,7 +570,7 @@ __global__ __launch_bounds__(64) void k_range_heap(JoinArgs a, G g) {
     } else if (L.active) {
       a.failed[L.q] = 1;
     }
-    tile_accounting<T, D>(a, tl, lane, ok, L.active && !ok, tileCand, ownBytes, K1);
+    tile_accounting<T, D>(a, tl, lane, ok, L.active && !ok, tileCand, K1);
   }
 }
 
@@ -643,8 +602,8 @@ uint32_t range_select_grid(uint32_t ntiles) {
 
 template <typename T>
 size_t range_select_lds_bytes(uint32_t smax) {
-  return (size_t)NB * 64 * sizeof(uint32_t) + (size_t)smax * 8 + (size_t)(smax + (smax & 1)) * 4 +
-         (size_t)smax * sizeof(T);
+  return (size_t)(NB / 2) * 64 * sizeof(uint32_t) + (size_t)smax * 8 +
+         (size_t)(smax + (smax & 1)) * 4 + (size_t)smax * sizeof(T);
 }
 
 #define DISPATCH_D(FN)                              \
