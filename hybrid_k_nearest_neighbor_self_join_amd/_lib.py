@@ -31,7 +31,10 @@ EXPORTED_SYMBOLS = [
     "KNNJoinPy", "KNNJoinPyDouble", "knnjoin_ex", "knnjoin_default_params",
     "knnjoin_last_status", "knnjoin_last_error", "knnjoin_release", "knnjoin_stage_reorder",
     "knnjoin_stage_estimate_eps", "knnjoin_stage_index", "knnjoin_stage_join_pass",
+    "knnjoin_set_radius_model", "knnjoin_radius_features", "knnjoin_mlp_forward",
+    "knnjoin_mlp_grad", "knnjoin_mlp_train",
 ]
+MLP_NW = 5057
 
 
 class KnnParams(ctypes.Structure):
@@ -127,9 +130,16 @@ def _setup(L):
     L.knnjoin_stage_index.restype = i32
     L.knnjoin_stage_join_pass.argtypes = [vp, u32, u32, u32, u32, flt, vp, u32, vp, vp, vp, vp]
     L.knnjoin_stage_join_pass.restype = i32
-    for name in ("knnjoin_learned_train", "knnjoin_learned_set_model", "knnjoin_batched"):
-        if hasattr(L, name):
-            pass
+    L.knnjoin_set_radius_model.argtypes = [u32, vp, u32, flt]
+    L.knnjoin_set_radius_model.restype = i32
+    L.knnjoin_radius_features.argtypes = [vp, u32, u32, u32, u32, i32, vp, vp, vp]
+    L.knnjoin_radius_features.restype = i32
+    L.knnjoin_mlp_forward.argtypes = [vp, vp, u32, vp]
+    L.knnjoin_mlp_forward.restype = i32
+    L.knnjoin_mlp_grad.argtypes = [vp, vp, vp, u32, vp, vp]
+    L.knnjoin_mlp_grad.restype = i32
+    L.knnjoin_mlp_train.argtypes = [vp, vp, vp, u32, u32, u32, flt, u64, vp]
+    L.knnjoin_mlp_train.restype = i32
 
 
 def p(a) -> ctypes.c_void_p | None:
@@ -223,3 +233,49 @@ def stage_join_pass(points: np.ndarray, nd: int, k: int, eps: float, queries: np
     check(lib().knnjoin_stage_join_pass(p(pts), n, d, nd, k, eps, p(q), len(q), p(ids), p(dists),
                                         p(failed), p(nf)))
     return ids, dists, failed, int(nf[0])
+
+
+# ------------------------------------------------------------------ learned radius
+def set_radius_model(nd: int, weights: np.ndarray, margin: float):
+    w = np.ascontiguousarray(weights, np.float32).reshape(-1)
+    check(lib().knnjoin_set_radius_model(nd, p(w), len(w), margin))
+
+
+def radius_features(points: np.ndarray, k: int, nd: int | None = None, reorder: int = 1,
+                    want_radius: bool = False):
+    pts = np.ascontiguousarray(points, np.float32)
+    n, d = pts.shape
+    nd = min(d, 6) if nd is None else nd
+    feats = np.zeros((n, 12), np.float32)
+    wf = np.zeros(1, np.float32)
+    rad = np.zeros(n, np.float32) if want_radius else None
+    check(lib().knnjoin_radius_features(p(pts), n, d, k, nd, reorder, p(feats), p(wf), p(rad)))
+    return (feats, float(wf[0]), rad) if want_radius else (feats, float(wf[0]))
+
+
+def mlp_forward(weights: np.ndarray, X: np.ndarray) -> np.ndarray:
+    w = np.ascontiguousarray(weights, np.float32)
+    X = np.ascontiguousarray(X, np.float32)
+    out = np.zeros(len(X), np.float32)
+    check(lib().knnjoin_mlp_forward(p(w), p(X), len(X), p(out)))
+    return out
+
+
+def mlp_grad(weights: np.ndarray, X: np.ndarray, Y: np.ndarray):
+    w = np.ascontiguousarray(weights, np.float32)
+    X = np.ascontiguousarray(X, np.float32)
+    Y = np.ascontiguousarray(Y, np.float32)
+    g = np.zeros(MLP_NW, np.float32)
+    loss = np.zeros(1, np.float32)
+    check(lib().knnjoin_mlp_grad(p(w), p(X), p(Y), len(X), p(g), p(loss)))
+    return g, float(loss[0])
+
+
+def mlp_train(weights: np.ndarray, X: np.ndarray, Y: np.ndarray, epochs: int, batch: int,
+              lr: float, seed: int):
+    w = np.array(weights, np.float32, copy=True)
+    X = np.ascontiguousarray(X, np.float32)
+    Y = np.ascontiguousarray(Y, np.float32)
+    losses = np.zeros(epochs, np.float32)
+    check(lib().knnjoin_mlp_train(p(w), p(X), p(Y), len(X), epochs, batch, lr, seed, p(losses)))
+    return w, losses

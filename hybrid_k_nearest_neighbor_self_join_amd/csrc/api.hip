@@ -45,7 +45,7 @@ enum Slot {
   S_PTS, S_X, S_XS, S_KEYS, S_KEYS_TMP, S_IDS, S_IDS_TMP, S_CELLKEYS, S_CELLSTART, S_FLAGS, S_POS,
   S_ACT, S_NTILES, S_TILEOFF, S_TILES, S_FAILED, S_ROWSUM, S_OUTIDS, S_OUTDISTS, S_SCRATCH,
   S_HEAP, S_SMALL, S_PART, S_CTR, S_CSUM, S_HOSTFLAGS, S_OVERFLOW, S_STAGE, S_DENSE, S_BFLAGS,
-  S_BPOS, S_BSTART, S_CELLOF, S_CELLMAX, S_NSLOTS
+  S_BPOS, S_BSTART, S_CELLOF, S_CELLMAX, S_FCNT, S_RADIUS, S_RHIST, S_MODEL, S_TRAIN, S_NSLOTS
 };
 
 struct Cache {
@@ -144,6 +144,44 @@ static int bits_for(uint64_t totalCells) {
     m >>= 1;
   }
   return b;
+}
+
+// ------------------------------------------------------------------ learned-radius model
+struct RadiusModel {
+  std::vector<float> w;
+  float margin = 0.0f;
+  bool ok = false;
+};
+static RadiusModel g_models[KNN_MAX_INDEXED + 1];  // by number of indexed dims
+
+// Feature grid of the learned radius: width wf = (V (K+1) / N)^(1/nd), V = product of the
+// indexed extents (an extent of 0 counts as 1), doubled until the grid has <= 2^27 cells.
+template <typename T>
+struct FeatureGrid {
+  typename GridOf<T>::type g;
+  uint64_t cells;
+  T wf;
+  float logk1, logdens;
+};
+
+template <typename T>
+static void feature_grid(const T *mn, const T *mx, uint32_t nd, uint32_t n, uint32_t k1,
+                         FeatureGrid<T> &F) {
+  double V = 1.0;
+  for (uint32_t j = 0; j < nd; j++) {
+    const double e = (double)(mx[j] - mn[j]);
+    V *= e > 0.0 ? e : 1.0;
+  }
+  T wf = (T)std::pow(V * (double)k1 / (double)n, 1.0 / (double)nd);
+  if (!(wf > (T)0)) wf = (T)1;
+  for (;;) {
+    grid_dims<T>(mn, mx, nd, wf, F.g, F.cells);
+    if (F.cells <= (1ull << 27)) break;
+    wf = wf * (T)2;
+  }
+  F.wf = wf;
+  F.logk1 = logf((float)k1);
+  F.logdens = (float)std::log((double)n / V);
 }
 
 // ------------------------------------------------------------------ pipeline
@@ -250,6 +288,7 @@ struct Pipeline {
   const T *radius = nullptr;  // per-query radii (learned schedule), null = eps
   uint32_t lastSpan = 1;
   uint32_t lastActive = 0;   // queries left in tiles after the pre-check
+  double avgReach = 1.0;     // typical search radius in cells (tile-span cost model)
 };
 
 template <typename T>
@@ -398,7 +437,7 @@ uint32_t Pipeline<T>::make_tiles(const uint8_t *activeFailed, uint32_t *&act, ui
     double best = 1e300;
     for (uint32_t w = 1; w <= spanMax; w *= 2) {
       const double q = std::max(1.0, std::min(64.0, w * rhoA));
-      const double cost = (w + 2.0) / q;
+      const double cost = (w + 2.0 * avgReach) / q;
       if (cost < best * 0.97) {
         best = cost;
         span = w;
@@ -538,6 +577,13 @@ __global__ void k_fill_failed_rows(const uint8_t *failed, uint32_t n, uint32_t k
 }  // namespace
 
 template <typename T>
+static void run_plain(Pipeline<T> &P, double beta, knnjoin_stats &stats, uint8_t *failed,
+                      int32_t *dIds, T *dDists, Timer &tPh, Timer &tK);
+template <typename T>
+static void run_learned(Pipeline<T> &P, const knnjoin_params &p, knnjoin_stats &stats,
+                        uint8_t *failed, int32_t *dIds, T *dDists, Timer &tPh, Timer &tK);
+
+template <typename T>
 static int run_join(const T *pts_in, uint32_t n, uint32_t d, uint32_t k, const knnjoin_params &p,
                     int32_t *outIds, T *outDists, knnjoin_stats *st) {
   knnjoin_stats stats{};
@@ -611,12 +657,49 @@ static int run_join(const T *pts_in, uint32_t n, uint32_t d, uint32_t k, const k
   stats.ms_reorder = tPh.stop();
   for (uint32_t j = 0; j < 16; j++) stats.perm[j] = j < d ? perm[j] : 0;
 
+  if (p.mode == KNN_MODE_LEARNED) {
+    run_learned<T>(P, p, stats, failed, dIds, dDists, tPh, tK);
+  } else {
+    run_plain<T>(P, beta, stats, failed, dIds, dDists, tPh, tK);
+  }
+  // ---- checksum (main.cu:452)
+  double *part = P.template buf<double>(S_CSUM, 64 + (n + 4095) / 4096 + 16);
+  double *csum = part + (n + 4095) / 4096 + 8;
+  row_sums<T>(dDists, n, K1, rowSum, s);
+  sum_rows(rowSum, n, part, csum, s);
+  double hcs = 0;
+  HIP_CHECK(hipMemcpyAsync(&hcs, csum, sizeof(double), hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipStreamSynchronize(s));
+  stats.ms_total = tAll.stop();
+  stats.checksum = hcs;
+  stats.failed_final = 0;
+
+  if (!devptr) {
+    auto t0 = std::chrono::steady_clock::now();
+    HIP_CHECK(hipMemcpyAsync(outIds, dIds, sizeof(int32_t) * (size_t)n * K1, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipMemcpyAsync(outDists, dDists, sizeof(T) * (size_t)n * K1, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    stats.ms_other = h2d_ms + std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  }
+  if (P.verbose)
+    printf("[knn] done: iterations %u reindexes %u eps0 %.9g eps %.9g checksum %.6f total %.3f ms\n",
+           stats.iterations, stats.reindexes, (double)stats.eps0, (double)stats.eps_final, hcs,
+           stats.ms_total);
+  if (st) *st = stats;
+  return KNN_OK;
+}
+
+// The plain method: BETA eps0 estimate and the failed-query loop of main.cu:305-430.
+template <typename T>
+static void run_plain(Pipeline<T> &P, double beta, knnjoin_stats &stats, uint8_t *failed,
+                      int32_t *dIds, T *dDists, Timer &tPh, Timer &tK) {
+  hipStream_t s = P.s;
+  const uint32_t n = P.n, d = P.d, nd = P.nd, k = P.k, K1 = P.K1;
   auto fail_all = [&](int code, const std::string &msg) {
     const size_t tot = (size_t)n * K1;
     k_fill_failed_rows<<<(unsigned)((tot + 255) / 256), 256, 0, s>>>(failed, n, K1, dIds, 1);
     throw Error(code, msg);
   };
-
   // ---- epsilon estimate (GPU.cu:3088-3566)
   tPh.start();
   EpsResult er = estimate_eps<T>(P.x, n, d, k, beta, nullptr, s);
@@ -692,39 +775,148 @@ static int run_join(const T *pts_in, uint32_t n, uint32_t d, uint32_t k, const k
   stats.ms_range_kernel = kMs;
   stats.ms_range_kernel_max = kMax;
   stats.algo_bytes_max = bytesAtMax;
-
-  // ---- checksum (main.cu:452)
-  double *part = P.template buf<double>(S_CSUM, 64 + (n + 4095) / 4096 + 16);
-  double *csum = part + (n + 4095) / 4096 + 8;
-  row_sums<T>(dDists, n, K1, rowSum, s);
-  sum_rows(rowSum, n, part, csum, s);
-  double hcs = 0;
   unsigned long long hctr[2];
-  HIP_CHECK(hipMemcpyAsync(&hcs, csum, sizeof(double), hipMemcpyDeviceToHost, s));
   HIP_CHECK(hipMemcpyAsync(hctr, ctr, sizeof(hctr), hipMemcpyDeviceToHost, s));
   HIP_CHECK(hipStreamSynchronize(s));
-  stats.ms_total = tAll.stop();
-  stats.checksum = hcs;
   stats.candidates = hctr[0];
   stats.algo_bytes = hctr[1] + P.accBytes;
   stats.eps_final = (float)eps;
   stats.iterations = numIter;
   stats.reindexes = reindexes;
   stats.kernel_launches = launches;
-  stats.failed_final = 0;
+}
 
-  if (!devptr) {
-    auto t0 = std::chrono::steady_clock::now();
-    HIP_CHECK(hipMemcpyAsync(outIds, dIds, sizeof(int32_t) * (size_t)n * K1, hipMemcpyDeviceToHost, s));
-    HIP_CHECK(hipMemcpyAsync(outDists, dDists, sizeof(T) * (size_t)n * K1, hipMemcpyDeviceToHost, s));
-    HIP_CHECK(hipStreamSynchronize(s));
-    stats.ms_other = h2d_ms + std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+// The learned schedule: per-query initial radius from the MLP, passes on one fine grid, and
+// the expansion loop (radius x2 for the failed queries, a coarser grid when the reach grows
+// past 48 cells) until every query has its K+1 neighbours.
+template <typename T>
+static void run_learned(Pipeline<T> &P, const knnjoin_params &p, knnjoin_stats &stats,
+                        uint8_t *failed, int32_t *dIds, T *dDists, Timer &tPh, Timer &tK) {
+  (void)p;
+  hipStream_t s = P.s;
+  const uint32_t n = P.n, nd = P.nd, K1 = P.K1;
+  const RadiusModel &M = g_models[nd];
+  if (!M.ok) throw Error(KNN_ERR_MODEL, "no learned-radius model registered for this number of indexed dims");
+  // min/max of the indexed dims
+  T *mm = P.template buf<T>(S_PART, 2 * KNN_MAX_INDEXED + minmax_scratch_elems(n, nd));
+  minmax_soa<T>(P.x, n, nd, mm, mm + KNN_MAX_INDEXED, mm + 2 * KNN_MAX_INDEXED, s);
+  T hmn[KNN_MAX_INDEXED], hmx[KNN_MAX_INDEXED];
+  HIP_CHECK(hipMemcpyAsync(hmn, mm, sizeof(T) * nd, hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipMemcpyAsync(hmx, mm + KNN_MAX_INDEXED, sizeof(T) * nd, hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipStreamSynchronize(s));
+
+  // ---- features + MLP -> per-query radius
+  tPh.start();
+  FeatureGrid<T> F;
+  feature_grid<T>(hmn, hmx, nd, n, K1, F);
+  uint32_t *cnt = P.template buf<uint32_t>(S_FCNT, (size_t)F.cells);
+  fill_u32(cnt, 0u, (size_t)F.cells, s);
+  feature_counts<T>(P.x, n, F.g, cnt, s);
+  float *Wd = P.template buf<float>(S_MODEL, MLP_NW);
+  HIP_CHECK(hipMemcpyAsync(Wd, M.w.data(), sizeof(float) * MLP_NW, hipMemcpyHostToDevice, s));
+  T *radius = P.template buf<T>(S_RADIUS, n);
+  uint32_t *rh = P.template buf<uint32_t>(S_RHIST, 64);
+  fill_u32(rh, 0u, 64, s);
+  radius_kernel<T>(P.x, n, F.g, cnt, F.logk1, F.logdens, Wd, M.margin, (float)F.wf, radius, nullptr, rh, s);
+  uint32_t hh[64];
+  HIP_CHECK(hipMemcpyAsync(hh, rh, sizeof(hh), hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipStreamSynchronize(s));
+  stats.ms_eps = tPh.stop();
+
+  // ---- join grid: cell width near the low end of the radius distribution (5th percentile
+  // for <= 2 indexed dims, the median above, where each extra cell of reach multiplies the
+  // rows), but no finer than 1/24 of the 95th percentile
+  auto quantile = [&](double qf, bool upper) {
+    uint64_t c = 0;
+    for (int b = 0; b < 64; b++) {
+      c += hh[b];
+      if ((double)c >= qf * n) return (double)F.wf * std::exp2(((upper ? b + 1 : b) - 32) / 4.0);
+    }
+    return (double)F.wf * std::exp2(8.0);
+  };
+  const double rlo = quantile(nd <= 2 ? 0.05 : 0.5, false);
+  const double rmed = quantile(0.5, true);
+  const double rhi = quantile(0.95, true);
+  T wj = (T)std::max(rlo, rhi / 24.0);
+  if (!(wj > (T)0)) wj = F.wf;
+  for (;;) {
+    typename GridOf<T>::type gtest;
+    uint64_t tc;
+    grid_dims<T>(hmn, hmx, nd, wj, gtest, tc);
+    if (tc <= (1ull << 27) || nd > 2) break;
+    wj = wj * (T)1.25;
   }
-  if (P.verbose)
-    printf("[knn] done: iterations %u reindexes %u eps0 %.9g eps %.9g checksum %.6f total %.3f ms\n",
-           numIter, reindexes, (double)eps0, (double)eps, hcs, stats.ms_total);
-  if (st) *st = stats;
-  return KNN_OK;
+  P.avgReach = std::max(1.0, rmed / (double)wj);
+  stats.eps0 = (float)wj;
+  tPh.start();
+  P.build_index(hmn, hmx, wj);
+  stats.ms_index = tPh.stop();
+  P.radius = radius;
+
+  unsigned long long *ctr = P.template buf<unsigned long long>(S_CTR, 64);
+  fill_u32(reinterpret_cast<uint32_t *>(ctr), 0u, 16, s);
+  uint64_t *maxr = reinterpret_cast<uint64_t *>(ctr + 8);
+  uint32_t nq = n, passes = 0, reindexes = 0;
+  bool first = true;
+  double joinMs = 0, kMs = 0, kMax = 0;
+  uint64_t bytesAtMax = 0, prevBytes = 0;
+  while (nq != 0) {
+    if (!first) {
+      grow_radius<T>(radius, failed, n, (T)2, s);
+      max_failed_radius(radius, failed, n, maxr, s);
+      uint64_t mb = 0;
+      HIP_CHECK(hipMemcpyAsync(&mb, maxr, sizeof(mb), hipMemcpyDeviceToHost, s));
+      HIP_CHECK(hipStreamSynchronize(s));
+      double rmax;
+      if (sizeof(T) == 4) {
+        uint32_t b32 = (uint32_t)mb;
+        float f;
+        std::memcpy(&f, &b32, 4);
+        rmax = f;
+      } else {
+        double dd;
+        std::memcpy(&dd, &mb, 8);
+        rmax = dd;
+      }
+      if (rmax / (double)P.g.eps > 48.0) {
+        tPh.start();
+        P.build_index(hmn, hmx, (T)(rmax / 8.0));
+        stats.ms_index += tPh.stop();
+        reindexes++;
+      }
+      P.avgReach = std::max(1.0, rmax / (double)P.g.eps);
+    }
+    tPh.start();
+    PassResult pr = P.pass(first ? nullptr : failed, nq, failed, dIds, dDists, ctr, tK);
+    stats.overflow_queries += pr.noverflow;
+    kMs += pr.kernel_ms;
+    if (pr.kernel_ms > kMax) {
+      kMax = pr.kernel_ms;
+      bytesAtMax = pr.bytes_total - prevBytes;
+    }
+    prevBytes = pr.bytes_total;
+    joinMs += tPh.stop();
+    if (P.verbose)
+      printf("[knn] learned pass %u grid %.9g queries %u failed %u tiles %u cells %u span %u overflow %u kernel %.3f ms\n",
+             passes, (double)P.g.eps, nq, pr.nfail, pr.ntiles, P.ncell, P.lastSpan, pr.noverflow, pr.kernel_ms);
+    nq = pr.nfail;
+    passes++;
+    first = false;
+  }
+  stats.ms_join = joinMs;
+  stats.ms_range_kernel = kMs;
+  stats.ms_range_kernel_max = kMax;
+  stats.algo_bytes_max = bytesAtMax;
+  unsigned long long hctr[2];
+  HIP_CHECK(hipMemcpyAsync(hctr, ctr, sizeof(hctr), hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipStreamSynchronize(s));
+  stats.candidates = hctr[0];
+  stats.algo_bytes = hctr[1] + P.accBytes;
+  stats.eps_final = (float)P.g.eps;
+  stats.iterations = passes;
+  stats.reindexes = reindexes;
+  stats.kernel_launches = passes;
+  P.radius = nullptr;
 }
 
 template <typename T>
@@ -983,6 +1175,186 @@ int knnjoin_stage_join_pass(const float *points, uint32_t n, uint32_t ndim, uint
     HIP_CHECK(hipMemcpyAsync(failed, dFailed, n, hipMemcpyDeviceToHost, P.s));
     HIP_CHECK(hipStreamSynchronize(P.s));
     if (num_failed) *num_failed = pr.nfail;
+    return KNN_OK;
+  });
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ learned-radius entries
+extern "C" {
+
+int knnjoin_set_radius_model(uint32_t nd, const float *weights, uint32_t nweights, float margin) {
+  return knn::guarded([&]() -> int {
+    using namespace knn;
+    if (nd == 0 || nd > KNN_MAX_INDEXED) throw Error(KNN_ERR_ARG, "nd must be in 1..8");
+    if (!weights || nweights != (uint32_t)MLP_NW) throw Error(KNN_ERR_MODEL, "radius model must have 5057 weights");
+    g_models[nd].w.assign(weights, weights + MLP_NW);
+    g_models[nd].margin = margin;
+    g_models[nd].ok = true;
+    return KNN_OK;
+  });
+}
+
+int knnjoin_radius_features(const float *points, uint32_t n, uint32_t ndim, uint32_t k, uint32_t nd,
+                            int reorder, float *features, float *wf_out, float *radius_out) {
+  return knn::guarded([&]() -> int {
+    using namespace knn;
+    if (!points || n == 0 || ndim == 0 || ndim > KNN_MAXDIM || nd == 0 || nd > ndim || nd > KNN_MAX_INDEXED)
+      throw Error(KNN_ERR_ARG, "bad feature arguments");
+    Pipeline<float> P;
+    P.s = lib_stream();
+    P.n = n;
+    P.d = ndim;
+    P.nd = nd;
+    P.k = k;
+    P.K1 = k + 1;
+    hipStream_t s = P.s;
+    float *b = P.buf<float>(S_PTS, (size_t)n * ndim);
+    HIP_CHECK(hipMemcpyAsync(b, points, sizeof(float) * (size_t)n * ndim, hipMemcpyHostToDevice, s));
+    uint32_t perm[KNN_MAXDIM];
+    for (uint32_t j = 0; j < KNN_MAXDIM; j++) perm[j] = j;
+    if (reorder) {
+      float *var = P.buf<float>(S_SMALL, 16 * 1024);
+      variance_rows<float>(b, n, ndim, var, s);
+      float hv[KNN_MAXDIM];
+      HIP_CHECK(hipMemcpyAsync(hv, var, sizeof(float) * ndim, hipMemcpyDeviceToHost, s));
+      HIP_CHECK(hipStreamSynchronize(s));
+      sort_perm_desc<float>(hv, ndim, perm);
+    }
+    P.x = P.buf<float>(S_X, (size_t)n * ndim);
+    permute_to_soa<float>(b, n, ndim, perm, P.x, s);
+    float *mm = P.buf<float>(S_PART, 2 * KNN_MAX_INDEXED + minmax_scratch_elems(n, nd));
+    minmax_soa<float>(P.x, n, nd, mm, mm + KNN_MAX_INDEXED, mm + 2 * KNN_MAX_INDEXED, s);
+    float hmn[KNN_MAX_INDEXED], hmx[KNN_MAX_INDEXED];
+    HIP_CHECK(hipMemcpyAsync(hmn, mm, sizeof(float) * nd, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipMemcpyAsync(hmx, mm + KNN_MAX_INDEXED, sizeof(float) * nd, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    FeatureGrid<float> F;
+    feature_grid<float>(hmn, hmx, nd, n, k + 1, F);
+    uint32_t *cnt = P.buf<uint32_t>(S_FCNT, (size_t)F.cells);
+    fill_u32(cnt, 0u, (size_t)F.cells, s);
+    feature_counts<float>(P.x, n, F.g, cnt, s);
+    float *fd = features ? P.buf<float>(S_TRAIN, (size_t)n * MLP_IN) : nullptr;
+    float *rd = nullptr;
+    float *Wd = nullptr;
+    const RadiusModel &M = g_models[nd];
+    if (radius_out) {
+      if (!M.ok) throw Error(KNN_ERR_MODEL, "no radius model registered for nd");
+      rd = P.buf<float>(S_RADIUS, n);
+      Wd = P.buf<float>(S_MODEL, MLP_NW);
+      HIP_CHECK(hipMemcpyAsync(Wd, M.w.data(), sizeof(float) * MLP_NW, hipMemcpyHostToDevice, s));
+    }
+    radius_kernel<float>(P.x, n, F.g, cnt, F.logk1, F.logdens, Wd, M.ok ? M.margin : 0.0f,
+                         (float)F.wf, rd, fd, nullptr, s);
+    if (features)
+      HIP_CHECK(hipMemcpyAsync(features, fd, sizeof(float) * (size_t)n * MLP_IN, hipMemcpyDeviceToHost, s));
+    if (radius_out)
+      HIP_CHECK(hipMemcpyAsync(radius_out, rd, sizeof(float) * n, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    if (wf_out) *wf_out = (float)F.wf;
+    return KNN_OK;
+  });
+}
+
+int knnjoin_mlp_forward(const float *weights, const float *X, uint32_t B, float *out) {
+  return knn::guarded([&]() -> int {
+    using namespace knn;
+    if (!weights || !X || !out) throw Error(KNN_ERR_ARG, "null argument");
+    hipStream_t s = lib_stream();
+    float *buf = reinterpret_cast<float *>(g_cache.get(S_TRAIN, sizeof(float) * ((size_t)MLP_NW + (size_t)B * (MLP_IN + 2 * MLP_H + 1))));
+    float *Wd = buf, *Xd = Wd + MLP_NW, *H1 = Xd + (size_t)B * MLP_IN, *H2 = H1 + (size_t)B * MLP_H, *o = H2 + (size_t)B * MLP_H;
+    HIP_CHECK(hipMemcpyAsync(Wd, weights, sizeof(float) * MLP_NW, hipMemcpyHostToDevice, s));
+    HIP_CHECK(hipMemcpyAsync(Xd, X, sizeof(float) * (size_t)B * MLP_IN, hipMemcpyHostToDevice, s));
+    mlp_forward_batch(Wd, Xd, B, H1, H2, o, s);
+    HIP_CHECK(hipMemcpyAsync(out, o, sizeof(float) * B, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    return KNN_OK;
+  });
+}
+
+int knnjoin_mlp_grad(const float *weights, const float *X, const float *Y, uint32_t B, float *grads,
+                     float *loss) {
+  return knn::guarded([&]() -> int {
+    using namespace knn;
+    if (!weights || !X || !Y || !grads) throw Error(KNN_ERR_ARG, "null argument");
+    hipStream_t s = lib_stream();
+    const size_t words = (size_t)2 * MLP_NW + 1 + (size_t)B * (MLP_IN + 1 + 4 * MLP_H + 2);
+    float *buf = reinterpret_cast<float *>(g_cache.get(S_TRAIN, sizeof(float) * words));
+    float *Wd = buf, *Gd = Wd + MLP_NW, *Ld = Gd + MLP_NW, *Xd = Ld + 1, *Yd = Xd + (size_t)B * MLP_IN;
+    float *H1 = Yd + B, *H2 = H1 + (size_t)B * MLP_H, *dH1 = H2 + (size_t)B * MLP_H, *dH2 = dH1 + (size_t)B * MLP_H;
+    float *o = dH2 + (size_t)B * MLP_H, *dout = o + B;
+    HIP_CHECK(hipMemcpyAsync(Wd, weights, sizeof(float) * MLP_NW, hipMemcpyHostToDevice, s));
+    HIP_CHECK(hipMemcpyAsync(Xd, X, sizeof(float) * (size_t)B * MLP_IN, hipMemcpyHostToDevice, s));
+    HIP_CHECK(hipMemcpyAsync(Yd, Y, sizeof(float) * B, hipMemcpyHostToDevice, s));
+    mlp_grad_batch(Wd, Xd, Yd, B, H1, H2, o, dout, dH1, dH2, Gd, Ld, s);
+    HIP_CHECK(hipMemcpyAsync(grads, Gd, sizeof(float) * MLP_NW, hipMemcpyDeviceToHost, s));
+    if (loss) HIP_CHECK(hipMemcpyAsync(loss, Ld, sizeof(float), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    return KNN_OK;
+  });
+}
+
+// Adam over mini-batches of a shuffled epoch order (splitmix64 Fisher-Yates, seeded); the
+// weights are updated in place, loss_per_epoch[e] = mean batch loss of epoch e.
+int knnjoin_mlp_train(float *weights, const float *X, const float *Y, uint32_t n, uint32_t epochs,
+                      uint32_t batch, float lr, uint64_t seed, float *loss_per_epoch) {
+  return knn::guarded([&]() -> int {
+    using namespace knn;
+    if (!weights || !X || !Y || n == 0 || batch == 0) throw Error(KNN_ERR_ARG, "bad training arguments");
+    hipStream_t s = lib_stream();
+    const uint32_t B = std::min(batch, n);
+    const size_t words = (size_t)4 * MLP_NW + 1 + (size_t)n * (MLP_IN + 1) +
+                         (size_t)B * (MLP_IN + 1 + 4 * MLP_H + 2) + n;
+    float *buf = reinterpret_cast<float *>(g_cache.get(S_TRAIN, sizeof(float) * words));
+    float *Wd = buf, *Gd = Wd + MLP_NW, *Md = Gd + MLP_NW, *Vd = Md + MLP_NW, *Ld = Vd + MLP_NW;
+    float *Xall = Ld + 1, *Yall = Xall + (size_t)n * MLP_IN;
+    float *Xb = Yall + n, *Yb = Xb + (size_t)B * MLP_IN;
+    float *H1 = Yb + B, *H2 = H1 + (size_t)B * MLP_H, *dH1 = H2 + (size_t)B * MLP_H, *dH2 = dH1 + (size_t)B * MLP_H;
+    float *o = dH2 + (size_t)B * MLP_H, *dout = o + B;
+    uint32_t *idx = reinterpret_cast<uint32_t *>(dout + B);
+    HIP_CHECK(hipMemcpyAsync(Wd, weights, sizeof(float) * MLP_NW, hipMemcpyHostToDevice, s));
+    HIP_CHECK(hipMemcpyAsync(Xall, X, sizeof(float) * (size_t)n * MLP_IN, hipMemcpyHostToDevice, s));
+    HIP_CHECK(hipMemcpyAsync(Yall, Y, sizeof(float) * n, hipMemcpyHostToDevice, s));
+    fill_u32(reinterpret_cast<uint32_t *>(Md), 0u, 2 * (size_t)MLP_NW, s);
+    std::vector<uint32_t> order(n);
+    for (uint32_t i = 0; i < n; i++) order[i] = i;
+    uint64_t st = seed;
+    auto next = [&]() {
+      uint64_t z = (st += 0x9E3779B97F4A7C15ull);
+      z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+      z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+      return z ^ (z >> 31);
+    };
+    const float b1 = 0.9f, b2 = 0.999f, epsA = 1e-8f;
+    uint64_t step = 0;
+    const uint32_t nb = n / B;
+    for (uint32_t e = 0; e < epochs; e++) {
+      for (uint32_t i = n - 1; i > 0; i--) {
+        const uint32_t j = (uint32_t)(next() % (uint64_t)(i + 1));
+        std::swap(order[i], order[j]);
+      }
+      HIP_CHECK(hipMemcpyAsync(idx, order.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice, s));
+      double lsum = 0;
+      for (uint32_t bi = 0; bi < nb; bi++) {
+        gather_rows(Xall, idx + (size_t)bi * B, B, MLP_IN, Xb, s);
+        gather_rows(Yall, idx + (size_t)bi * B, B, 1, Yb, s);
+        mlp_grad_batch(Wd, Xb, Yb, B, H1, H2, o, dout, dH1, dH2, Gd, Ld, s);
+        step++;
+        const float c1 = (float)(1.0 - std::pow(0.9, (double)step));
+        const float c2 = (float)(1.0 - std::pow(0.999, (double)step));
+        adam_step(Wd, Gd, Md, Vd, lr, b1, b2, c1, c2, epsA, s);
+        float hl = 0;
+        if (loss_per_epoch) {
+          HIP_CHECK(hipMemcpyAsync(&hl, Ld, sizeof(float), hipMemcpyDeviceToHost, s));
+          HIP_CHECK(hipStreamSynchronize(s));
+          lsum += hl;
+        }
+      }
+      if (loss_per_epoch) loss_per_epoch[e] = nb ? (float)(lsum / nb) : 0.0f;
+    }
+    HIP_CHECK(hipMemcpyAsync(weights, Wd, sizeof(float) * MLP_NW, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
     return KNN_OK;
   });
 }

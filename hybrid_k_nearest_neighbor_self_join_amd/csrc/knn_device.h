@@ -14,6 +14,10 @@
 #define KNN_MAXDIM 16
 #define KNN_MAX_INDEXED 8
 #define KNN_WAVE 64
+// learned-radius MLP 12 -> 64 -> 64 -> 1
+#define MLP_IN 12
+#define MLP_H 64
+#define MLP_NW (MLP_H * MLP_IN + MLP_H + MLP_H * MLP_H + MLP_H + MLP_H + 1)
 
 namespace knn {
 

@@ -152,6 +152,31 @@ size_t range_heap_scratch_bytes(uint32_t ntiles, uint32_t k1);
 // per-row sums of (T)(d*d) over the K+1 entries of each row in entry order (main.cu:452)
 template <typename T>
 void row_sums(const T *dists, uint32_t n, uint32_t k1, double *rowSum, hipStream_t s);
+// ------------------------------------------------------------- learned radius (learned.hip)
+// per-cell point counts of the feature grid (cnt zeroed by the caller, totalCells entries)
+template <typename T>
+void feature_counts(const T *x, uint32_t n, const typename GridOf<T>::type &g, uint32_t *cnt,
+                    hipStream_t s);
+// features (featOut, n x 12, optional) and radius = wf * exp(mlp + margin) (radius, optional),
+// with a 64-bin histogram of 4*log2(radius/wf)+32 (rhist, optional, zeroed by the caller)
+template <typename T>
+void radius_kernel(const T *x, uint32_t n, const typename GridOf<T>::type &g, const uint32_t *cnt,
+                   float logk1, float logdens, const float *W, float margin, float wf, T *radius,
+                   float *featOut, uint32_t *rhist, hipStream_t s);
+template <typename T>
+void grow_radius(T *radius, const uint8_t *failed, uint32_t n, T factor, hipStream_t s);
+// largest radius among failed queries into out (8 bytes: float bits or double bits)
+void max_failed_radius(const float *radius, const uint8_t *failed, uint32_t n, void *out, hipStream_t s);
+void max_failed_radius(const double *radius, const uint8_t *failed, uint32_t n, void *out, hipStream_t s);
+void gather_rows(const float *in, const uint32_t *idx, uint32_t B, uint32_t w, float *out, hipStream_t s);
+void mlp_forward_batch(const float *W, const float *X, uint32_t B, float *H1, float *H2, float *out,
+                       hipStream_t s);
+void mlp_grad_batch(const float *W, const float *X, const float *Y, uint32_t B, float *H1,
+                    float *H2, float *out, float *dout, float *dH1, float *dH2, float *G,
+                    float *loss, hipStream_t s);
+void adam_step(float *W, const float *G, float *M, float *V, float lr, float b1, float b2, float c1,
+               float c2, float eps, hipStream_t s);
+
 // deterministic sum of rowSum over the rows (chunks of 4096 rows, then chunk sums): out[0]
 void sum_rows(const double *rowSum, uint32_t n, double *partial, double *out, hipStream_t s);
 

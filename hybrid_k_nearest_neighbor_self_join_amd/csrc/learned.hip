@@ -1,0 +1,321 @@
+// learned.hip -- learned per-query initial radius: features, MLP 12->64->64->1, training.
+//
+// The radius only changes the epsilon schedule of the join (which radius each query is first
+// tried at); every query's row is still decided by the exact range kernel, and a query that
+// fails at its radius goes through the expansion loop, so no result depends on the model.
+//
+// Features of a query (12), on the "feature grid" of width wf = (V * (K+1) / N)^(1/nd) over
+// the data's bounding box (V = its volume in the indexed dims: a cell of the grid would hold
+// K+1 points of a uniform set):
+//   f0      log1p(count of the query's own cell)
+//   f1..f9  log1p(counts of the 3x3 cells around it in indexed dims 0 and 1, dim 1 outer,
+//           dim 0 inner, other indexed dims at the query's own coordinate; for nd == 1 the 9
+//           cells c-4..c+4 of dim 0); cells outside the grid count 0
+//   f10     log(K+1)
+//   f11     log(N / V)  (global density)
+// Target: log(r_K1 / wf), r_K1 = distance to the (K+1)-th nearest point (self included).
+// Radius: r = wf * exp(mlp(f) + margin), margin calibrated on the training residuals.
+//
+// MLP: h1 = relu(W1 f + b1), h2 = relu(W2 h1 + b2), y = w3 . h2 + b3, FP32 with fmaf in
+// index order (so the CPU oracle, using C's fmaf in the same order, reproduces it bitwise).
+// Weights, flat: W1[64][12], b1[64], W2[64][64], b2[64], w3[64], b3.
+#include "knn_launch.h"
+
+namespace knn {
+
+namespace {
+
+__device__ __forceinline__ float relu(float v) { return v > 0.0f ? v : 0.0f; }
+
+// Forward pass of one sample; W = flat weights (LDS).  Layer 1 unit j is computed and
+// immediately folded into the 64 layer-2 accumulators, so no register array is indexed by a
+// loop variable (the summation order per unit is unchanged: bias first, inputs ascending).
+// H1/H2 (optional) receive the activations.
+__device__ __forceinline__ float mlp_forward(const float *W, const float *f, float *H1, float *H2) {
+  const float *W1 = W, *b1 = W + 768, *W2 = W + 832, *b2 = W + 4928, *w3 = W + 4992;
+  float acc[MLP_H];
+#pragma unroll
+  for (int i = 0; i < MLP_H; i++) acc[i] = b2[i];
+  for (int j = 0; j < MLP_H; j++) {
+    float a = b1[j];
+#pragma unroll
+    for (int t = 0; t < MLP_IN; t++) a = fmaf(W1[j * MLP_IN + t], f[t], a);
+    const float h = relu(a);
+    if (H1) H1[j] = h;
+#pragma unroll
+    for (int i = 0; i < MLP_H; i++) acc[i] = fmaf(W2[i * MLP_H + j], h, acc[i]);
+  }
+  float y = W[5056];
+#pragma unroll
+  for (int i = 0; i < MLP_H; i++) {
+    const float h = relu(acc[i]);
+    if (H2) H2[i] = h;
+    y = fmaf(w3[i], h, y);
+  }
+  return y;
+}
+
+template <typename T, typename G>
+__global__ void k_feature_counts(const T *__restrict__ x, uint32_t n, G g, uint32_t *cnt) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint64_t id = 0;
+  for (uint32_t j = 0; j < g.nd; j++)
+    id += (uint64_t)cell_coord<T>(x[(size_t)j * n + i], (T)g.minArr[j], (T)g.eps, g.nCells[j]) * g.mult[j];
+  atomicAdd(&cnt[id], 1u);
+}
+
+template <typename T, typename G>
+__device__ __forceinline__ void features_of(const T *__restrict__ x, uint32_t n, uint32_t q,
+                                            const G &g, const uint32_t *__restrict__ cnt,
+                                            float logk1, float logdens, float *f) {
+  int64_t c[KNN_MAX_INDEXED];
+  uint64_t own = 0;
+  for (uint32_t j = 0; j < g.nd; j++) {
+    c[j] = cell_coord<T>(x[(size_t)j * n + q], (T)g.minArr[j], (T)g.eps, g.nCells[j]);
+    own += (uint64_t)c[j] * g.mult[j];
+  }
+  f[0] = log1pf((float)cnt[own]);
+  int k = 1;
+  for (int dy = -1; dy <= 1; dy++) {
+    for (int dx = -1; dx <= 1; dx++) {
+      int64_t c0, c1 = 0;
+      if (g.nd == 1) {
+        c0 = c[0] + (dy + 1) * 3 + dx - 4;  // c-4 .. c+4
+      } else {
+        c0 = c[0] + dx;
+        c1 = c[1] + dy;
+      }
+      float v = 0.0f;
+      const bool in0 = c0 >= 0 && c0 < (int64_t)g.nCells[0];
+      const bool in1 = g.nd == 1 || (c1 >= 0 && c1 < (int64_t)g.nCells[1]);
+      if (in0 && in1) {
+        uint64_t id = own - (uint64_t)c[0] * g.mult[0] + (uint64_t)c0 * g.mult[0];
+        if (g.nd > 1) id = id - (uint64_t)c[1] * g.mult[1] + (uint64_t)c1 * g.mult[1];
+        v = log1pf((float)cnt[id]);
+      }
+      f[k++] = v;
+    }
+  }
+  f[10] = logk1;
+  f[11] = logdens;
+}
+
+// features + MLP + radius for every query; optional feature output (training) and a
+// 64-bin histogram of 4*log2(r/wf) + 32 (grid width choice)
+template <typename T, typename G>
+__global__ __launch_bounds__(256) void k_radius(const T *__restrict__ x, uint32_t n, G g,
+                                                const uint32_t *__restrict__ cnt, float logk1,
+                                                float logdens, const float *__restrict__ Wg,
+                                                float margin, float wf, T *radius,
+                                                float *featOut, uint32_t *rhist) {
+  __shared__ float W[MLP_NW];
+  __shared__ uint32_t lh[64];
+  for (uint32_t i = threadIdx.x; i < MLP_NW; i += blockDim.x) W[i] = Wg ? Wg[i] : 0.0f;
+  if (threadIdx.x < 64) lh[threadIdx.x] = 0;
+  __syncthreads();
+  const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < n) {
+    float f[MLP_IN];
+    features_of<T, G>(x, n, q, g, cnt, logk1, logdens, f);
+    if (featOut)
+      for (int j = 0; j < MLP_IN; j++) featOut[(size_t)q * MLP_IN + j] = f[j];
+    if (radius) {
+      const float y = mlp_forward(W, f, nullptr, nullptr);
+      const float r = wf * expf(y + margin);
+      radius[q] = (T)r;
+      if (rhist) {
+        int b = (int)floorf(4.0f * log2f(r / wf)) + 32;
+        b = b < 0 ? 0 : (b > 63 ? 63 : b);
+        atomicAdd(&lh[b], 1u);
+      }
+    }
+  }
+  __syncthreads();
+  if (rhist && threadIdx.x < 64 && lh[threadIdx.x]) atomicAdd(&rhist[threadIdx.x], lh[threadIdx.x]);
+}
+
+// radius *= factor for the queries that failed the last pass
+template <typename T>
+__global__ void k_grow_radius(T *radius, const uint8_t *failed, uint32_t n, T factor) {
+  const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < n && failed[q]) radius[q] = radius[q] * factor;
+}
+
+// largest radius among the failed queries (bit pattern of a positive value orders as an
+// unsigned integer)
+__global__ void k_max_radius_f(const float *radius, const uint8_t *failed, uint32_t n, uint32_t *out) {
+  const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < n && failed[q]) atomicMax(out, __float_as_uint(radius[q]));
+}
+__global__ void k_max_radius_d(const double *radius, const uint8_t *failed, uint32_t n,
+                               unsigned long long *out) {
+  const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < n && failed[q]) atomicMax(out, (unsigned long long)__double_as_longlong(radius[q]));
+}
+
+// gather rows of a (n x w) float matrix: out[b] = in[idx[b]]
+__global__ void k_gather_rows(const float *in, const uint32_t *idx, uint32_t B, uint32_t w, float *out) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= B * w) return;
+  const uint32_t b = t / w, j = t % w;
+  out[t] = in[(size_t)idx[b] * w + j];
+}
+
+// ---------------------------------------------------------------- training kernels
+__global__ void k_mlp_fwd(const float *__restrict__ Wg, const float *__restrict__ X, uint32_t B,
+                          float *H1, float *H2, float *out) {
+  __shared__ float W[MLP_NW];
+  for (uint32_t i = threadIdx.x; i < MLP_NW; i += blockDim.x) W[i] = Wg[i];
+  __syncthreads();
+  const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= B) return;
+  float f[MLP_IN];
+#pragma unroll
+  for (int j = 0; j < MLP_IN; j++) f[j] = X[(size_t)b * MLP_IN + j];
+  out[b] = mlp_forward(W, f, H1 + (size_t)b * MLP_H, H2 + (size_t)b * MLP_H);
+}
+
+// per sample: dout = 2 (out - y) / B, dH2 = dout w3 [h2 > 0], dH1 = (W2^T dH2) [h1 > 0]
+__global__ void k_mlp_bwd_samples(const float *__restrict__ Wg, const float *__restrict__ Y,
+                                  uint32_t B, const float *H1, const float *H2, const float *out,
+                                  float *dout, float *dH1, float *dH2) {
+  __shared__ float W[MLP_NW];
+  for (uint32_t i = threadIdx.x; i < MLP_NW; i += blockDim.x) W[i] = Wg[i];
+  __syncthreads();
+  const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= B) return;
+  const float *W2 = W + 832, *w3 = W + 4992;
+  const float d = 2.0f * (out[b] - Y[b]) / (float)B;
+  dout[b] = d;
+  float g2[MLP_H];
+#pragma unroll
+  for (int i = 0; i < MLP_H; i++) {
+    g2[i] = H2[(size_t)b * MLP_H + i] > 0.0f ? d * w3[i] : 0.0f;
+    dH2[(size_t)b * MLP_H + i] = g2[i];
+  }
+  for (int j = 0; j < MLP_H; j++) {
+    float a = 0.0f;
+#pragma unroll
+    for (int i = 0; i < MLP_H; i++) a = fmaf(W2[i * MLP_H + j], g2[i], a);
+    dH1[(size_t)b * MLP_H + j] = H1[(size_t)b * MLP_H + j] > 0.0f ? a : 0.0f;
+  }
+}
+
+// one thread per parameter: gradients summed over the batch in sample order
+__global__ void k_mlp_wgrad(const float *__restrict__ X, uint32_t B, const float *H1, const float *H2,
+                            const float *dout, const float *dH1, const float *dH2, float *G) {
+  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= MLP_NW) return;
+  float a = 0.0f;
+  if (p < 768) {  // W1[i][j] = sum dH1[b][i] X[b][j]
+    const uint32_t i = p / MLP_IN, j = p % MLP_IN;
+    for (uint32_t b = 0; b < B; b++) a = fmaf(dH1[(size_t)b * MLP_H + i], X[(size_t)b * MLP_IN + j], a);
+  } else if (p < 832) {  // b1
+    const uint32_t i = p - 768;
+    for (uint32_t b = 0; b < B; b++) a += dH1[(size_t)b * MLP_H + i];
+  } else if (p < 4928) {  // W2[i][j] = sum dH2[b][i] H1[b][j]
+    const uint32_t i = (p - 832) / MLP_H, j = (p - 832) % MLP_H;
+    for (uint32_t b = 0; b < B; b++) a = fmaf(dH2[(size_t)b * MLP_H + i], H1[(size_t)b * MLP_H + j], a);
+  } else if (p < 4992) {  // b2
+    const uint32_t i = p - 4928;
+    for (uint32_t b = 0; b < B; b++) a += dH2[(size_t)b * MLP_H + i];
+  } else if (p < 5056) {  // w3[i] = sum dout[b] H2[b][i]
+    const uint32_t i = p - 4992;
+    for (uint32_t b = 0; b < B; b++) a = fmaf(dout[b], H2[(size_t)b * MLP_H + i], a);
+  } else {  // b3
+    for (uint32_t b = 0; b < B; b++) a += dout[b];
+  }
+  G[p] = a;
+}
+
+// mean squared error of the batch, summed in sample order
+__global__ void k_mlp_loss(const float *out, const float *Y, uint32_t B, float *loss) {
+  if (threadIdx.x || blockIdx.x) return;
+  float s = 0.0f;
+  for (uint32_t b = 0; b < B; b++) {
+    const float e = out[b] - Y[b];
+    s = fmaf(e, e, s);
+  }
+  *loss = s / (float)B;
+}
+
+// Adam: m = b1 m + (1-b1) g, v = b2 v + (1-b2) g^2, w -= lr * (m / c1) / (sqrt(v / c2) + eps)
+__global__ void k_adam(float *W, const float *G, float *M, float *V, float lr, float b1, float b2,
+                       float c1, float c2, float eps) {
+  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= MLP_NW) return;
+  const float g = G[p];
+  const float m = b1 * M[p] + (1.0f - b1) * g;
+  const float v = b2 * V[p] + (1.0f - b2) * (g * g);
+  M[p] = m;
+  V[p] = v;
+  W[p] = W[p] - lr * ((m / c1) / (sqrtf(v / c2) + eps));
+}
+
+}  // namespace
+
+template <typename T>
+void feature_counts(const T *x, uint32_t n, const typename GridOf<T>::type &g, uint32_t *cnt,
+                    hipStream_t s) {
+  if (n) k_feature_counts<T, typename GridOf<T>::type><<<(n + 255) / 256, 256, 0, s>>>(x, n, g, cnt);
+}
+
+template <typename T>
+void radius_kernel(const T *x, uint32_t n, const typename GridOf<T>::type &g, const uint32_t *cnt,
+                   float logk1, float logdens, const float *W, float margin, float wf, T *radius,
+                   float *featOut, uint32_t *rhist, hipStream_t s) {
+  if (n)
+    k_radius<T, typename GridOf<T>::type><<<(n + 255) / 256, 256, 0, s>>>(
+        x, n, g, cnt, logk1, logdens, W, margin, wf, radius, featOut, rhist);
+}
+
+template <typename T>
+void grow_radius(T *radius, const uint8_t *failed, uint32_t n, T factor, hipStream_t s) {
+  if (n) k_grow_radius<T><<<(n + 255) / 256, 256, 0, s>>>(radius, failed, n, factor);
+}
+
+void max_failed_radius(const float *radius, const uint8_t *failed, uint32_t n, void *out, hipStream_t s) {
+  fill_u32(reinterpret_cast<uint32_t *>(out), 0u, 2, s);
+  if (n) k_max_radius_f<<<(n + 255) / 256, 256, 0, s>>>(radius, failed, n, reinterpret_cast<uint32_t *>(out));
+}
+void max_failed_radius(const double *radius, const uint8_t *failed, uint32_t n, void *out, hipStream_t s) {
+  fill_u32(reinterpret_cast<uint32_t *>(out), 0u, 2, s);
+  if (n) k_max_radius_d<<<(n + 255) / 256, 256, 0, s>>>(radius, failed, n, reinterpret_cast<unsigned long long *>(out));
+}
+void gather_rows(const float *in, const uint32_t *idx, uint32_t B, uint32_t w, float *out, hipStream_t s) {
+  if (B) k_gather_rows<<<(B * w + 255) / 256, 256, 0, s>>>(in, idx, B, w, out);
+}
+
+void mlp_forward_batch(const float *W, const float *X, uint32_t B, float *H1, float *H2, float *out,
+                       hipStream_t s) {
+  if (B) k_mlp_fwd<<<(B + 127) / 128, 128, 0, s>>>(W, X, B, H1, H2, out);
+}
+
+void mlp_grad_batch(const float *W, const float *X, const float *Y, uint32_t B, float *H1,
+                    float *H2, float *out, float *dout, float *dH1, float *dH2, float *G,
+                    float *loss, hipStream_t s) {
+  if (!B) return;
+  k_mlp_fwd<<<(B + 127) / 128, 128, 0, s>>>(W, X, B, H1, H2, out);
+  k_mlp_bwd_samples<<<(B + 127) / 128, 128, 0, s>>>(W, Y, B, H1, H2, out, dout, dH1, dH2);
+  k_mlp_wgrad<<<(MLP_NW + 63) / 64, 64, 0, s>>>(X, B, H1, H2, dout, dH1, dH2, G);
+  if (loss) k_mlp_loss<<<1, 64, 0, s>>>(out, Y, B, loss);
+}
+
+void adam_step(float *W, const float *G, float *M, float *V, float lr, float b1, float b2, float c1,
+               float c2, float eps, hipStream_t s) {
+  k_adam<<<(MLP_NW + 255) / 256, 256, 0, s>>>(W, G, M, V, lr, b1, b2, c1, c2, eps);
+}
+
+#define INST(T)                                                                                   \
+  template void feature_counts<T>(const T *, uint32_t, const GridOf<T>::type &, uint32_t *,        \
+                                  hipStream_t);                                                   \
+  template void radius_kernel<T>(const T *, uint32_t, const GridOf<T>::type &, const uint32_t *,   \
+                                 float, float, const float *, float, float, T *, float *,         \
+                                 uint32_t *, hipStream_t);                                        \
+  template void grow_radius<T>(T *, const uint8_t *, uint32_t, T, hipStream_t);
+INST(float)
+INST(double)
+#undef INST
+
+}  // namespace knn

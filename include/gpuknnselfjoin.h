@@ -142,6 +142,34 @@ int knnjoin_stage_join_pass(const float *points, uint32_t n, uint32_t ndim, uint
                             int32_t *out_ids, float *out_dists, uint8_t *failed,
                             uint32_t *num_failed);
 
+/* ---------------------------------------------------------------------------------------
+ * Learned per-query initial radius (no counterpart in the reference, whose only schedule is
+ * the BETA estimate + growth loop of main.cu:305-430; used by KNN_MODE_LEARNED).
+ * MLP 12 -> 64 -> 64 -> 1, ReLU, FP32; flat weights W1[64][12], b1[64], W2[64][64], b2[64],
+ * w3[64], b3 (5057 floats).  Radius = wf * exp(mlp(features) + margin).
+ * ------------------------------------------------------------------------------------- */
+
+/* Registers the model used for data with `nd` indexed dims. */
+int knnjoin_set_radius_model(uint32_t nd, const float *weights, uint32_t nweights, float margin);
+
+/* The 12 features of every point (n x 12, row-major) as the learned schedule computes them
+ * (after the dimension reorder when `reorder`), the feature-grid width wf, and (when a model
+ * is registered for nd and radius_out != NULL) the radius of every point. */
+int knnjoin_radius_features(const float *points, uint32_t n, uint32_t ndim, uint32_t k, uint32_t nd,
+                            int reorder, float *features, float *wf_out, float *radius_out);
+
+/* MLP forward over B samples (X: B x 12). */
+int knnjoin_mlp_forward(const float *weights, const float *X, uint32_t B, float *out);
+
+/* Gradient of the mean squared error over B samples: grads[5057] in the weight layout. */
+int knnjoin_mlp_grad(const float *weights, const float *X, const float *Y, uint32_t B, float *grads,
+                     float *loss);
+
+/* Adam training on the GPU (lr, beta1 0.9, beta2 0.999, eps 1e-8), `epochs` passes over a
+ * seeded shuffle in mini-batches of `batch`; weights updated in place. */
+int knnjoin_mlp_train(float *weights, const float *X, const float *Y, uint32_t n, uint32_t epochs,
+                      uint32_t batch, float lr, uint64_t seed, float *loss_per_epoch);
+
 #ifdef __cplusplus
 }
 #endif

@@ -224,3 +224,202 @@ double orc_checksum_f32(const float *dists, uint64_t rows, uint32_t k1) {
   }
   return s;
 }
+
+/* ======================================================================================
+ * Learned-radius MLP (12 -> 64 -> 64 -> 1, ReLU) -- CPU restatement of the product's
+ * forward / gradient / Adam arithmetic, in the same fmaf and summation order, so that the
+ * GPU results must agree bit for bit.  Weights: W1[64][12], b1[64], W2[64][64], b2[64],
+ * w3[64], b3.
+ * ====================================================================================== */
+#define MLP_IN 12
+#define MLP_H 64
+#define MLP_NW 5057
+
+static float orc_relu(float v) { return v > 0.0f ? v : 0.0f; }
+
+static float orc_mlp_fwd1(const float *W, const float *f, float *H1, float *H2) {
+  const float *W1 = W, *b1 = W + 768, *W2 = W + 832, *b2 = W + 4928, *w3 = W + 4992;
+  float acc[MLP_H];
+  for (int i = 0; i < MLP_H; i++) acc[i] = b2[i];
+  for (int j = 0; j < MLP_H; j++) {
+    float a = b1[j];
+    for (int t = 0; t < MLP_IN; t++) a = fmaf(W1[j * MLP_IN + t], f[t], a);
+    float h = orc_relu(a);
+    if (H1) H1[j] = h;
+    for (int i = 0; i < MLP_H; i++) acc[i] = fmaf(W2[i * MLP_H + j], h, acc[i]);
+  }
+  float y = W[5056];
+  for (int i = 0; i < MLP_H; i++) {
+    float h = orc_relu(acc[i]);
+    if (H2) H2[i] = h;
+    y = fmaf(w3[i], h, y);
+  }
+  return y;
+}
+
+void orc_mlp_forward(const float *W, const float *X, uint32_t B, float *out) {
+  for (uint32_t b = 0; b < B; b++) out[b] = orc_mlp_fwd1(W, X + (size_t)b * MLP_IN, NULL, NULL);
+}
+
+/* gradient of mean((out - y)^2) over the batch, in the product's order */
+void orc_mlp_grad(const float *W, const float *X, const float *Y, uint32_t B, float *G, float *loss) {
+  float *H1 = (float *)malloc(sizeof(float) * B * MLP_H);
+  float *H2 = (float *)malloc(sizeof(float) * B * MLP_H);
+  float *dH1 = (float *)malloc(sizeof(float) * B * MLP_H);
+  float *dH2 = (float *)malloc(sizeof(float) * B * MLP_H);
+  float *out = (float *)malloc(sizeof(float) * B);
+  float *dout = (float *)malloc(sizeof(float) * B);
+  const float *W2 = W + 832, *w3 = W + 4992;
+  for (uint32_t b = 0; b < B; b++) out[b] = orc_mlp_fwd1(W, X + (size_t)b * MLP_IN, H1 + (size_t)b * MLP_H, H2 + (size_t)b * MLP_H);
+  for (uint32_t b = 0; b < B; b++) {
+    float d = 2.0f * (out[b] - Y[b]) / (float)B;
+    dout[b] = d;
+    float g2[MLP_H];
+    for (int i = 0; i < MLP_H; i++) {
+      g2[i] = H2[(size_t)b * MLP_H + i] > 0.0f ? d * w3[i] : 0.0f;
+      dH2[(size_t)b * MLP_H + i] = g2[i];
+    }
+    for (int j = 0; j < MLP_H; j++) {
+      float a = 0.0f;
+      for (int i = 0; i < MLP_H; i++) a = fmaf(W2[i * MLP_H + j], g2[i], a);
+      dH1[(size_t)b * MLP_H + j] = H1[(size_t)b * MLP_H + j] > 0.0f ? a : 0.0f;
+    }
+  }
+  for (uint32_t p = 0; p < MLP_NW; p++) {
+    float a = 0.0f;
+    if (p < 768) {
+      uint32_t i = p / MLP_IN, j = p % MLP_IN;
+      for (uint32_t b = 0; b < B; b++) a = fmaf(dH1[(size_t)b * MLP_H + i], X[(size_t)b * MLP_IN + j], a);
+    } else if (p < 832) {
+      for (uint32_t b = 0; b < B; b++) a += dH1[(size_t)b * MLP_H + (p - 768)];
+    } else if (p < 4928) {
+      uint32_t i = (p - 832) / MLP_H, j = (p - 832) % MLP_H;
+      for (uint32_t b = 0; b < B; b++) a = fmaf(dH2[(size_t)b * MLP_H + i], H1[(size_t)b * MLP_H + j], a);
+    } else if (p < 4992) {
+      for (uint32_t b = 0; b < B; b++) a += dH2[(size_t)b * MLP_H + (p - 4928)];
+    } else if (p < 5056) {
+      for (uint32_t b = 0; b < B; b++) a = fmaf(dout[b], H2[(size_t)b * MLP_H + (p - 4992)], a);
+    } else {
+      for (uint32_t b = 0; b < B; b++) a += dout[b];
+    }
+    G[p] = a;
+  }
+  if (loss) {
+    float s = 0.0f;
+    for (uint32_t b = 0; b < B; b++) {
+      float e = out[b] - Y[b];
+      s = fmaf(e, e, s);
+    }
+    *loss = s / (float)B;
+  }
+  free(H1); free(H2); free(dH1); free(dH2); free(out); free(dout);
+}
+
+/* Adam training restated (splitmix64 Fisher-Yates per epoch, batches of the shuffled order) */
+void orc_mlp_train(float *W, const float *X, const float *Y, uint32_t n, uint32_t epochs,
+                   uint32_t batch, float lr, uint64_t seed) {
+  uint32_t B = batch < n ? batch : n;
+  float *M = (float *)calloc(MLP_NW, sizeof(float));
+  float *V = (float *)calloc(MLP_NW, sizeof(float));
+  float *G = (float *)malloc(sizeof(float) * MLP_NW);
+  float *Xb = (float *)malloc(sizeof(float) * B * MLP_IN);
+  float *Yb = (float *)malloc(sizeof(float) * B);
+  uint32_t *order = (uint32_t *)malloc(sizeof(uint32_t) * n);
+  for (uint32_t i = 0; i < n; i++) order[i] = i;
+  uint64_t st = seed, step = 0;
+  const float b1 = 0.9f, b2 = 0.999f, epsA = 1e-8f;
+  for (uint32_t e = 0; e < epochs; e++) {
+    for (uint32_t i = n - 1; i > 0; i--) {
+      uint64_t z = (st += 0x9E3779B97F4A7C15ull);
+      z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+      z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+      z = z ^ (z >> 31);
+      uint32_t j = (uint32_t)(z % (uint64_t)(i + 1));
+      uint32_t t = order[i]; order[i] = order[j]; order[j] = t;
+    }
+    for (uint32_t bi = 0; bi < n / B; bi++) {
+      for (uint32_t b = 0; b < B; b++) {
+        memcpy(Xb + (size_t)b * MLP_IN, X + (size_t)order[bi * B + b] * MLP_IN, sizeof(float) * MLP_IN);
+        Yb[b] = Y[order[bi * B + b]];
+      }
+      orc_mlp_grad(W, Xb, Yb, B, G, NULL);
+      step++;
+      float c1 = (float)(1.0 - pow(0.9, (double)step));
+      float c2 = (float)(1.0 - pow(0.999, (double)step));
+      for (uint32_t p = 0; p < MLP_NW; p++) {
+        float g = G[p];
+        float m = b1 * M[p] + (1.0f - b1) * g;
+        float v = b2 * V[p] + (1.0f - b2) * (g * g);
+        M[p] = m;
+        V[p] = v;
+        W[p] = W[p] - lr * ((m / c1) / (sqrtf(v / c2) + epsA));
+      }
+    }
+  }
+  free(M); free(V); free(G); free(Xb); free(Yb); free(order);
+}
+
+/* Features of the learned radius (see the product's learned.hip header for the definition),
+ * on points already in the reordered dim order (row-major n x d). */
+void orc_radius_features(const float *pts, uint32_t n, uint32_t d, uint32_t k, uint32_t nd,
+                         float *feat, float *wf_out) {
+  float mn[ORC_MAXDIM], mx[ORC_MAXDIM], minArr[ORC_MAXDIM];
+  uint32_t nCells[ORC_MAXDIM];
+  uint64_t tot;
+  orc_minmax_impl_f32(pts, n, d, nd, mn, mx);
+  double V = 1.0;
+  for (uint32_t j = 0; j < nd; j++) {
+    double e = (double)(mx[j] - mn[j]);
+    V *= e > 0.0 ? e : 1.0;
+  }
+  float wf = (float)pow(V * (double)(k + 1) / (double)n, 1.0 / (double)nd);
+  if (!(wf > 0.0f)) wf = 1.0f;
+  for (;;) {
+    orc_grid_dims_impl_f32(mn, mx, nd, wf, minArr, nCells, &tot);
+    if (tot <= (1ull << 27)) break;
+    wf = wf * 2.0f;
+  }
+  uint64_t mult[ORC_MAXDIM];
+  uint64_t m = 1;
+  for (uint32_t j = 0; j < nd; j++) { mult[j] = m; m *= nCells[j]; }
+  uint32_t *cnt = (uint32_t *)calloc(tot, sizeof(uint32_t));
+  int64_t *cc = (int64_t *)malloc(sizeof(int64_t) * (size_t)n * nd);
+  for (uint64_t i = 0; i < n; i++) {
+    uint64_t id = 0;
+    for (uint32_t j = 0; j < nd; j++) {
+      cc[i * nd + j] = orc_coord_f32(pts[i * d + j], minArr[j], wf, nCells[j]);
+      id += (uint64_t)cc[i * nd + j] * mult[j];
+    }
+    cnt[id]++;
+  }
+  float logk1 = logf((float)(k + 1));
+  float logdens = (float)log((double)n / V);
+  for (uint64_t q = 0; q < n; q++) {
+    const int64_t *c = cc + q * nd;
+    uint64_t own = 0;
+    for (uint32_t j = 0; j < nd; j++) own += (uint64_t)c[j] * mult[j];
+    float *f = feat + q * 12;
+    f[0] = log1pf((float)cnt[own]);
+    int kk = 1;
+    for (int dy = -1; dy <= 1; dy++)
+      for (int dx = -1; dx <= 1; dx++) {
+        int64_t c0, c1 = 0;
+        if (nd == 1) c0 = c[0] + (dy + 1) * 3 + dx - 4;
+        else { c0 = c[0] + dx; c1 = c[1] + dy; }
+        float v = 0.0f;
+        int in0 = c0 >= 0 && c0 < (int64_t)nCells[0];
+        int in1 = nd == 1 || (c1 >= 0 && c1 < (int64_t)nCells[1]);
+        if (in0 && in1) {
+          uint64_t id = own - (uint64_t)c[0] * mult[0] + (uint64_t)c0 * mult[0];
+          if (nd > 1) id = id - (uint64_t)c[1] * mult[1] + (uint64_t)c1 * mult[1];
+          v = log1pf((float)cnt[id]);
+        }
+        f[kk++] = v;
+      }
+    f[10] = logk1;
+    f[11] = logdens;
+  }
+  if (wf_out) *wf_out = wf;
+  free(cnt);
+  free(cc);
+}

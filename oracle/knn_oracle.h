@@ -98,6 +98,14 @@ float orc_grow_eps_f32(float eps, float eps0);
 /* the FP32 cell coordinate of kernel.cu:1685-1687 (with clamping to [0, ncell-1]) */
 uint32_t orc_cell_coord_f32(float x, float mn, float eps, uint32_t ncell);
 
+/* ---- learned radius: MLP 12 -> 64 -> 64 -> 1 and its features ---- */
+void orc_mlp_forward(const float *W, const float *X, uint32_t B, float *out);
+void orc_mlp_grad(const float *W, const float *X, const float *Y, uint32_t B, float *G, float *loss);
+void orc_mlp_train(float *W, const float *X, const float *Y, uint32_t n, uint32_t epochs,
+                   uint32_t batch, float lr, uint64_t seed);
+void orc_radius_features(const float *pts, uint32_t n, uint32_t d, uint32_t k, uint32_t nd,
+                         float *feat, float *wf_out);
+
 #ifdef __cplusplus
 }
 #endif

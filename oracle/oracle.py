@@ -97,6 +97,10 @@ def _setup(L):
     L.orc_grow_eps_f32.restype = flt
     L.orc_cell_coord_f32.argtypes = [flt, flt, flt, u32]
     L.orc_cell_coord_f32.restype = u32
+    L.orc_mlp_forward.argtypes = [vp, vp, u32, vp]
+    L.orc_mlp_grad.argtypes = [vp, vp, vp, u32, vp, vp]
+    L.orc_mlp_train.argtypes = [vp, vp, vp, u32, u32, u32, flt, u64]
+    L.orc_radius_features.argtypes = [vp, u32, u32, u32, u32, vp, vp]
 
 
 def _threads(nthreads):
@@ -241,3 +245,42 @@ def join_pass(pts_r: np.ndarray, nd: int, k: int, eps: float, queries: np.ndarra
                                  _p(cellKeys), _p(cellStart), _p(sortedIds), _p(q), len(q),
                                  _threads(nthreads), _p(ids), _p(dists), _p(failed), _p(cand))
     return ids, dists, failed, int(nf)
+
+
+# ---------------------------------------------------------------- learned radius
+MLP_NW = 5057
+
+
+def mlp_forward(W, X):
+    W = np.ascontiguousarray(W, np.float32)
+    X = np.ascontiguousarray(X, np.float32)
+    out = np.zeros(len(X), np.float32)
+    lib().orc_mlp_forward(_p(W), _p(X), len(X), _p(out))
+    return out
+
+
+def mlp_grad(W, X, Y):
+    W = np.ascontiguousarray(W, np.float32)
+    X = np.ascontiguousarray(X, np.float32)
+    Y = np.ascontiguousarray(Y, np.float32)
+    G = np.zeros(MLP_NW, np.float32)
+    loss = np.zeros(1, np.float32)
+    lib().orc_mlp_grad(_p(W), _p(X), _p(Y), len(X), _p(G), _p(loss))
+    return G, float(loss[0])
+
+
+def mlp_train(W, X, Y, epochs, batch, lr, seed):
+    W = np.array(W, np.float32, copy=True)
+    X = np.ascontiguousarray(X, np.float32)
+    Y = np.ascontiguousarray(Y, np.float32)
+    lib().orc_mlp_train(_p(W), _p(X), _p(Y), len(X), epochs, batch, lr, seed)
+    return W
+
+
+def radius_features(pts_r, k, nd):
+    pts = np.ascontiguousarray(pts_r, np.float32)
+    n, d = pts.shape
+    feat = np.zeros((n, 12), np.float32)
+    wf = np.zeros(1, np.float32)
+    lib().orc_radius_features(_p(pts), n, d, k, nd, _p(feat), _p(wf))
+    return feat, float(wf[0])

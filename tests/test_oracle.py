@@ -234,3 +234,46 @@ def test_checksum_definition(oracle):
     assert oracle.checksum(d) == want
     rc, ids, dd, st = oracle.knnjoin(np.random.default_rng(4).uniform(0, 1, (9000, 2)).astype(np.float32), 6)
     assert st.checksum == oracle.checksum(dd)
+
+
+# ---------------------------------------------------------------- learned-radius MLP oracle
+def _np_mlp(W, X):
+    W = W.astype(np.float64)
+    W1 = W[:768].reshape(64, 12)
+    b1, W2, b2, w3, b3 = W[768:832], W[832:4928].reshape(64, 64), W[4928:4992], W[4992:5056], W[5056]
+    h1 = np.maximum(X @ W1.T + b1, 0)
+    h2 = np.maximum(h1 @ W2.T + b2, 0)
+    return h2 @ w3 + b3
+
+
+def test_oracle_mlp_forward_and_gradient_closed_form(oracle):
+    rng = np.random.default_rng(0)
+    W = rng.normal(0, 0.2, 5057).astype(np.float32)
+    X = rng.normal(0, 1, (200, 12)).astype(np.float32)
+    Y = rng.normal(0, 1, 200).astype(np.float32)
+    out = oracle.mlp_forward(W, X)
+    assert np.allclose(out, _np_mlp(W, X.astype(np.float64)), rtol=1e-4, atol=1e-5)
+    G, loss = oracle.mlp_grad(W, X, Y)
+
+    def L(Wv):
+        return np.mean((_np_mlp(Wv, X.astype(np.float64)) - Y) ** 2)
+
+    assert abs(loss - L(W)) < 1e-5 * max(1.0, L(W))
+    Wd = W.astype(np.float64)
+    for p in [0, 11, 700, 770, 831, 900, 3000, 4927, 4930, 4991, 5000, 5056]:
+        e = 1e-4
+        a, b = Wd.copy(), Wd.copy()
+        a[p] += e
+        b[p] -= e
+        fd = (L(a) - L(b)) / (2 * e)
+        assert abs(G[p] - fd) <= 1e-4 * max(1.0, abs(fd)), (p, G[p], fd)
+
+
+def test_oracle_features_definition(oracle):
+    # lattice with known counts: 3 points per unit cell in a 10x10 box
+    g = np.stack(np.meshgrid(np.arange(10), np.arange(10)), -1).reshape(-1, 2).astype(np.float32)
+    pts = np.concatenate([g + 0.25, g + 0.5, g + 0.75]).astype(np.float32)
+    f, wf = oracle.radius_features(pts, 2, 2)
+    assert f.shape == (300, 12)
+    assert np.allclose(f[:, 10], np.log(3.0)) and np.allclose(f[:, 11], np.log(300 / (9.5 * 9.5)))
+    assert np.all(f[:, 0] > 0) and np.all(f[:, 1:10].max(1) >= f[:, 0] - 1e-6)
