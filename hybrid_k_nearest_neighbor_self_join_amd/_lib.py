@@ -71,6 +71,8 @@ class KnnStats(ctypes.Structure):
         ("ms_range_kernel_max", ctypes.c_double),
         ("algo_bytes_max", ctypes.c_uint64),
         ("overflow_queries", ctypes.c_uint64),
+        ("eps_bucket", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
     ]
 
     def as_dict(self) -> dict:

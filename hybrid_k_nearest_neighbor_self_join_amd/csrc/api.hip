@@ -704,6 +704,7 @@ static void run_plain(Pipeline<T> &P, double beta, knnjoin_stats &stats, uint8_t
   tPh.start();
   EpsResult er = estimate_eps<T>(P.x, n, d, k, beta, nullptr, s);
   T eps0 = (T)er.best;
+  stats.eps_bucket = er.bucket;
   stats.ms_eps = tPh.stop();
 
   // ---- min/max of the indexed dims (main.cu:2439-2456)

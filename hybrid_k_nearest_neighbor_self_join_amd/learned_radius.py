@@ -28,6 +28,20 @@ def weights_path(nd: int, directory: str = WEIGHTS_DIR) -> str:
     return os.path.join(directory, f"radius_mlp_nd{nd}.npz")
 
 
+def raw_path(nd: int, directory: str = WEIGHTS_DIR) -> str:
+    """Model file of the C++ command line (`knnjoin_main --learned FILE`): the 5057 weights
+    then the margin, little-endian float32."""
+    return os.path.join(directory, f"radius_mlp_nd{nd}.bin")
+
+
+def save(nd: int, w: np.ndarray, margin: float, info: dict, directory: str = WEIGHTS_DIR):
+    os.makedirs(directory, exist_ok=True)
+    np.savez(weights_path(nd, directory), weights=w.astype(np.float32), margin=np.float32(margin),
+             info=np.array(json.dumps(info)))
+    with open(raw_path(nd, directory), "wb") as f:
+        f.write(np.concatenate([w.astype("<f4"), np.array([margin], "<f4")]).tobytes())
+
+
 def init_weights(seed: int) -> np.ndarray:
     """He-normal weights, zero biases, in the flat layout W1, b1, W2, b2, w3, b3."""
     rng = np.random.default_rng(seed)
@@ -99,14 +113,12 @@ def train(nd: int = 2, n: int = 2_000_000, ks=(8, 16, 32, 64, 128), per_set: int
     base_val = float(np.mean((Y[val] - Y[tr].mean()) ** 2))
     margin = calibrate_margin(w, X[tr], Y[tr], quantile)
     cover = float(np.mean(Y[val] - pred_val <= margin))
-    os.makedirs(out_dir, exist_ok=True)
     info = {"nd": nd, "pairs": int(len(Y)), "epochs": epochs, "batch": batch, "lr": lr,
             "seed": seed, "quantile": quantile, "margin": margin, "val_mse": mse_val,
             "val_mse_constant_predictor": base_val, "val_coverage": cover,
             "train_seconds": t_train, "loss_per_epoch": [float(v) for v in losses],
             "training_sets": f"{len(default_sets(nd, 16, seed))} seeded sets of {n} points, k in {list(ks)}"}
-    np.savez(weights_path(nd, out_dir), weights=w, margin=np.float32(margin),
-             info=np.array(json.dumps(info)))
+    save(nd, w, margin, info, out_dir)
     log(f"[train] nd={nd}: val mse {mse_val:.4f} (constant predictor {base_val:.4f}), "
         f"margin {margin:.3f} covers {cover:.3f} of validation queries; {t_train:.1f} s")
     return info

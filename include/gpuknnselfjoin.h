@@ -72,6 +72,8 @@ typedef struct {
   double ms_range_kernel_max;
   uint64_t algo_bytes_max;  /* algorithmic bytes of the longest range-kernel launch */
   uint64_t overflow_queries; /* queries re-run by the heap kernel (staging overflow) */
+  uint32_t eps_bucket;      /* k-distance histogram bucket of eps0 (GPU.cu:3484-3527) */
+  uint32_t reserved;
 } knnjoin_stats;
 
 /*
