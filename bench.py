@@ -1,16 +1,19 @@
 """Benchmark: exact KNN self-join throughput on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--k 64] [--n 10000000]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--k 64] [--n 10000000] [--mode auto]
 
 Workload (default): 10,000,000 2-D float32 points, 70% uniform over [0,360)x[-90,90] and 30%
 from 64 Gaussian clusters (sigma 0.5), k=64, BETA=0.025, 2 indexed dims.  One step = the
-whole device pipeline of one self-join (dimension reorder, BETA epsilon estimate, grid index,
-every failed-query iteration, checksum) from HBM-resident points to the HBM-resident
-(K+1)-table.  value = queries/s = N / (time per step).
+whole device pipeline of one self-join (dimension reorder, radius schedule, grid index,
+every failed-query pass, checksum) from HBM-resident points to the HBM-resident
+(K+1)-table.  Both radius schedules are timed when a trained 2-D radius model ships with the
+package: the plain BETA schedule and the learned per-query radius; value = queries/s
+(N / time per step) of the faster one, with the learned/plain speedup beside it.
 
 Also reported: the range kernel's achieved algorithmic HBM bandwidth against the 8 TB/s peak
-(HIP events on the kernel's stream, inside the library), and a CPU baseline: the oracle's
-OpenMP restatement of the same grid join on a bounded sample of the same point set.
+(HIP events on the kernel's stream, inside the library), its measured HBM traffic per launch
+when a rocprofv3 counter measurement of this workload is committed, and a CPU baseline: the
+oracle's OpenMP restatement of the same grid join on a bounded sample of the same point set.
 """
 from __future__ import annotations
 
@@ -100,6 +103,19 @@ def cpu_baseline(pts: np.ndarray, k: int, beta: float, sample_n: int):
     }
 
 
+def load_traffic(workload_key: str):
+    """HBM bytes per launch of the range kernel measured with rocprofv3 PMC counters
+    (FETCH_SIZE / WRITE_SIZE, scaled by the same counters' reading on a copy of known size),
+    committed beside the profile summaries; None when no measurement of this workload exists."""
+    path = os.path.join(ROOT, "profiles", "k_range_select_traffic.json")
+    try:
+        with open(path) as f:
+            e = json.load(f).get(workload_key)
+    except (OSError, ValueError):
+        return None
+    return e
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -108,30 +124,65 @@ def main():
     ap.add_argument("--n", type=int, default=10_000_000)
     ap.add_argument("--k", type=int, default=64)
     ap.add_argument("--beta", type=float, default=0.025)
-    ap.add_argument("--mode", choices=["plain", "learned"], default="plain")
+    ap.add_argument("--mode", choices=["auto", "plain", "learned"], default="auto",
+                    help="auto: time the plain BETA schedule and, when a trained radius model "
+                         "ships with the package, the learned schedule; value = the faster")
     ap.add_argument("--cpu-sample", type=int, default=1_000_000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run 5 device copies of 1 GiB (the reference kernel for scaling "
+                         "rocprofv3 FETCH_SIZE / WRITE_SIZE readings to bytes)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        a = torch.empty(1 << 28, dtype=torch.float32, device="cuda:0").fill_(1.0)
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": a.numel() * 4, "copy_bytes_written": a.numel() * 4,
+                          "copies": 5}), flush=True)
+        return
 
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (this build has no multi-GPU path); "
               f"--gpus {args.gpus} is not supported", file=sys.stderr)
         sys.exit(2)
 
-    from hybrid_k_nearest_neighbor_self_join_amd import _lib, synthetic
+    from hybrid_k_nearest_neighbor_self_join_amd import _lib, learned_radius, synthetic
 
     t0 = time.perf_counter()
     pts = synthetic.mixed_box(args.n)
     log(f"[bench] generated {args.n} points in {time.perf_counter() - t0:.1f} s")
-    mode = _lib.KNN_MODE_LEARNED if args.mode == "learned" else _lib.KNN_MODE_BETA
-    g = run_gpu(pts, args.k, args.beta, mode, args.steps, args.warmup)
+    models = learned_radius.register_default_models()
+    if args.mode == "plain":
+        modes = ["plain"]
+    elif args.mode == "learned":
+        if 2 not in models:
+            print("no trained 2-D radius model in the package", file=sys.stderr)
+            sys.exit(2)
+        modes = ["learned"]
+    else:
+        modes = ["plain"] + (["learned"] if 2 in models else [])
+    res = {}
+    for m in modes:
+        mode = _lib.KNN_MODE_LEARNED if m == "learned" else _lib.KNN_MODE_BETA
+        g = run_gpu(pts, args.k, args.beta, mode, args.steps, args.warmup)
+        g["qps"] = args.n / (g["ms_per_step"] / 1e3)
+        res[m] = g
+        log(f"[bench] {m}: {g['ms_per_step']:.2f} ms/step, {g['qps'] / 1e6:.2f} M queries/s")
+    best = max(res, key=lambda m: res[m]["qps"])
+    g = res[best]
     st = g["stats"]
-    value = args.n / (g["ms_per_step"] / 1e3)
     kern_s = g["kernel_ms"] / 1e3
     achieved = (g["algo_bytes"] / kern_s / 1e9) if kern_s > 0 else 0.0
+    launches = max(g["kernel_launches"], 1)
+    wkey = f"mixed{args.n}_k{args.k}_{best}"
+    tr = load_traffic(wkey)
     line = {
         "metric": "queries/s, 10M 2-D float32 k=64 exact; learned-radius vs plain BETA speedup",
-        "value": value,
+        "value": g["qps"],
         "unit": "queries/s",
         "n_gpus": 1,
         "steps": args.steps,
@@ -145,20 +196,29 @@ def main():
         "config": {
             "workload": (f"exact KNN self-join, {args.n} 2-D float32 points (70% uniform box + 30% "
                          f"in 64 Gaussian clusters), k={args.k}, BETA={args.beta}, "
-                         f"{args.mode} radius schedule"),
-            "n_points": args.n, "dim": 2, "k": args.k, "beta": args.beta, "mode": args.mode,
+                         f"{best} radius schedule"),
+            "n_points": args.n, "dim": 2, "k": args.k, "beta": args.beta, "mode": best,
             "num_indexed_dims": 2,
         },
+        "modes": {m: {"queries_per_s": r["qps"], "ms_per_step": r["ms_per_step"],
+                      "iterations": r["stats"]["iterations"], "reindexes": r["stats"]["reindexes"],
+                      "candidates": r["stats"]["candidates"],
+                      "range_kernel_ms": r["stats"]["ms_range_kernel"],
+                      "checksum": r["stats"]["checksum"]} for m, r in res.items()},
+        "speedup_learned_vs_plain": (res["learned"]["qps"] / res["plain"]["qps"]
+                                     if "learned" in res and "plain" in res else None),
         "roofline": {
             "bound": "hbm",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
-            "kernel": "k_range_topk",
-            "algo_bytes_per_launch": g["algo_bytes"] / max(g["kernel_launches"], 1),
-            "avg_launch_ms": g["kernel_ms"] / max(g["kernel_launches"], 1),
+            "traffic": tr["bytes_per_launch"] if tr else None,
+            "kernel": "k_range_select",
+            "algo_bytes_per_launch": g["algo_bytes"] / launches,
+            "avg_launch_ms": g["kernel_ms"] / launches,
+            "launches_per_step": g["kernel_launches"] / max(args.steps, 1),
+            "traffic_source": tr["source"] if tr else None,
         },
         "pipeline": {
             "ms_device_median": g["ms_device_median"],

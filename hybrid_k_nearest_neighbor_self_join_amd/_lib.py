@@ -32,7 +32,8 @@ EXPORTED_SYMBOLS = [
     "knnjoin_last_status", "knnjoin_last_error", "knnjoin_release", "knnjoin_stage_reorder",
     "knnjoin_stage_estimate_eps", "knnjoin_stage_index", "knnjoin_stage_join_pass",
     "knnjoin_set_radius_model", "knnjoin_radius_features", "knnjoin_mlp_forward",
-    "knnjoin_mlp_grad", "knnjoin_mlp_train",
+    "knnjoin_mlp_grad", "knnjoin_mlp_train", "knnjoin_batch_default_params", "knnjoin_batched",
+    "knnjoin_batch_autotune",
 ]
 MLP_NW = 5057
 

@@ -45,7 +45,7 @@ enum Slot {
   S_PTS, S_X, S_XS, S_KEYS, S_KEYS_TMP, S_IDS, S_IDS_TMP, S_CELLKEYS, S_CELLSTART, S_FLAGS, S_POS,
   S_ACT, S_NTILES, S_TILEOFF, S_TILES, S_FAILED, S_ROWSUM, S_OUTIDS, S_OUTDISTS, S_SCRATCH,
   S_HEAP, S_SMALL, S_PART, S_CTR, S_CSUM, S_HOSTFLAGS, S_OVERFLOW, S_STAGE, S_DENSE, S_BFLAGS,
-  S_BPOS, S_BSTART, S_CELLOF, S_CELLMAX, S_FCNT, S_RADIUS, S_RHIST, S_MODEL, S_TRAIN, S_NSLOTS
+  S_BPOS, S_BSTART, S_CELLOF, S_CELLMAX, S_FCNT, S_RADIUS, S_RHIST, S_MODEL, S_TRAIN, S_OCC, S_NSLOTS
 };
 
 struct Cache {
@@ -74,6 +74,7 @@ struct Cache {
 
 static std::mutex g_mu;
 static Cache g_cache;
+static void release_batch();
 static hipStream_t g_stream = nullptr;
 static int g_status = KNN_OK;
 static std::string g_err;
@@ -118,6 +119,7 @@ static void grid_dims(const T *mn, const T *mx, uint32_t nd, T eps, typename Gri
     g.nCells[j] = 1;
     g.mult[j] = tot;
   }
+  g.keyBase = 0;
   totalCells = tot;
 }
 
@@ -181,7 +183,19 @@ static void feature_grid(const T *mn, const T *mx, uint32_t nd, uint32_t n, uint
   }
   F.wf = wf;
   F.logk1 = logf((float)k1);
-  F.logdens = (float)std::log((double)n / V);
+  F.logdens = 0.0f;  // set by feature_occupancy once the cells are counted
+}
+
+// f11 = log(N / occupied feature cells) from the per-cell counts (one host read)
+static void feature_occupancy(const uint32_t *cnt, uint64_t cells, uint32_t n, float &logdens,
+                              hipStream_t s) {
+  unsigned long long *d = reinterpret_cast<unsigned long long *>(g_cache.get(S_OCC, 64));
+  fill_u32(reinterpret_cast<uint32_t *>(d), 0u, 2, s);
+  count_nonzero(cnt, cells, d, s);
+  unsigned long long occ = 0;
+  HIP_CHECK(hipMemcpyAsync(&occ, d, sizeof(occ), hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipStreamSynchronize(s));
+  logdens = (float)std::log((double)n / (double)(occ ? occ : 1ull));
 }
 
 // ------------------------------------------------------------------ pipeline
@@ -813,6 +827,7 @@ static void run_learned(Pipeline<T> &P, const knnjoin_params &p, knnjoin_stats &
   uint32_t *cnt = P.template buf<uint32_t>(S_FCNT, (size_t)F.cells);
   fill_u32(cnt, 0u, (size_t)F.cells, s);
   feature_counts<T>(P.x, n, F.g, cnt, s);
+  feature_occupancy(cnt, F.cells, n, F.logdens, s);
   float *Wd = P.template buf<float>(S_MODEL, MLP_NW);
   HIP_CHECK(hipMemcpyAsync(Wd, M.w.data(), sizeof(float) * MLP_NW, hipMemcpyHostToDevice, s));
   T *radius = P.template buf<T>(S_RADIUS, n);
@@ -1014,6 +1029,7 @@ const char *knnjoin_last_error(void) {
 void knnjoin_release(void) {
   std::lock_guard<std::mutex> lk(knn::g_mu);
   knn::g_cache.release();
+  knn::release_batch();
 }
 
 }  // extern "C"
@@ -1236,6 +1252,7 @@ int knnjoin_radius_features(const float *points, uint32_t n, uint32_t ndim, uint
     uint32_t *cnt = P.buf<uint32_t>(S_FCNT, (size_t)F.cells);
     fill_u32(cnt, 0u, (size_t)F.cells, s);
     feature_counts<float>(P.x, n, F.g, cnt, s);
+    feature_occupancy(cnt, F.cells, n, F.logdens, s);
     float *fd = features ? P.buf<float>(S_TRAIN, (size_t)n * MLP_IN) : nullptr;
     float *rd = nullptr;
     float *Wd = nullptr;
@@ -1361,3 +1378,434 @@ int knnjoin_mlp_train(float *weights, const float *X, const float *Y, uint32_t n
 }
 
 }  // extern "C"
+
+// ================================================================== batched sets
+// Host side of batched.hip: a plan per set-shape signature (allocations, descriptors, and the
+// captured hipGraph of the whole join), replayed while the caller's shapes stay the same.
+namespace knn {
+
+struct BatchGroup {
+  uint32_t d = 0, nsets = 0, N = 0, k1max = 0, smax = 0;
+  uint64_t outElems = 0, bound = 0;
+  int bits = 0;
+  std::vector<uint32_t> members;  // caller set indices
+  std::vector<SetDesc> hsets;
+  float *xin = nullptr, *x = nullptr, *xs = nullptr, *radius = nullptr, *width = nullptr,
+        *mnmx = nullptr, *var = nullptr, *outDists = nullptr;
+  uint32_t *perm = nullptr, *setOf = nullptr, *keys = nullptr, *keysTmp = nullptr, *ids = nullptr,
+           *idsTmp = nullptr, *cflags = nullptr, *cpos = nullptr, *cellStart = nullptr,
+           *dense = nullptr, *aflags = nullptr, *apos = nullptr, *act = nullptr,
+           *ntiles = nullptr, *tileOff = nullptr, *scratch = nullptr, *status = nullptr,
+           *setMaxBits = nullptr, *ctr = nullptr;
+  uint64_t *cellKeys = nullptr;
+  unsigned long long *cand = nullptr;
+  uint4 *tiles = nullptr;
+  uint8_t *failed = nullptr, *overflow = nullptr;
+  SetDesc *sets = nullptr;
+  int32_t *outIds = nullptr;
+  void *heap = nullptr;
+};
+
+struct BatchPlan {
+  std::vector<uint64_t> sig;
+  std::vector<BatchGroup> groups;
+  std::vector<void *> allocs;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  uint32_t tq = 64, grid = 2048, passes = 3;
+  int reorder = 1;
+  bool useGraph = true;
+  template <typename U>
+  U *alloc(size_t count) {
+    void *p = nullptr;
+    HIP_CHECK(hipMalloc(&p, std::max<size_t>(count * sizeof(U), 256)));
+    allocs.push_back(p);
+    return reinterpret_cast<U *>(p);
+  }
+  void drop_graph() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
+  }
+  void free_all() {
+    drop_graph();
+    for (void *p : allocs) (void)hipFree(p);
+    allocs.clear();
+    groups.clear();
+    sig.clear();
+  }
+};
+static BatchPlan g_batch;
+// tuned (tile queries, grid blocks) of the last autotuned signature
+static std::vector<uint64_t> g_tunedSig;
+static uint32_t g_tunedTq = 0, g_tunedGrid = 0;
+
+constexpr uint32_t kBatchMaxGrid = 8192;
+
+static void batch_build(BatchPlan &B, uint32_t nsets, const uint32_t *n, const uint32_t *ndim,
+                        const uint32_t *k, uint32_t ndParam, hipStream_t s) {
+  B.free_all();
+  std::vector<int> groupOf(KNN_MAXDIM + 1, -1);
+  for (uint32_t si = 0; si < nsets; si++) {
+    const uint32_t d = ndim[si];
+    if (d == 0 || d > KNN_MAXDIM) throw Error(KNN_ERR_ARG, "set dimensionality must be in 1..16");
+    const uint32_t nd = ndParam ? ndParam : std::min<uint32_t>(d, 6);
+    if (nd > d || nd > KNN_MAX_INDEXED) throw Error(KNN_ERR_ARG, "NUMINDEXEDDIM must be <= NDIM and <= 8");
+    if (n[si] != 0 && (uint64_t)n[si] < (uint64_t)k[si] + 1)
+      throw Error(KNN_ERR_TOO_FEW, "a set has fewer than K+1 points");
+    if (n[si] == 0) continue;
+    if (groupOf[d] < 0) {
+      groupOf[d] = (int)B.groups.size();
+      B.groups.emplace_back();
+      B.groups.back().d = d;
+    }
+    BatchGroup &G = B.groups[groupOf[d]];
+    SetDesc sd{};
+    sd.pStart = G.N;
+    sd.n = n[si];
+    sd.k1 = k[si] + 1;
+    sd.nd = nd;
+    sd.outOff = G.outElems;
+    if ((uint64_t)G.N + n[si] > 0x7fffffffull) throw Error(KNN_ERR_ARG, "more than 2^31 points of one dimensionality");
+    G.N += n[si];
+    G.outElems += (uint64_t)n[si] * sd.k1;
+    G.k1max = std::max(G.k1max, sd.k1);
+    G.hsets.push_back(sd);
+    G.members.push_back(si);
+  }
+  for (BatchGroup &G : B.groups) {
+    G.nsets = (uint32_t)G.hsets.size();
+    const uint32_t N = G.N, d = G.d;
+    G.bound = std::min<uint64_t>(1ull << 27, std::max<uint64_t>(1ull << 16, 8ull * N));
+    G.bits = 1;
+    while ((1ull << G.bits) < G.bound) G.bits++;
+    G.smax = std::min<uint32_t>(2 * G.k1max + 32, 512);
+    const size_t sw = std::max({scan_scratch_words((uint32_t)G.bound + 1), scan_scratch_words(N + 2),
+                                radix_scratch_words(N)}) + 1024;
+    G.xin = B.alloc<float>((size_t)N * d);
+    G.x = B.alloc<float>((size_t)N * d);
+    G.xs = B.alloc<float>((size_t)N * d);
+    G.radius = B.alloc<float>(N);
+    G.width = B.alloc<float>(G.nsets);
+    G.mnmx = B.alloc<float>((size_t)G.nsets * 16);
+    G.var = B.alloc<float>((size_t)G.nsets * d);
+    G.perm = B.alloc<uint32_t>((size_t)G.nsets * KNN_MAXDIM);
+    G.setOf = B.alloc<uint32_t>(N);
+    G.keys = B.alloc<uint32_t>(N);
+    G.keysTmp = B.alloc<uint32_t>(N);
+    G.ids = B.alloc<uint32_t>(N);
+    G.idsTmp = B.alloc<uint32_t>(N);
+    G.cflags = B.alloc<uint32_t>((size_t)N + 2);
+    G.cpos = B.alloc<uint32_t>((size_t)N + 2);
+    G.cellKeys = B.alloc<uint64_t>((size_t)N + 1);
+    G.cellStart = B.alloc<uint32_t>((size_t)N + 2);
+    G.dense = B.alloc<uint32_t>(G.bound + 2);
+    G.aflags = B.alloc<uint32_t>((size_t)N + 2);
+    G.apos = B.alloc<uint32_t>((size_t)N + 2);
+    G.act = B.alloc<uint32_t>((size_t)N + 1);
+    G.ntiles = B.alloc<uint32_t>((size_t)N + 2);
+    G.tileOff = B.alloc<uint32_t>((size_t)N + 3);
+    G.tiles = B.alloc<uint4>((size_t)N + 1);
+    G.scratch = B.alloc<uint32_t>(sw);
+    G.status = B.alloc<uint32_t>(4);
+    G.setMaxBits = B.alloc<uint32_t>(G.nsets);
+    G.ctr = B.alloc<uint32_t>(8);
+    G.cand = B.alloc<unsigned long long>(4);
+    G.failed = B.alloc<uint8_t>(N);
+    G.overflow = B.alloc<uint8_t>(N);
+    G.sets = B.alloc<SetDesc>(G.nsets);
+    G.outIds = B.alloc<int32_t>(G.outElems);
+    G.outDists = B.alloc<float>(G.outElems);
+    const size_t heapBytes = (size_t)G.k1max * 64 * 12;
+    G.heap = heapBytes > 65536 ? B.alloc<unsigned char>(heapBytes * kBatchMaxGrid) : nullptr;
+    std::vector<uint32_t> so(N);
+    for (uint32_t i = 0; i < G.nsets; i++)
+      std::fill(so.begin() + G.hsets[i].pStart, so.begin() + G.hsets[i].pStart + G.hsets[i].n, i);
+    HIP_CHECK(hipMemcpyAsync(G.setOf, so.data(), sizeof(uint32_t) * N, hipMemcpyHostToDevice, s));
+    HIP_CHECK(hipMemcpyAsync(G.sets, G.hsets.data(), sizeof(SetDesc) * G.nsets, hipMemcpyHostToDevice, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+  }
+}
+
+static JoinArgs batch_args(const BatchPlan &B, const BatchGroup &G) {
+  JoinArgs a{};
+  a.x = G.x;
+  a.xs = G.xs;
+  a.sortedIds = G.ids;
+  a.cellKeys = G.cellKeys;
+  a.cellStart = G.cellStart;
+  a.ncell = 0;
+  a.denseStart = G.dense;
+  a.act = G.act;
+  a.tiles = G.tiles;
+  a.ntiles = B.grid;
+  a.n = G.N;
+  a.d = G.d;
+  a.k1 = G.k1max;
+  a.radius = G.radius;
+  a.outIds = G.outIds;
+  a.outDists = G.outDists;
+  a.failed = G.failed;
+  a.failCount = G.ctr + 1;
+  a.candCount = G.cand;
+  a.tileCounter = G.ctr;
+  a.heapScratch = G.heap;
+  a.smax = G.smax;
+  a.overflow = G.overflow;
+  a.overflowCount = G.ctr + 2;
+  a.sets = G.sets;
+  a.setOf = G.setOf;
+  a.ntilesDev = G.tileOff + G.N + 1;
+  return a;
+}
+
+// One radius pass of a group: per-set grids at the current widths, the group index, tiles
+// of the failed (initially all) queries, the select kernel, the heap kernel for staging
+// overflows, and radius growth of the queries that still failed.
+static void batch_pass(const BatchPlan &B, BatchGroup &G, hipStream_t s) {
+  const uint32_t N = G.N;
+  batch_grids(G.sets, G.nsets, G.mnmx, G.width, G.bound, G.status, s);
+  batch_cell_keys(G.x, N, G.sets, G.setOf, G.keys, G.ids, s);
+  radix_sort_pairs_u32(G.keys, G.ids, G.keysTmp, G.idsTmp, N, G.bits, G.scratch, s);
+  boundary_flags_u32(G.keys, N, G.cflags, s);
+  scan_exclusive_u32(G.cflags, G.cpos, N, G.scratch, s);
+  compact_cells_u32(G.keys, G.cflags, G.cpos, N, G.cellKeys, G.cellStart, s);
+  gather_soa<float>(G.x, G.ids, N, G.d, G.xs, s);
+  fill_u32(G.dense, 0u, G.bound + 1, s);
+  batch_dense_counts(G.cellKeys, G.cellStart, G.cpos + N, N, G.dense, s);
+  scan_exclusive_u32(G.dense, G.dense, (uint32_t)G.bound, G.scratch, s);
+
+  const JoinArgs a = batch_args(B, G);
+  const GridParams g0{};
+  batch_flags_u8(G.ids, G.failed, N, G.aflags, s);
+  scan_exclusive_u32(G.aflags, G.apos, N, G.scratch, s);
+  compact_active(G.ids, G.aflags, G.apos, N, G.act, s);
+  batch_tiles(G.cellStart, G.apos, G.cpos + N, N, B.tq, G.ntiles, G.tileOff, G.scratch, G.tiles, s);
+  fill_u32(G.ctr, 0u, 4, s);
+  range_select<float>(a, g0, s);
+
+  batch_flags_u8(G.ids, G.overflow, N, G.aflags, s);
+  scan_exclusive_u32(G.aflags, G.apos, N, G.scratch, s);
+  compact_active(G.ids, G.aflags, G.apos, N, G.act, s);
+  batch_tiles(G.cellStart, G.apos, G.cpos + N, N, B.tq, G.ntiles, G.tileOff, G.scratch, G.tiles, s);
+  fill_u32(G.ctr, 0u, 1, s);
+  range_topk<float>(a, g0, s);
+  fill_u8(G.overflow, 0, N, s);
+
+  batch_grow(G.setOf, G.failed, N, G.radius, G.setMaxBits, G.width, G.nsets, s);
+}
+
+static void batch_body(const BatchPlan &B, hipStream_t s) {
+  for (BatchGroup &G : const_cast<BatchPlan &>(B).groups) {
+    batch_variance_perm(G.xin, G.sets, G.nsets, G.d, B.reorder, G.var, G.perm, s);
+    batch_permute(G.xin, G.N, G.d, G.setOf, G.perm, G.x, s);
+    batch_minmax(G.x, G.N, G.sets, G.nsets, G.mnmx, s);
+    batch_init_radius(G.sets, G.nsets, G.mnmx, G.width, G.setOf, G.N, G.radius, G.failed, s);
+    fill_u8(G.overflow, 0, G.N, s);
+    fill_u32(reinterpret_cast<uint32_t *>(G.cand), 0u, 8, s);
+    for (uint32_t p = 0; p < B.passes; p++) batch_pass(B, G, s);
+  }
+}
+
+static int run_batched(uint32_t nsets, const float *const *points, const uint32_t *n,
+                       const uint32_t *ndim, const uint32_t *k, const knnjoin_batch_params &p,
+                       int32_t *const *outIds, float *const *outDists, knnjoin_batch_stats *st) {
+  if (nsets == 0) {
+    if (st) *st = knnjoin_batch_stats{};
+    return KNN_OK;
+  }
+  if (!points || !n || !ndim || !k || !outIds || !outDists) throw Error(KNN_ERR_ARG, "null argument");
+  hipStream_t s = lib_stream();
+  knnjoin_batch_stats stats{};
+  BatchPlan &B = g_batch;
+  std::vector<uint64_t> sig;
+  sig.reserve((size_t)nsets * 3 + 4);
+  for (uint32_t i = 0; i < nsets; i++) {
+    sig.push_back(n[i]);
+    sig.push_back(((uint64_t)ndim[i] << 32) | k[i]);
+  }
+  sig.push_back(p.num_indexed_dims);
+  uint32_t tq = p.tile_queries, grid = p.grid_blocks;
+  if ((!tq || !grid) && sig == g_tunedSig) {
+    if (!tq) tq = g_tunedTq;
+    if (!grid) grid = g_tunedGrid;
+  }
+  if (!tq) tq = 64;
+  if (!grid) grid = 2048;
+  if (tq > 64) throw Error(KNN_ERR_ARG, "tile_queries must be <= 64");
+  if (grid > kBatchMaxGrid) throw Error(KNN_ERR_ARG, "grid_blocks must be <= 8192");
+  const uint32_t passes = p.graph_passes ? p.graph_passes : 3;
+  Timer tAll(s);
+  auto w0 = std::chrono::steady_clock::now();
+  if (B.sig != sig) {
+    batch_build(B, nsets, n, ndim, k, p.num_indexed_dims, s);
+    B.sig = sig;
+  }
+  if (B.groups.empty()) {  // every set is empty
+    if (st) *st = stats;
+    return KNN_OK;
+  }
+  if (B.tq != tq || B.grid != grid || B.passes != passes || B.reorder != (p.reorder ? 1 : 0) ||
+      B.useGraph != (p.use_graph != 0)) {
+    B.drop_graph();
+    B.tq = tq;
+    B.grid = grid;
+    B.passes = passes;
+    B.reorder = p.reorder ? 1 : 0;
+    B.useGraph = p.use_graph != 0;
+  }
+  // inputs
+  const hipMemcpyKind kin = p.device_ptrs ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+  for (BatchGroup &G : B.groups)
+    for (uint32_t i = 0; i < G.nsets; i++) {
+      const uint32_t si = G.members[i];
+      if (!points[si]) throw Error(KNN_ERR_ARG, "null set pointer");
+      HIP_CHECK(hipMemcpyAsync(G.xin + (size_t)G.hsets[i].pStart * G.d, points[si],
+                               sizeof(float) * (size_t)n[si] * G.d, kin, s));
+    }
+  bool replay = false;
+  if (B.useGraph && !B.exec) {
+    HIP_CHECK(hipStreamSynchronize(s));
+    HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    batch_body(B, s);
+    hipGraph_t gr = nullptr;
+    HIP_CHECK(hipStreamEndCapture(s, &gr));
+    B.graph = gr;
+    HIP_CHECK(hipGraphInstantiate(&B.exec, B.graph, nullptr, nullptr, 0));
+  } else if (B.useGraph) {
+    replay = true;
+  }
+  HIP_CHECK(hipStreamSynchronize(s));
+  stats.ms_setup = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - w0).count();
+  tAll.start();
+  if (B.useGraph) {
+    HIP_CHECK(hipGraphLaunch(B.exec, s));
+  } else {
+    batch_body(B, s);
+  }
+  // leftover failures: eager passes until none remain
+  uint32_t eager = 0;
+  for (;;) {
+    bool any = false;
+    for (BatchGroup &G : B.groups) {
+      uint32_t f = 0;
+      HIP_CHECK(hipMemcpyAsync(&f, G.ctr + 1, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+      HIP_CHECK(hipStreamSynchronize(s));
+      if (f) {
+        any = true;
+        batch_pass(B, G, s);
+      }
+    }
+    if (!any) break;
+    if (++eager > 64) throw Error(KNN_ERR_NO_PROGRESS, "batched join: queries still failing after 64 extra passes");
+  }
+  stats.ms_device = tAll.stop();
+  for (BatchGroup &G : B.groups) {
+    unsigned long long c[2] = {0, 0};
+    HIP_CHECK(hipMemcpyAsync(c, G.cand, sizeof(c), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    stats.candidates += c[0];
+    stats.algo_bytes += c[1];
+    stats.points += G.N;
+  }
+  // outputs
+  const hipMemcpyKind kout = p.device_ptrs ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+  for (BatchGroup &G : B.groups)
+    for (uint32_t i = 0; i < G.nsets; i++) {
+      const uint32_t si = G.members[i];
+      const SetDesc &sd = G.hsets[i];
+      const size_t cnt = (size_t)sd.n * sd.k1;
+      if (!outIds[si] || !outDists[si]) throw Error(KNN_ERR_ARG, "null output pointer");
+      HIP_CHECK(hipMemcpyAsync(outIds[si], G.outIds + sd.outOff, sizeof(int32_t) * cnt, kout, s));
+      HIP_CHECK(hipMemcpyAsync(outDists[si], G.outDists + sd.outOff, sizeof(float) * cnt, kout, s));
+    }
+  HIP_CHECK(hipStreamSynchronize(s));
+  stats.groups = (uint32_t)B.groups.size();
+  stats.graph_replayed = replay ? 1u : 0u;
+  stats.passes_graph = B.passes;
+  stats.passes_eager = eager;
+  stats.tile_queries = B.tq;
+  stats.grid_blocks = B.grid;
+  if (p.verbose)
+    printf("[knn] batched: %u sets in %u groups, %llu points, device %.3f ms (%s), eager passes %u\n",
+           nsets, stats.groups, (unsigned long long)stats.points, stats.ms_device,
+           B.useGraph ? (replay ? "graph replay" : "graph, first launch") : "eager", eager);
+  if (st) *st = stats;
+  return KNN_OK;
+}
+
+}  // namespace knn
+
+extern "C" {
+
+void knnjoin_batch_default_params(knnjoin_batch_params *p) {
+  if (!p) return;
+  std::memset(p, 0, sizeof(*p));
+  p->reorder = 1;
+  p->use_graph = 1;
+}
+
+int knnjoin_batched(uint32_t nsets, const float *const *points, const uint32_t *n, const uint32_t *ndim,
+                    const uint32_t *k, const knnjoin_batch_params *params, int32_t *const *out_ids,
+                    float *const *out_dists, knnjoin_batch_stats *stats) {
+  knnjoin_batch_params p;
+  if (params) p = *params; else knnjoin_batch_default_params(&p);
+  return knn::guarded([&]() -> int {
+    return knn::run_batched(nsets, points, n, ndim, k, p, out_ids, out_dists, stats);
+  });
+}
+
+int knnjoin_batch_autotune(uint32_t nsets, const float *const *points, const uint32_t *n,
+                           const uint32_t *ndim, const uint32_t *k, const knnjoin_batch_params *params,
+                           int32_t *const *out_ids, float *const *out_dists, uint32_t *best_tq,
+                           uint32_t *best_grid, double *ms_table, uint32_t *nconfigs) {
+  knnjoin_batch_params p;
+  if (params) p = *params; else knnjoin_batch_default_params(&p);
+  return knn::guarded([&]() -> int {
+    using namespace knn;
+    static const uint32_t tqs[] = {16, 32, 64};
+    static const uint32_t grids[] = {1024, 2048, 4096, 8192};
+    double best = 1e300;
+    uint32_t bt = 64, bg = 2048, c = 0;
+    for (uint32_t tq : tqs)
+      for (uint32_t gr : grids) {
+        knnjoin_batch_params q = p;
+        q.tile_queries = tq;
+        q.grid_blocks = gr;
+        knnjoin_batch_stats st{};
+        run_batched(nsets, points, n, ndim, k, q, out_ids, out_dists, &st);  // build / capture
+        double ms = 1e300;
+        for (int r = 0; r < 2; r++) {
+          run_batched(nsets, points, n, ndim, k, q, out_ids, out_dists, &st);
+          ms = std::min(ms, st.ms_device);
+        }
+        if (ms_table) ms_table[c] = ms;
+        c++;
+        if (ms < best) {
+          best = ms;
+          bt = tq;
+          bg = gr;
+        }
+      }
+    std::vector<uint64_t> sig;
+    for (uint32_t i = 0; i < nsets; i++) {
+      sig.push_back(n[i]);
+      sig.push_back(((uint64_t)ndim[i] << 32) | k[i]);
+    }
+    sig.push_back(p.num_indexed_dims);
+    g_tunedSig = sig;
+    g_tunedTq = bt;
+    g_tunedGrid = bg;
+    if (best_tq) *best_tq = bt;
+    if (best_grid) *best_grid = bg;
+    if (nconfigs) *nconfigs = c;
+    return KNN_OK;
+  });
+}
+
+}  // extern "C"
+
+namespace knn {
+static void release_batch() { g_batch.free_all(); }
+}  // namespace knn

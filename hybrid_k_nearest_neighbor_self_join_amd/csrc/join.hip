@@ -109,7 +109,7 @@ __global__ void k_prefilter(const T *__restrict__ x, uint32_t n, G g, const T *r
   }
   uint32_t cnt = 0;
   for (;;) {
-    uint64_t rowBase = 0;
+    uint64_t rowBase = g.keyBase;
     for (uint32_t j = 1; j < g.nd; j++) rowBase += (uint64_t)cur[j] * g.mult[j];
     cnt += dense[rowBase + hi[0] + 1] - dense[rowBase + lo[0]];
     uint32_t j = 1;

@@ -25,6 +25,7 @@ struct GridParams {
   float minArr[KNN_MAX_INDEXED];
   uint32_t nCells[KNN_MAX_INDEXED];
   uint64_t mult[KNN_MAX_INDEXED];  // prod_{i<j} nCells_i
+  uint64_t keyBase;                 // added to every linear id (one key range per set)
   float eps;
   uint32_t nd;                      // indexed dims
 };
@@ -33,8 +34,20 @@ struct GridParamsD {
   double minArr[KNN_MAX_INDEXED];
   uint32_t nCells[KNN_MAX_INDEXED];
   uint64_t mult[KNN_MAX_INDEXED];
+  uint64_t keyBase;
   double eps;
   uint32_t nd;
+};
+
+// One point set of a batched join: its grid, its points [pStart, pStart + n) of the group
+// arrays, its K+1 and the first element of its rows in the group's output table.
+struct SetDesc {
+  GridParams g;
+  uint32_t pStart, n;
+  uint32_t k1, nd;
+  uint64_t outOff;
+  float r0;          // initial search radius
+  uint32_t pad;
 };
 
 template <typename T> struct GridOf;

@@ -121,7 +121,7 @@ struct JoinArgs {
   const uint32_t *denseStart;  // dense cell-start table, or null (binary search of cellKeys)
   const uint32_t *act;   // active query ids, grouped by cell
   const uint4 *tiles;    // {block, first active position, count, first tile of its block}
-  uint32_t ntiles;
+  uint32_t ntiles;       // tile count; with ntilesDev: the persistent grid size (blocks)
   uint32_t n, d, k1;
   const void *radius;    // per-query search radius (T); null = the grid's eps for all
   int32_t *outIds;
@@ -134,6 +134,11 @@ struct JoinArgs {
   uint32_t smax;         // select kernel: LDS staging slots per query
   uint8_t *overflow;     // select kernel: per point, 1 = needs the heap kernel
   uint32_t *overflowCount;
+  // batched joins: per-set grids / K+1 / row offsets (sets[setOf[q]]), and the tile count
+  // read from device memory at kernel start (null: single set, `ntiles`)
+  const SetDesc *sets;
+  const uint32_t *setOf;
+  const uint32_t *ntilesDev;
 };
 // Two-pass select kernel (histogram, then bucket-ordered collect); queries whose candidates
 // do not fit in `smax` staging slots are flagged in `overflow` for the heap kernel.
@@ -163,6 +168,8 @@ template <typename T>
 void radius_kernel(const T *x, uint32_t n, const typename GridOf<T>::type &g, const uint32_t *cnt,
                    float logk1, float logdens, const float *W, float margin, float wf, T *radius,
                    float *featOut, uint32_t *rhist, hipStream_t s);
+// occupied cells of a count table: *out += number of non-zero entries
+void count_nonzero(const uint32_t *cnt, uint64_t cells, unsigned long long *out, hipStream_t s);
 template <typename T>
 void grow_radius(T *radius, const uint8_t *failed, uint32_t n, T factor, hipStream_t s);
 // largest radius among failed queries into out (8 bytes: float bits or double bits)
@@ -176,6 +183,33 @@ void mlp_grad_batch(const float *W, const float *X, const float *Y, uint32_t B, 
                     float *loss, hipStream_t s);
 void adam_step(float *W, const float *G, float *M, float *V, float lr, float b1, float b2, float c1,
                float c2, float eps, hipStream_t s);
+
+// ------------------------------------------------------------- batched sets (batched.hip)
+// Group arrays hold the concatenated points of sets with one dimensionality d; sets[si]
+// describes set si, setOf[p] the set of point p.  Sizes are read from device memory where a
+// count is produced on the device (cells, tiles), so every launch below has a fixed shape.
+void batch_variance_perm(const float *xin, const SetDesc *sets, uint32_t nsets, uint32_t d, int reorder,
+                         float *var, uint32_t *perm, hipStream_t s);
+void batch_permute(const float *xin, uint32_t N, uint32_t d, const uint32_t *setOf, const uint32_t *perm,
+                   float *x, hipStream_t s);
+void batch_minmax(const float *x, uint32_t N, const SetDesc *sets, uint32_t nsets, float *mnmx,
+                  hipStream_t s);
+void batch_init_radius(SetDesc *sets, uint32_t nsets, const float *mnmx, float *width,
+                       const uint32_t *setOf, uint32_t N, float *radius, uint8_t *failed, hipStream_t s);
+void batch_grids(SetDesc *sets, uint32_t nsets, const float *mnmx, float *width, uint64_t bound,
+                 uint32_t *status, hipStream_t s);
+void batch_cell_keys(const float *x, uint32_t N, const SetDesc *sets, const uint32_t *setOf,
+                     uint32_t *keys, uint32_t *ids, hipStream_t s);
+void batch_dense_counts(const uint64_t *cellKeys, const uint32_t *cellStart, const uint32_t *ncell,
+                        uint32_t N, uint32_t *dense, hipStream_t s);
+void batch_flags_u8(const uint32_t *sortedIds, const uint8_t *f, uint32_t N, uint32_t *flags, hipStream_t s);
+// tiles of <= tq active queries per cell: ntiles/tileOff hold N + 2 words, tileOff[N + 1] =
+// tile count
+void batch_tiles(const uint32_t *cellStart, const uint32_t *pos, const uint32_t *ncell, uint32_t N,
+                 uint32_t tq, uint32_t *ntiles, uint32_t *tileOff, uint32_t *scratch, uint4 *tiles,
+                 hipStream_t s);
+void batch_grow(const uint32_t *setOf, const uint8_t *failed, uint32_t N, float *radius,
+                uint32_t *setMaxBits, float *width, uint32_t nsets, hipStream_t s);
 
 // deterministic sum of rowSum over the rows (chunks of 4096 rows, then chunk sums): out[0]
 void sum_rows(const double *rowSum, uint32_t n, double *partial, double *out, hipStream_t s);

@@ -12,7 +12,9 @@
 //           dim 0 inner, other indexed dims at the query's own coordinate; for nd == 1 the 9
 //           cells c-4..c+4 of dim 0); cells outside the grid count 0
 //   f10     log(K+1)
-//   f11     log(N / V)  (global density)
+//   f11     log(N / occupied feature cells): the mean population of a non-empty cell, a
+//           global clustering measure that, like every other feature, does not change when
+//           the same distribution is sampled with more points
 // Target: log(r_K1 / wf), r_K1 = distance to the (K+1)-th nearest point (self included).
 // Radius: r = wf * exp(mlp(f) + margin), margin calibrated on the training residuals.
 //
@@ -133,6 +135,14 @@ __global__ __launch_bounds__(256) void k_radius(const T *__restrict__ x, uint32_
   }
   __syncthreads();
   if (rhist && threadIdx.x < 64 && lh[threadIdx.x]) atomicAdd(&rhist[threadIdx.x], lh[threadIdx.x]);
+}
+
+// number of non-zero counts (occupied feature cells) into *out
+__global__ void k_count_nonzero(const uint32_t *cnt, uint64_t cells, unsigned long long *out) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool nz = i < cells && cnt[i] != 0u;
+  const uint64_t m = __ballot(nz);
+  if ((threadIdx.x & 63) == 0 && m) atomicAdd(out, (unsigned long long)__popcll(m));
 }
 
 // radius *= factor for the queries that failed the last pass
@@ -268,6 +278,10 @@ void radius_kernel(const T *x, uint32_t n, const typename GridOf<T>::type &g, co
   if (n)
     k_radius<T, typename GridOf<T>::type><<<(n + 255) / 256, 256, 0, s>>>(
         x, n, g, cnt, logk1, logdens, W, margin, wf, radius, featOut, rhist);
+}
+
+void count_nonzero(const uint32_t *cnt, uint64_t cells, unsigned long long *out, hipStream_t s) {
+  if (cells) k_count_nonzero<<<(unsigned)((cells + 255) / 256), 256, 0, s>>>(cnt, cells, out);
 }
 
 template <typename T>

@@ -32,6 +32,8 @@
 //   A query whose staged list exceeds `smax` LDS slots (e.g. thousands of duplicates in one
 //   bucket) is flagged for k_range_heap.
 // k_range_heap: per-lane max-heap of (key, id), in LDS when K+1 fits, else in global scratch.
+#include <type_traits>
+
 #include "knn_launch.h"
 
 namespace knn {
@@ -134,7 +136,7 @@ __device__ __forceinline__ void tile_setup(const JoinArgs &a, const G &g, const 
   L.Tq = (double)r * (double)r * (1.0 - 0x1p-20);
   L.F = Num<T>::bound(L.Tq);
   const T eps = (T)g.eps;
-  cellId = 0;
+  cellId = g.keyBase;
   for (uint32_t j = 0; j < g.nd; j++) {
     const T mn = (T)g.minArr[j];
     const uint32_t nc = g.nCells[j];
@@ -164,7 +166,7 @@ __device__ __forceinline__ void for_each_range(const JoinArgs &a, const G &g, co
   uint32_t cur[KNN_MAX_INDEXED];
   for (uint32_t j = 0; j < g.nd; j++) cur[j] = lo[j];
   for (;;) {
-    uint64_t rowBase = 0;
+    uint64_t rowBase = g.keyBase;
     for (uint32_t j = 1; j < g.nd; j++) rowBase += (uint64_t)cur[j] * g.mult[j];
     uint32_t pBeg, pEnd;
     if (a.denseStart) {
@@ -194,6 +196,36 @@ __device__ __forceinline__ uint32_t next_tile(uint32_t *counter, uint32_t lane) 
   uint32_t t = 0;
   if (lane == 0) t = atomicAdd(counter, 1u);
   return (uint32_t)__shfl((int)t, 0, 64);
+}
+
+// The grid, K+1 and row addressing of one tile: the kernel's own for a single set; the
+// tile's set (all queries of a tile share one cell, hence one set) in a batched join.
+template <typename G>
+struct TileCtx {
+  G g;
+  uint32_t k1;
+  int64_t rowAdj;  // row of query q starts at element (q * k1 + rowAdj)
+  __device__ __forceinline__ size_t row(uint32_t q) const { return (size_t)((int64_t)q * k1 + rowAdj); }
+};
+template <typename G>
+__device__ __forceinline__ TileCtx<G> tile_ctx(const JoinArgs &a, const G &g, const uint4 &tl) {
+  TileCtx<G> c;
+  c.g = g;
+  c.k1 = a.k1;
+  c.rowAdj = 0;
+  if constexpr (std::is_same<G, GridParams>::value) {
+    if (a.sets) {
+      const SetDesc &sd = a.sets[a.setOf[a.act[tl.y]]];
+      c.g = sd.g;
+      c.k1 = sd.k1;
+      c.rowAdj = (int64_t)sd.outOff - (int64_t)sd.pStart * sd.k1;
+    }
+  }
+  return c;
+}
+
+__device__ __forceinline__ uint32_t tile_count(const JoinArgs &a) {
+  return a.ntilesDev ? *a.ntilesDev : a.ntiles;
 }
 
 template <typename T, int D>
@@ -234,7 +266,7 @@ __device__ __forceinline__ uint32_t bucket_of(T s, T scale) {
 __device__ __forceinline__ uint32_t hword(uint32_t b, uint32_t col) { return (b >> 1) * 64 + col; }
 __device__ __forceinline__ uint32_t hshift(uint32_t b) { return (b & 1u) << 4; }
 template <typename T, int D, typename G>
-__global__ __launch_bounds__(64) void k_range_select(JoinArgs a, G g) {
+__global__ __launch_bounds__(64) void k_range_select(JoinArgs a, G g0) {
   extern __shared__ __align__(16) unsigned char smem[];
   uint32_t *hist = reinterpret_cast<uint32_t *>(smem);
   const uint32_t SM = a.smax;
@@ -242,15 +274,18 @@ __global__ __launch_bounds__(64) void k_range_select(JoinArgs a, G g) {
   uint32_t *sid = reinterpret_cast<uint32_t *>(skey + SM);
   T *ss = reinterpret_cast<T *>(sid + SM + (SM & 1));
   const uint32_t lane = threadIdx.x;
-  const uint32_t K1 = a.k1;
   const T *__restrict__ xs = reinterpret_cast<const T *>(a.xs);
   const size_t n = a.n;
-  const T eps = (T)g.eps;
+  const uint32_t NT = tile_count(a);
 
   for (;;) {
     const uint32_t t = next_tile(a.tileCounter, lane);
-    if (t >= a.ntiles) break;
+    if (t >= NT) break;
     const uint4 tl = a.tiles[t];
+    const TileCtx<G> C = tile_ctx<G>(a, g0, tl);
+    const G &g = C.g;
+    const uint32_t K1 = C.k1;
+    const T eps = (T)g.eps;
     Lane<T, D> L;
     uint32_t lo[KNN_MAX_INDEXED], hi[KNN_MAX_INDEXED], ownLo[KNN_MAX_INDEXED], ownHi[KNN_MAX_INDEXED];
     uint64_t cellId;
@@ -397,8 +432,8 @@ __global__ __launch_bounds__(64) void k_range_select(JoinArgs a, G g) {
         for (int u = 0; u < MAXI; u++) {
           const uint32_t tI = lane + 64u * u;
           if (tI < m && rk[u] < K1) {
-            a.outIds[(size_t)qid * K1 + rk[u]] = (int32_t)sid[tI];
-            od[(size_t)qid * K1 + rk[u]] = sqrt(ss[tI]);
+            a.outIds[C.row(qid) + rk[u]] = (int32_t)sid[tI];
+            od[C.row(qid) + rk[u]] = sqrt(ss[tI]);
           }
         }
       }
@@ -484,29 +519,33 @@ struct Heap {
 };
 
 template <typename T, int D, bool LDS, typename G>
-__global__ __launch_bounds__(64) void k_range_heap(JoinArgs a, G g) {
+__global__ __launch_bounds__(64) void k_range_heap(JoinArgs a, G g0) {
   extern __shared__ __align__(16) unsigned char smem[];
   const uint32_t lane = threadIdx.x;
-  const uint32_t K1 = a.k1;
+  const uint32_t KMAX = a.k1;  // heap capacity (the largest K+1 of a batch)
   Heap h;
   h.lane = lane;
   if (LDS) {
     h.key = reinterpret_cast<double *>(smem);
-    h.id = reinterpret_cast<uint32_t *>(smem + (size_t)K1 * 64 * sizeof(double));
+    h.id = reinterpret_cast<uint32_t *>(smem + (size_t)KMAX * 64 * sizeof(double));
   } else {
     unsigned char *base =
-        reinterpret_cast<unsigned char *>(a.heapScratch) + (size_t)blockIdx.x * K1 * 64 * 12;
+        reinterpret_cast<unsigned char *>(a.heapScratch) + (size_t)blockIdx.x * KMAX * 64 * 12;
     h.key = reinterpret_cast<double *>(base);
-    h.id = reinterpret_cast<uint32_t *>(base + (size_t)K1 * 64 * sizeof(double));
+    h.id = reinterpret_cast<uint32_t *>(base + (size_t)KMAX * 64 * sizeof(double));
   }
   const T *__restrict__ x = reinterpret_cast<const T *>(a.x);
   const T *__restrict__ xs = reinterpret_cast<const T *>(a.xs);
   const size_t n = a.n;
+  const uint32_t NT = tile_count(a);
 
   for (;;) {
     const uint32_t t = next_tile(a.tileCounter, lane);
-    if (t >= a.ntiles) break;
+    if (t >= NT) break;
     const uint4 tl = a.tiles[t];
+    const TileCtx<G> C = tile_ctx<G>(a, g0, tl);
+    const G &g = C.g;
+    const uint32_t K1 = C.k1;
     Lane<T, D> L;
     uint32_t lo[KNN_MAX_INDEXED], hi[KNN_MAX_INDEXED], ownLo[KNN_MAX_INDEXED], ownHi[KNN_MAX_INDEXED];
     uint64_t cellId;
@@ -563,8 +602,8 @@ __global__ __launch_bounds__(64) void k_range_heap(JoinArgs a, G g) {
         T c[D];
 #pragma unroll
         for (int j = 0; j < D; j++) c[j] = x[(size_t)j * n + id];
-        a.outIds[(size_t)L.q * K1 + e] = (int32_t)id;
-        od[(size_t)L.q * K1 + e] = sqrt(Num<T>::template sq<D>(c, L.qc));
+        a.outIds[C.row(L.q) + e] = (int32_t)id;
+        od[C.row(L.q) + e] = sqrt(Num<T>::template sq<D>(c, L.qc));
       }
       a.failed[L.q] = 0;
     } else if (L.active) {
@@ -581,7 +620,8 @@ template <typename T, int D, typename G>
 void launch_heap(const JoinArgs &a, const G &g, hipStream_t s) {
   if (a.ntiles == 0) return;
   const size_t heapBytes = (size_t)a.k1 * 64 * 12;
-  const uint32_t grid = a.ntiles < HEAP_MAX_GRID ? a.ntiles : HEAP_MAX_GRID;
+  // with ntilesDev, ntiles is the persistent grid size (the tile count is on the device)
+  const uint32_t grid = a.ntilesDev ? a.ntiles : (a.ntiles < HEAP_MAX_GRID ? a.ntiles : HEAP_MAX_GRID);
   if (heapBytes <= MAX_LDS_BYTES)
     k_range_heap<T, D, true, G><<<grid, 64, heapBytes, s>>>(a, g);
   else
@@ -591,7 +631,8 @@ void launch_heap(const JoinArgs &a, const G &g, hipStream_t s) {
 template <typename T, int D, typename G>
 void launch_select(const JoinArgs &a, const G &g, hipStream_t s) {
   if (a.ntiles == 0) return;
-  k_range_select<T, D, G><<<range_select_grid(a.ntiles), 64, range_select_lds_bytes<T>(a.smax), s>>>(a, g);
+  const uint32_t grid = a.ntilesDev ? a.ntiles : range_select_grid(a.ntiles);
+  k_range_select<T, D, G><<<grid, 64, range_select_lds_bytes<T>(a.smax), s>>>(a, g);
 }
 
 }  // namespace

@@ -52,18 +52,20 @@ def skewed_cube(n: int, d: int, seed: int = 0x5EED0005, lam: float = 4.0) -> np.
 
 def batched_sets(nsets: int = 256, seed: int = 0x5EED0006, lo: int = 1000, hi: int = 200000,
                  dims=(2, 3, 4, 6), ks=(8, 32)):
-    """Returns a list of (points[N,d] float32, k)."""
+    """Many independent sets: sizes log-uniform in [lo, hi], d and k drawn from `dims` / `ks`;
+    each set is 70% uniform in [0,1]^d plus 30% from 2..16 Gaussian clusters (sigma 0.01,
+    centres uniform), the mix of the 10M mixed_box set.  Returns a list of (points, k)."""
     rng = np.random.default_rng(seed)
     sets = []
-    for i in range(nsets):
+    for _ in range(nsets):
         n = int(np.exp(rng.uniform(np.log(lo), np.log(hi))))
         d = int(rng.choice(dims))
         k = int(rng.choice(ks))
-        if rng.uniform() < 0.7:
-            pts = rng.uniform(0.0, 1.0, (n, d))
-        else:
-            nc = int(rng.integers(2, 16))
-            centres = rng.uniform(0.0, 1.0, (nc, d))
-            pts = centres[rng.integers(0, nc, n)] + rng.normal(0.0, 0.02, (n, d))
+        nu = int(round(0.7 * n))
+        nc = int(rng.integers(2, 17))
+        centres = rng.uniform(0.0, 1.0, (nc, d))
+        pts = np.concatenate([rng.uniform(0.0, 1.0, (nu, d)),
+                              centres[rng.integers(0, nc, n - nu)] + rng.normal(0.0, 0.01, (n - nu, d))])
+        pts = pts[rng.permutation(n)]
         sets.append((np.ascontiguousarray(pts.astype(np.float32)), k))
     return sets

@@ -172,6 +172,52 @@ int knnjoin_mlp_grad(const float *weights, const float *X, const float *Y, uint3
 int knnjoin_mlp_train(float *weights, const float *X, const float *Y, uint32_t n, uint32_t epochs,
                       uint32_t batch, float lr, uint64_t seed, float *loss_per_epoch);
 
+/* ---------------------------------------------------------------------------------------
+ * Batched joins: many independent point sets (float32), each joined as knnjoin_ex would
+ * join it alone (same rows, same distance bits), processed together: one index and one
+ * persistent range-kernel launch per radius pass for all sets of a dimensionality, the whole
+ * join captured in a hipGraph and replayed for later calls with the same set shapes.
+ * No counterpart in the reference (one set per process, main.cu:109-510); for callers with
+ * many small sets, where per-set launches and host round trips dominate.
+ * ------------------------------------------------------------------------------------- */
+typedef struct {
+  uint32_t num_indexed_dims; /* per set: 0 = min(NDIM, 6) */
+  int32_t reorder;           /* 1 = reorder each set's dims by variance */
+  int32_t device_ptrs;       /* 1 = set points and outputs are device pointers */
+  int32_t use_graph;         /* 1 = capture the join in a hipGraph, replay for equal shapes */
+  uint32_t graph_passes;     /* radius passes inside the graph (0 = 3); later passes run eagerly */
+  uint32_t tile_queries;     /* queries per tile, <= 64 (0 = autotuned value or 64) */
+  uint32_t grid_blocks;      /* persistent range-kernel blocks, <= 8192 (0 = autotuned or 2048) */
+  int32_t verbose;
+} knnjoin_batch_params;
+
+typedef struct {
+  double ms_device;          /* device time of the join (graph + eager passes), HIP events */
+  double ms_setup;           /* host time before the launch: plan build, capture, input copies */
+  uint32_t groups;           /* distinct dimensionalities */
+  uint32_t graph_replayed;   /* 1 = an existing graph was replayed */
+  uint32_t passes_graph, passes_eager;
+  uint32_t tile_queries, grid_blocks;
+  uint64_t points, candidates, algo_bytes;
+} knnjoin_batch_stats;
+
+void knnjoin_batch_default_params(knnjoin_batch_params *p);
+
+/* Set i: points[i] (row-major n[i] x ndim[i]), k[i]; outputs out_ids[i] (int32, n[i] x (k[i]+1))
+ * and out_dists[i] (float32, same shape), the rows knnjoin_ex gives for that set alone. */
+int knnjoin_batched(uint32_t nsets, const float *const *points, const uint32_t *n, const uint32_t *ndim,
+                    const uint32_t *k, const knnjoin_batch_params *params, int32_t *const *out_ids,
+                    float *const *out_dists, knnjoin_batch_stats *stats);
+
+/* Times knnjoin_batched over tile_queries x grid_blocks candidates ({16,32,64} x
+ * {1024,2048,4096,8192}) on these sets, keeps the fastest for later calls with the same
+ * shapes (parameters left 0), and returns it; ms_table (12 entries) gets each candidate's
+ * best device time.  The outputs hold the last run's rows. */
+int knnjoin_batch_autotune(uint32_t nsets, const float *const *points, const uint32_t *n,
+                           const uint32_t *ndim, const uint32_t *k, const knnjoin_batch_params *params,
+                           int32_t *const *out_ids, float *const *out_dists, uint32_t *best_tq,
+                           uint32_t *best_grid, double *ms_table, uint32_t *nconfigs);
+
 #ifdef __cplusplus
 }
 #endif

@@ -393,7 +393,9 @@ void orc_radius_features(const float *pts, uint32_t n, uint32_t d, uint32_t k, u
     cnt[id]++;
   }
   float logk1 = logf((float)(k + 1));
-  float logdens = (float)log((double)n / V);
+  uint64_t occ = 0;
+  for (uint64_t c = 0; c < tot; c++) occ += cnt[c] != 0;
+  float logdens = (float)log((double)n / (double)occ);
   for (uint64_t q = 0; q < n; q++) {
     const int64_t *c = cc + q * nd;
     uint64_t own = 0;

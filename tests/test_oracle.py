@@ -275,5 +275,11 @@ def test_oracle_features_definition(oracle):
     pts = np.concatenate([g + 0.25, g + 0.5, g + 0.75]).astype(np.float32)
     f, wf = oracle.radius_features(pts, 2, 2)
     assert f.shape == (300, 12)
-    assert np.allclose(f[:, 10], np.log(3.0)) and np.allclose(f[:, 11], np.log(300 / (9.5 * 9.5)))
+    lo = pts.min(0) - np.float32(wf)
+    cells = np.floor((pts - lo) / np.float32(wf)).astype(np.int64)
+    occ = len(np.unique(cells, axis=0))
+    assert np.allclose(f[:, 10], np.log(3.0)) and np.allclose(f[:, 11], np.log(300 / occ))
+    # own-cell counts are the lattice's: every point's cell holds the points sharing it
+    _, inv, cnt = np.unique(cells, axis=0, return_inverse=True, return_counts=True)
+    assert np.allclose(f[:, 0], np.log1p(cnt[inv.ravel()]))
     assert np.all(f[:, 0] > 0) and np.all(f[:, 1:10].max(1) >= f[:, 0] - 1e-6)
