@@ -45,7 +45,7 @@ enum Slot {
   S_PTS, S_X, S_XS, S_KEYS, S_KEYS_TMP, S_IDS, S_IDS_TMP, S_CELLKEYS, S_CELLSTART, S_FLAGS, S_POS,
   S_ACT, S_NTILES, S_TILEOFF, S_TILES, S_FAILED, S_ROWSUM, S_OUTIDS, S_OUTDISTS, S_SCRATCH,
   S_HEAP, S_SMALL, S_PART, S_CTR, S_CSUM, S_HOSTFLAGS, S_OVERFLOW, S_STAGE, S_DENSE, S_BFLAGS,
-  S_BPOS, S_BSTART, S_CELLOF, S_CELLMAX, S_FCNT, S_RADIUS, S_RHIST, S_MODEL, S_TRAIN, S_OCC, S_NSLOTS
+  S_BPOS, S_BSTART, S_CELLOF, S_CELLMAX, S_FCNT, S_RADIUS, S_RHIST, S_MODEL, S_TRAIN, S_OCC, S_ACTQ, S_NSLOTS
 };
 
 struct Cache {
@@ -801,9 +801,9 @@ static void run_plain(Pipeline<T> &P, double beta, knnjoin_stats &stats, uint8_t
   stats.kernel_launches = launches;
 }
 
-// The learned schedule: per-query initial radius from the MLP, passes on one fine grid, and
-// the expansion loop (radius x2 for the failed queries, a coarser grid when the reach grows
-// past 48 cells) until every query has its K+1 neighbours.
+// The learned schedule: per-query initial radius from the MLP, then passes over radius
+// octaves (each on a grid matched to its radii) with the expansion loop (radius x2 for the
+// queries that fail) until every query has its K+1 neighbours.
 template <typename T>
 static void run_learned(Pipeline<T> &P, const knnjoin_params &p, knnjoin_stats &stats,
                         uint8_t *failed, int32_t *dIds, T *dDists, Timer &tPh, Timer &tK) {
@@ -831,79 +831,89 @@ static void run_learned(Pipeline<T> &P, const knnjoin_params &p, knnjoin_stats &
   float *Wd = P.template buf<float>(S_MODEL, MLP_NW);
   HIP_CHECK(hipMemcpyAsync(Wd, M.w.data(), sizeof(float) * MLP_NW, hipMemcpyHostToDevice, s));
   T *radius = P.template buf<T>(S_RADIUS, n);
-  uint32_t *rh = P.template buf<uint32_t>(S_RHIST, 64);
-  fill_u32(rh, 0u, 64, s);
-  radius_kernel<T>(P.x, n, F.g, cnt, F.logk1, F.logdens, Wd, M.margin, (float)F.wf, radius, nullptr, rh, s);
-  uint32_t hh[64];
-  HIP_CHECK(hipMemcpyAsync(hh, rh, sizeof(hh), hipMemcpyDeviceToHost, s));
-  HIP_CHECK(hipStreamSynchronize(s));
+  radius_kernel<T>(P.x, n, F.g, cnt, F.logk1, F.logdens, Wd, M.margin, (float)F.wf, radius, nullptr,
+                   nullptr, s);
   stats.ms_eps = tPh.stop();
-
-  // ---- join grid: cell width near the low end of the radius distribution (5th percentile
-  // for <= 2 indexed dims, the median above, where each extra cell of reach multiplies the
-  // rows), but no finer than 1/24 of the 95th percentile
-  auto quantile = [&](double qf, bool upper) {
-    uint64_t c = 0;
-    for (int b = 0; b < 64; b++) {
-      c += hh[b];
-      if ((double)c >= qf * n) return (double)F.wf * std::exp2(((upper ? b + 1 : b) - 32) / 4.0);
-    }
-    return (double)F.wf * std::exp2(8.0);
-  };
-  const double rlo = quantile(nd <= 2 ? 0.05 : 0.5, false);
-  const double rmed = quantile(0.5, true);
-  const double rhi = quantile(0.95, true);
-  T wj = (T)std::max(rlo, rhi / 24.0);
-  if (!(wj > (T)0)) wj = F.wf;
-  for (;;) {
-    typename GridOf<T>::type gtest;
-    uint64_t tc;
-    grid_dims<T>(hmn, hmx, nd, wj, gtest, tc);
-    if (tc <= (1ull << 27) || nd > 2) break;
-    wj = wj * (T)1.25;
-  }
-  P.avgReach = std::max(1.0, rmed / (double)wj);
-  stats.eps0 = (float)wj;
-  tPh.start();
-  P.build_index(hmn, hmx, wj);
-  stats.ms_index = tPh.stop();
   P.radius = radius;
 
+  // ---- radius classes: the pending queries whose radius lies in the lowest occupied octave
+  // [R/2, R) are joined on a grid of cell width R/2 (a query's block then spans at most 5
+  // cells per indexed dim and about 2.4x the area of its ball); queries that fail double
+  // their radius, which moves them to a later octave.  One index build per occupied octave.
   unsigned long long *ctr = P.template buf<unsigned long long>(S_CTR, 64);
   fill_u32(reinterpret_cast<uint32_t *>(ctr), 0u, 16, s);
   uint64_t *maxr = reinterpret_cast<uint64_t *>(ctr + 8);
-  uint32_t nq = n, passes = 0, reindexes = 0;
-  bool first = true;
+  uint32_t *ph = P.template buf<uint32_t>(S_RHIST, 64);
+  uint8_t *act = P.template buf<uint8_t>(S_ACTQ, n);
+  fill_u8(failed, 1, n, s);  // pending
+  const float wfF = (float)F.wf;
+  uint32_t passes = 0, reindexes = 0;
   double joinMs = 0, kMs = 0, kMax = 0;
   uint64_t bytesAtMax = 0, prevBytes = 0;
-  while (nq != 0) {
-    if (!first) {
-      grow_radius<T>(radius, failed, n, (T)2, s);
+  T curW = (T)0;
+  stats.eps0 = 0.0f;
+  for (;;) {
+    fill_u32(ph, 0u, 64, s);
+    radius_bins(radius, failed, n, wfF, ph, s);
+    uint32_t h[64];
+    HIP_CHECK(hipMemcpyAsync(h, ph, sizeof(h), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    uint64_t pending = 0;
+    int c = -1;
+    for (int b = 0; b < 64; b++) {
+      pending += h[b];
+      if (c < 0 && h[b]) c = b;
+    }
+    if (pending == 0) break;
+    if (passes > 200) throw Error(KNN_ERR_NO_PROGRESS, "learned schedule: queries still failing after 200 passes");
+    const int cHi = std::min(c + 3, 63);
+    uint32_t nq = 0;
+    for (int b = c; b <= cHi; b++) nq += h[b];
+    double R;
+    if (cHi < 63) {
+      R = (double)wfF * std::exp2((cHi + 1 - 32) / 4.0);
+    } else {
+      // the top bin is open-ended: the class's largest radius bounds it
       max_failed_radius(radius, failed, n, maxr, s);
       uint64_t mb = 0;
       HIP_CHECK(hipMemcpyAsync(&mb, maxr, sizeof(mb), hipMemcpyDeviceToHost, s));
       HIP_CHECK(hipStreamSynchronize(s));
-      double rmax;
       if (sizeof(T) == 4) {
         uint32_t b32 = (uint32_t)mb;
         float f;
         std::memcpy(&f, &b32, 4);
-        rmax = f;
+        R = f;
       } else {
-        double dd;
-        std::memcpy(&dd, &mb, 8);
-        rmax = dd;
+        std::memcpy(&R, &mb, 8);
       }
-      if (rmax / (double)P.g.eps > 48.0) {
-        tPh.start();
-        P.build_index(hmn, hmx, (T)(rmax / 8.0));
-        stats.ms_index += tPh.stop();
-        reindexes++;
-      }
-      P.avgReach = std::max(1.0, rmax / (double)P.g.eps);
+      R = R * (1.0 + 1e-6);
     }
+    T w = (T)(R / 2.0);
+    if (!(w > (T)0)) w = F.wf;
+    for (;;) {  // keep the grid within 2^30 cells
+      typename GridOf<T>::type gt;
+      uint64_t tc = 0;
+      bool ok = true;
+      try {
+        grid_dims<T>(hmn, hmx, nd, w, gt, tc);
+      } catch (const Error &) {
+        ok = false;
+      }
+      if (ok && tc <= (1ull << 30)) break;
+      w = w * (T)2;
+    }
+    if (w != curW) {
+      tPh.start();
+      P.build_index(hmn, hmx, w);
+      stats.ms_index += tPh.stop();
+      if (curW != (T)0) reindexes++;
+      curW = w;
+      if (stats.eps0 == 0.0f) stats.eps0 = (float)w;
+    }
+    P.avgReach = 1.5;
+    radius_class_flags(radius, failed, n, wfF, (uint32_t)c, (uint32_t)cHi, act, s);
     tPh.start();
-    PassResult pr = P.pass(first ? nullptr : failed, nq, failed, dIds, dDists, ctr, tK);
+    PassResult pr = P.pass(act, nq, failed, dIds, dDists, ctr, tK);
     stats.overflow_queries += pr.noverflow;
     kMs += pr.kernel_ms;
     if (pr.kernel_ms > kMax) {
@@ -911,13 +921,12 @@ static void run_learned(Pipeline<T> &P, const knnjoin_params &p, knnjoin_stats &
       bytesAtMax = pr.bytes_total - prevBytes;
     }
     prevBytes = pr.bytes_total;
+    grow_class_radius(radius, act, failed, n, (T)2, s);
     joinMs += tPh.stop();
     if (P.verbose)
-      printf("[knn] learned pass %u grid %.9g queries %u failed %u tiles %u cells %u span %u overflow %u kernel %.3f ms\n",
-             passes, (double)P.g.eps, nq, pr.nfail, pr.ntiles, P.ncell, P.lastSpan, pr.noverflow, pr.kernel_ms);
-    nq = pr.nfail;
+      printf("[knn] learned pass %u octave %d grid %.9g queries %u failed %u tiles %u cells %u span %u overflow %u kernel %.3f ms\n",
+             passes, c, (double)P.g.eps, nq, pr.nfail, pr.ntiles, P.ncell, P.lastSpan, pr.noverflow, pr.kernel_ms);
     passes++;
-    first = false;
   }
   stats.ms_join = joinMs;
   stats.ms_range_kernel = kMs;

@@ -16,24 +16,60 @@ namespace knn {
 
 namespace {
 
+// One block per dim.  Waves 1..3 stage the sampled values (rows 0,100,...) of the next chunk
+// in LDS while lane 0 of wave 0 adds the current chunk in row order, so the sequential sum
+// (whose rounding decides the dimension order) runs at LDS rather than HBM latency.
+constexpr uint32_t VAR_CHUNK = 2048;
 template <typename T>
-__global__ void k_variance(const T *__restrict__ pts, uint32_t n, uint32_t d, T *var) {
-  const uint32_t i = threadIdx.x;
-  if (i >= d) return;
+__device__ __forceinline__ void stage_chunk(const T *__restrict__ pts, uint32_t d, uint32_t i, uint64_t base,
+                                            uint32_t cnt, T *buf) {
+  if (threadIdx.x < 64) return;
+  for (uint32_t t = threadIdx.x - 64; t < cnt; t += blockDim.x - 64)
+    buf[t] = pts[(base + t) * 100 * (uint64_t)d + i];
+}
+template <typename T, bool SQ>
+__device__ T sequential_pass(const T *__restrict__ pts, uint32_t n, uint32_t d, uint32_t i, T avg,
+                             T (*buf)[VAR_CHUNK]) {
+  const uint64_t m = ((uint64_t)n + 99) / 100;
+  T sum = 0;
+  uint32_t cur = 0;
+  if (m) stage_chunk<T>(pts, d, i, 0, (uint32_t)(m < VAR_CHUNK ? m : VAR_CHUNK), buf[0]);
+  __syncthreads();
+  for (uint64_t base = 0; base < m; base += VAR_CHUNK) {
+    const uint32_t cnt = (uint32_t)(m - base < VAR_CHUNK ? m - base : VAR_CHUNK);
+    const uint64_t nb = base + VAR_CHUNK;
+    if (nb < m) stage_chunk<T>(pts, d, i, nb, (uint32_t)(m - nb < VAR_CHUNK ? m - nb : VAR_CHUNK), buf[cur ^ 1]);
+    if (threadIdx.x == 0) {
+      const T *b = buf[cur];
+      for (uint32_t t = 0; t < cnt; t++) {
+        if (SQ) {
+          T df = b[t] - avg;
+          T sq = df * df;
+          sum += sq;
+        } else {
+          sum += b[t];
+        }
+      }
+    }
+    __syncthreads();
+    cur ^= 1;
+  }
+  return sum;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_variance(const T *__restrict__ pts, uint32_t n, uint32_t d, T *var) {
+  __shared__ T buf[2][VAR_CHUNK];
+  __shared__ T shAvg;
+  const uint32_t i = blockIdx.x;
   const T inv = (T)(1.0 / (100 * 1.0));
   const T denom = (T)n * inv;
-  T sum = 0;
-#pragma unroll 16
-  for (uint64_t j = 0; j < n; j += 100) sum += pts[j * d + i];
-  const T avg = sum / denom;
-  sum = 0;
-#pragma unroll 16
-  for (uint64_t j = 0; j < n; j += 100) {
-    T df = pts[j * d + i] - avg;
-    T sq = df * df;
-    sum += sq;
-  }
-  var[i] = sum / denom;
+  const T sum = sequential_pass<T, false>(pts, n, d, i, (T)0, buf);
+  if (threadIdx.x == 0) shAvg = sum / denom;
+  __syncthreads();
+  const T avg = shAvg;
+  const T sum2 = sequential_pass<T, true>(pts, n, d, i, avg, buf);
+  if (threadIdx.x == 0) var[i] = sum2 / denom;
 }
 
 struct Perm {
@@ -133,7 +169,7 @@ __global__ __launch_bounds__(256) void k_kdist(const T *__restrict__ x, uint32_t
 
 template <typename T>
 void variance_rows(const T *pts, uint32_t n, uint32_t d, T *var, hipStream_t s) {
-  k_variance<T><<<1, 64, 0, s>>>(pts, n, d, var);
+  k_variance<T><<<d, 256, 0, s>>>(pts, n, d, var);
 }
 
 template <typename T>

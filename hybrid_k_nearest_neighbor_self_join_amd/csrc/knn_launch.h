@@ -168,6 +168,17 @@ template <typename T>
 void radius_kernel(const T *x, uint32_t n, const typename GridOf<T>::type &g, const uint32_t *cnt,
                    float logk1, float logdens, const float *W, float margin, float wf, T *radius,
                    float *featOut, uint32_t *rhist, hipStream_t s);
+// radius octaves of the learned schedule: histogram of the pending queries' quarter-octave
+// bins floor(4 log2(r / wf)) + 32 (hist zeroed by the caller, 64 bins); flags of the pending
+// queries with a bin in [lo, hi]; radius *= factor for flagged queries that failed
+template <typename T>
+void radius_bins(const T *radius, const uint8_t *pending, uint32_t n, float wf, uint32_t *hist, hipStream_t s);
+template <typename T>
+void radius_class_flags(const T *radius, const uint8_t *pending, uint32_t n, float wf, uint32_t lo,
+                        uint32_t hi, uint8_t *act, hipStream_t s);
+template <typename T>
+void grow_class_radius(T *radius, const uint8_t *act, const uint8_t *failed, uint32_t n, T factor,
+                       hipStream_t s);
 // occupied cells of a count table: *out += number of non-zero entries
 void count_nonzero(const uint32_t *cnt, uint64_t cells, unsigned long long *out, hipStream_t s);
 template <typename T>

@@ -29,6 +29,17 @@ namespace {
 
 __device__ __forceinline__ float relu(float v) { return v > 0.0f ? v : 0.0f; }
 
+// quarter-octave bin of a radius relative to the feature-grid width: floor(4 log2(r / wf)) + 32,
+// clamped to [0, 63]; zero, NaN and tiny radii fall in bin 0, huge ones in bin 63
+template <typename T>
+__device__ __forceinline__ int radius_bin(T r, float wf) {
+  const float x = 4.0f * log2f((float)r / wf);
+  if (!(x >= -32.0f)) return 0;
+  if (x >= 31.0f) return 63;
+  const int b = (int)floorf(x) + 32;
+  return b < 0 ? 0 : (b > 63 ? 63 : b);
+}
+
 // Forward pass of one sample; W = flat weights (LDS).  Layer 1 unit j is computed and
 // immediately folded into the 64 layer-2 accumulators, so no register array is indexed by a
 // loop variable (the summation order per unit is unchanged: bias first, inputs ascending).
@@ -124,17 +135,52 @@ __global__ __launch_bounds__(256) void k_radius(const T *__restrict__ x, uint32_
       for (int j = 0; j < MLP_IN; j++) featOut[(size_t)q * MLP_IN + j] = f[j];
     if (radius) {
       const float y = mlp_forward(W, f, nullptr, nullptr);
-      const float r = wf * expf(y + margin);
+      // r = wf e^(y + margin), the exponent kept within [-8, 14] (about wf/3000 .. 1.2e6 wf)
+      // so that any weights give a finite radius the expansion loop can start from
+      float e = y + margin;
+      if (!(e == e)) e = 0.0f;
+      e = fminf(fmaxf(e, -8.0f), 14.0f);
+      const float r = wf * expf(e);
       radius[q] = (T)r;
-      if (rhist) {
-        int b = (int)floorf(4.0f * log2f(r / wf)) + 32;
-        b = b < 0 ? 0 : (b > 63 ? 63 : b);
-        atomicAdd(&lh[b], 1u);
-      }
+      if (rhist) atomicAdd(&lh[radius_bin<float>(r, wf)], 1u);
     }
   }
   __syncthreads();
   if (rhist && threadIdx.x < 64 && lh[threadIdx.x]) atomicAdd(&rhist[threadIdx.x], lh[threadIdx.x]);
+}
+
+// 64-bin histogram of the pending queries' radius bins
+template <typename T>
+__global__ __launch_bounds__(256) void k_radius_bins(const T *radius, const uint8_t *pending, uint32_t n,
+                                                     float wf, uint32_t *hist) {
+  __shared__ uint32_t lh[64];
+  if (threadIdx.x < 64) lh[threadIdx.x] = 0;
+  __syncthreads();
+  const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < n && pending[q]) atomicAdd(&lh[radius_bin<T>(radius[q], wf)], 1u);
+  __syncthreads();
+  if (threadIdx.x < 64 && lh[threadIdx.x]) atomicAdd(&hist[threadIdx.x], lh[threadIdx.x]);
+}
+
+// act[q] = pending with a radius bin in [lo, hi]
+template <typename T>
+__global__ void k_radius_class(const T *radius, const uint8_t *pending, uint32_t n, float wf, uint32_t lo,
+                               uint32_t hi, uint8_t *act) {
+  const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  bool a = false;
+  if (pending[q]) {
+    const uint32_t b = (uint32_t)radius_bin<T>(radius[q], wf);
+    a = b >= lo && b <= hi;
+  }
+  act[q] = a ? 1 : 0;
+}
+
+// radius *= factor for the queries of the last pass (act) that failed it
+template <typename T>
+__global__ void k_grow_class(T *radius, const uint8_t *act, const uint8_t *failed, uint32_t n, T factor) {
+  const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < n && act[q] && failed[q]) radius[q] = radius[q] * factor;
 }
 
 // number of non-zero counts (occupied feature cells) into *out
@@ -280,6 +326,21 @@ void radius_kernel(const T *x, uint32_t n, const typename GridOf<T>::type &g, co
         x, n, g, cnt, logk1, logdens, W, margin, wf, radius, featOut, rhist);
 }
 
+template <typename T>
+void radius_bins(const T *radius, const uint8_t *pending, uint32_t n, float wf, uint32_t *hist, hipStream_t s) {
+  if (n) k_radius_bins<T><<<(n + 255) / 256, 256, 0, s>>>(radius, pending, n, wf, hist);
+}
+template <typename T>
+void radius_class_flags(const T *radius, const uint8_t *pending, uint32_t n, float wf, uint32_t lo,
+                        uint32_t hi, uint8_t *act, hipStream_t s) {
+  if (n) k_radius_class<T><<<(n + 255) / 256, 256, 0, s>>>(radius, pending, n, wf, lo, hi, act);
+}
+template <typename T>
+void grow_class_radius(T *radius, const uint8_t *act, const uint8_t *failed, uint32_t n, T factor,
+                       hipStream_t s) {
+  if (n) k_grow_class<T><<<(n + 255) / 256, 256, 0, s>>>(radius, act, failed, n, factor);
+}
+
 void count_nonzero(const uint32_t *cnt, uint64_t cells, unsigned long long *out, hipStream_t s) {
   if (cells) k_count_nonzero<<<(unsigned)((cells + 255) / 256), 256, 0, s>>>(cnt, cells, out);
 }
@@ -327,7 +388,11 @@ void adam_step(float *W, const float *G, float *M, float *V, float lr, float b1,
   template void radius_kernel<T>(const T *, uint32_t, const GridOf<T>::type &, const uint32_t *,   \
                                  float, float, const float *, float, float, T *, float *,         \
                                  uint32_t *, hipStream_t);                                        \
-  template void grow_radius<T>(T *, const uint8_t *, uint32_t, T, hipStream_t);
+  template void grow_radius<T>(T *, const uint8_t *, uint32_t, T, hipStream_t);                   \
+  template void radius_bins<T>(const T *, const uint8_t *, uint32_t, float, uint32_t *, hipStream_t); \
+  template void radius_class_flags<T>(const T *, const uint8_t *, uint32_t, float, uint32_t, uint32_t, \
+                                      uint8_t *, hipStream_t);                                     \
+  template void grow_class_radius<T>(T *, const uint8_t *, const uint8_t *, uint32_t, T, hipStream_t);
 INST(float)
 INST(double)
 #undef INST

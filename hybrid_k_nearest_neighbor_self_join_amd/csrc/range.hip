@@ -204,7 +204,8 @@ template <typename G>
 struct TileCtx {
   G g;
   uint32_t k1;
-  int64_t rowAdj;  // row of query q starts at element (q * k1 + rowAdj)
+  uint32_t idBase;  // reported id = point index - idBase (the set's first point in a batch)
+  int64_t rowAdj;   // row of query q starts at element (q * k1 + rowAdj)
   __device__ __forceinline__ size_t row(uint32_t q) const { return (size_t)((int64_t)q * k1 + rowAdj); }
 };
 template <typename G>
@@ -212,12 +213,14 @@ __device__ __forceinline__ TileCtx<G> tile_ctx(const JoinArgs &a, const G &g, co
   TileCtx<G> c;
   c.g = g;
   c.k1 = a.k1;
+  c.idBase = 0;
   c.rowAdj = 0;
   if constexpr (std::is_same<G, GridParams>::value) {
     if (a.sets) {
       const SetDesc &sd = a.sets[a.setOf[a.act[tl.y]]];
       c.g = sd.g;
       c.k1 = sd.k1;
+      c.idBase = sd.pStart;
       c.rowAdj = (int64_t)sd.outOff - (int64_t)sd.pStart * sd.k1;
     }
   }
@@ -322,21 +325,30 @@ __global__ __launch_bounds__(64) void k_range_select(JoinArgs a, G g0) {
 
     // ---- scan: b* per query; counts become exclusive slot offsets (saturated past b*)
     const bool big = inF >= 65536u;
-    uint32_t cum = 0, bstar = NB;
+    uint32_t cum = 0, bstar = NB, cumB = 0;
     for (int w = 0; w < NB / 2; w++) {
       const uint32_t hw = hist[w * 64 + lane];
       const uint32_t h0 = hw & 0xffffu, h1 = hw >> 16;
       const uint32_t o0 = cum < 0xffffu ? cum : 0xffffu;
-      if (bstar == NB && cum + h0 >= K1) bstar = (uint32_t)(2 * w);
+      if (bstar == NB && cum + h0 >= K1) {
+        bstar = (uint32_t)(2 * w);
+        cumB = cum + h0;
+      }
       cum += h0;
       const uint32_t o1 = cum < 0xffffu ? cum : 0xffffu;
-      if (bstar == NB && cum + h1 >= K1) bstar = (uint32_t)(2 * w + 1);
+      if (bstar == NB && cum + h1 >= K1) {
+        bstar = (uint32_t)(2 * w + 1);
+        cumB = cum + h1;
+      }
       cum += h1;
       hist[w * 64 + lane] = o0 | (o1 << 16);
     }
-    const bool go = L.active && !big && bstar < (uint32_t)NB;
+    // buckets <= b* alone exceed the staging slots: the heap kernel takes the query, no
+    // collect pass
+    const bool preOvf = L.active && !big && bstar < (uint32_t)NB && cumB > SM;
+    const bool go = L.active && !big && bstar < (uint32_t)NB && !preOvf;
     const T edge = go ? ((T)(bstar + 1) / scale) * Num<T>::margin : (T)0;
-    bool okMine = false, ovfMine = false;
+    bool okMine = false, ovfMine = preOvf;
     const uint64_t goMask = __ballot(go);
     __syncthreads();
 
@@ -432,7 +444,7 @@ __global__ __launch_bounds__(64) void k_range_select(JoinArgs a, G g0) {
         for (int u = 0; u < MAXI; u++) {
           const uint32_t tI = lane + 64u * u;
           if (tI < m && rk[u] < K1) {
-            a.outIds[C.row(qid) + rk[u]] = (int32_t)sid[tI];
+            a.outIds[C.row(qid) + rk[u]] = (int32_t)(sid[tI] - C.idBase);
             od[C.row(qid) + rk[u]] = sqrt(ss[tI]);
           }
         }
@@ -602,7 +614,7 @@ __global__ __launch_bounds__(64) void k_range_heap(JoinArgs a, G g0) {
         T c[D];
 #pragma unroll
         for (int j = 0; j < D; j++) c[j] = x[(size_t)j * n + id];
-        a.outIds[C.row(L.q) + e] = (int32_t)id;
+        a.outIds[C.row(L.q) + e] = (int32_t)(id - C.idBase);
         od[C.row(L.q) + e] = sqrt(Num<T>::template sq<D>(c, L.qc));
       }
       a.failed[L.q] = 0;
