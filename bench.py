@@ -36,7 +36,8 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def run_gpu(pts: np.ndarray, k: int, beta: float, mode: int, steps: int, warmup: int):
+def run_gpu(pts: np.ndarray, k: int, beta: float, mode: int, steps: int, warmup: int,
+            nd: int = 0, keep_rows: bool = False):
     import torch
 
     from hybrid_k_nearest_neighbor_self_join_amd import _lib
@@ -48,7 +49,7 @@ def run_gpu(pts: np.ndarray, k: int, beta: float, mode: int, steps: int, warmup:
     ids = torch.empty((n, k + 1), dtype=torch.int32, device=dev)
     dists = torch.empty((n, k + 1), dtype=torch.float32, device=dev)
     stream = torch.cuda.current_stream(dev)
-    prm = _lib.default_params(beta=beta, mode=mode, device_ptrs=1,
+    prm = _lib.default_params(beta=beta, mode=mode, device_ptrs=1, num_indexed_dims=nd,
                               stream=ctypes.c_void_p(stream.cuda_stream).value)
     torch.cuda.synchronize()
 
@@ -80,6 +81,13 @@ def run_gpu(pts: np.ndarray, k: int, beta: float, mode: int, steps: int, warmup:
         "stats": last.as_dict(),
         "ids_row0": ids[0].cpu().numpy().tolist()[:4],
     }
+    if keep_rows:
+        import hashlib
+        h = hashlib.sha1(ids.cpu().numpy().tobytes())
+        h.update(dists.cpu().numpy().tobytes())
+        out["rows_digest"] = h.hexdigest()
+        ids_host = ids.cpu().numpy()
+        out["ids_sample"] = lambda q: ids_host[q]
     return out
 
 
@@ -116,6 +124,74 @@ def load_traffic(workload_key: str):
     return e
 
 
+def suite_highdim(args):
+    """5,000,000 points in 4-D and 6-D (uniform and skewed cubes), k=32, NUMINDEXEDDIM = d and
+    d-2: plain vs learned schedule, one JSON line per case."""
+    import torch  # noqa: F401  (before the library: one HIP runtime per process)
+
+    from hybrid_k_nearest_neighbor_self_join_amd import _lib, learned_radius, synthetic
+    from oracle import oracle
+
+    models = learned_radius.register_default_models()
+    n = args.n if args.n != 10_000_000 else 5_000_000
+    for d, nd, gen in [(4, 4, "uniform"), (4, 2, "uniform"), (4, 4, "skewed"), (4, 2, "skewed"),
+                       (6, 6, "uniform"), (6, 4, "uniform"), (6, 6, "skewed"), (6, 4, "skewed")]:
+        pts = (synthetic.unit_cube if gen == "uniform" else synthetic.skewed_cube)(n, d, seed=0x5EED0100 + d * 10 + nd)
+        out = {"suite": "highdim", "n": n, "d": d, "nd": nd, "data": gen, "k": 32}
+        res = {}
+        for mode in (["plain", "learned"] if learned_radius.has_model(models, d, nd) else ["plain"]):
+            m = _lib.KNN_MODE_LEARNED if mode == "learned" else _lib.KNN_MODE_BETA
+            g = run_gpu(pts, 32, 0.025, m, args.steps, args.warmup, nd=nd, keep_rows=True)
+            res[mode] = g
+            out[mode] = {"queries_per_s": n / (g["ms_per_step"] / 1e3), "ms_per_step": g["ms_per_step"],
+                         "iterations": g["stats"]["iterations"], "candidates": g["stats"]["candidates"],
+                         "checksum": g["stats"]["checksum"]}
+        if "learned" in res:
+            out["speedup_learned_vs_plain"] = res["plain"]["ms_per_step"] / res["learned"]["ms_per_step"]
+            out["identical_rows"] = bool(res["plain"]["rows_digest"] == res["learned"]["rows_digest"])
+        q = np.random.default_rng(d * nd).choice(n, 500, replace=False)
+        perm, _ = oracle.reorder(pts)
+        bi, _ = oracle.bruteforce(np.ascontiguousarray(pts[:, perm]), 32, queries=q)
+        out["bruteforce_sample_match"] = bool(np.array_equal(res["plain"]["ids_sample"](q), bi))
+        print(json.dumps(out), flush=True)
+
+
+def suite_batched(args):
+    """256 sets, sizes log-uniform in [1,000, 200,000], d in {2,3,4,6}, k in {8,32}: batched
+    join (hipGraph replay, autotuned) vs the same sets joined one by one."""
+    import torch  # noqa: F401
+
+    from hybrid_k_nearest_neighbor_self_join_amd import _lib, batched, synthetic
+
+    sets = synthetic.batched_sets(256)
+    pts, ks = [s for s, _ in sets], [k for _, k in sets]
+    N = sum(len(p) for p in pts)
+    batched.knnjoin_batched(pts, ks)  # build + capture
+    tq, gb, table = batched.autotune(pts, ks, cache=None)
+    times = []
+    for _ in range(max(args.steps, 3)):
+        ids, dists, st = batched.knnjoin_batched(pts, ks)
+        times.append(st.ms_device)
+    lat = []
+    mism = 0
+    for i, (p_, k) in enumerate(sets):
+        ri, rd, s1 = _lib.knnjoin_ex(p_, k, beta=0.025)
+        lat.append(s1.ms_total)
+        mism += int(not (np.array_equal(ri, ids[i]) and np.array_equal(rd.view(np.uint32), dists[i].view(np.uint32))))
+    lat = np.array(lat)
+    print(json.dumps({
+        "suite": "batched", "sets": len(sets), "points": N,
+        "batched": {"ms_device_median": float(np.median(times)), "queries_per_s": N / (np.median(times) / 1e3),
+                    "tile_queries": tq, "grid_blocks": gb, "graph_replayed": st.graph_replayed,
+                    "autotune_ms": table},
+        "one_by_one": {"ms_total": float(lat.sum()), "queries_per_s": N / (lat.sum() / 1e3),
+                       "per_set_ms_p50": float(np.percentile(lat, 50)),
+                       "per_set_ms_p99": float(np.percentile(lat, 99))},
+        "speedup_batched_vs_one_by_one": float(lat.sum() / np.median(times)),
+        "rows_differing_from_single_set_joins": mism,
+    }), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -129,6 +205,8 @@ def main():
                          "ships with the package, the learned schedule; value = the faster")
     ap.add_argument("--cpu-sample", type=int, default=1_000_000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--suite", choices=["highdim", "batched"], default=None,
+                    help="instead of the headline line: the 4-D/6-D cases or the 256-set batch")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 5 device copies of 1 GiB (the reference kernel for scaling "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE readings to bytes)")
@@ -145,10 +223,18 @@ def main():
                           "copies": 5}), flush=True)
         return
 
+    if args.suite == "highdim":
+        return suite_highdim(args)
+    if args.suite == "batched":
+        return suite_batched(args)
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (this build has no multi-GPU path); "
               f"--gpus {args.gpus} is not supported", file=sys.stderr)
         sys.exit(2)
+
+    # torch ships its own HIP runtime under the same soname as ROCm's: it must be loaded before
+    # the library, or the process ends up with two runtimes
+    import torch  # noqa: F401
 
     from hybrid_k_nearest_neighbor_self_join_amd import _lib, learned_radius, synthetic
 
@@ -159,12 +245,12 @@ def main():
     if args.mode == "plain":
         modes = ["plain"]
     elif args.mode == "learned":
-        if 2 not in models:
+        if not learned_radius.has_model(models, 2, 2):
             print("no trained 2-D radius model in the package", file=sys.stderr)
             sys.exit(2)
         modes = ["learned"]
     else:
-        modes = ["plain"] + (["learned"] if 2 in models else [])
+        modes = ["plain"] + (["learned"] if learned_radius.has_model(models, 2, 2) else [])
     res = {}
     for m in modes:
         mode = _lib.KNN_MODE_LEARNED if m == "learned" else _lib.KNN_MODE_BETA

@@ -31,7 +31,8 @@ EXPORTED_SYMBOLS = [
     "KNNJoinPy", "KNNJoinPyDouble", "knnjoin_ex", "knnjoin_default_params",
     "knnjoin_last_status", "knnjoin_last_error", "knnjoin_release", "knnjoin_stage_reorder",
     "knnjoin_stage_estimate_eps", "knnjoin_stage_index", "knnjoin_stage_join_pass",
-    "knnjoin_set_radius_model", "knnjoin_radius_features", "knnjoin_mlp_forward",
+    "knnjoin_set_radius_model", "knnjoin_set_radius_model_dims", "knnjoin_radius_features",
+    "knnjoin_mlp_forward",
     "knnjoin_mlp_grad", "knnjoin_mlp_train", "knnjoin_batch_default_params", "knnjoin_batched",
     "knnjoin_batch_autotune",
 ]
@@ -135,6 +136,8 @@ def _setup(L):
     L.knnjoin_stage_join_pass.restype = i32
     L.knnjoin_set_radius_model.argtypes = [u32, vp, u32, flt]
     L.knnjoin_set_radius_model.restype = i32
+    L.knnjoin_set_radius_model_dims.argtypes = [u32, u32, vp, u32, flt]
+    L.knnjoin_set_radius_model_dims.restype = i32
     L.knnjoin_radius_features.argtypes = [vp, u32, u32, u32, u32, i32, vp, vp, vp]
     L.knnjoin_radius_features.restype = i32
     L.knnjoin_mlp_forward.argtypes = [vp, vp, u32, vp]
@@ -239,9 +242,11 @@ def stage_join_pass(points: np.ndarray, nd: int, k: int, eps: float, queries: np
 
 
 # ------------------------------------------------------------------ learned radius
-def set_radius_model(nd: int, weights: np.ndarray, margin: float):
+def set_radius_model(nd: int, weights: np.ndarray, margin: float, ndim: int = 0):
+    """Registers a radius model for `nd` indexed dims: for any NDIM (ndim 0) or for NDIM =
+    ndim only (preferred over the any-NDIM model)."""
     w = np.ascontiguousarray(weights, np.float32).reshape(-1)
-    check(lib().knnjoin_set_radius_model(nd, p(w), len(w), margin))
+    check(lib().knnjoin_set_radius_model_dims(ndim, nd, p(w), len(w), margin))
 
 
 def radius_features(points: np.ndarray, k: int, nd: int | None = None, reorder: int = 1,

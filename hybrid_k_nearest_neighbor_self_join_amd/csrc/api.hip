@@ -154,7 +154,12 @@ struct RadiusModel {
   float margin = 0.0f;
   bool ok = false;
 };
-static RadiusModel g_models[KNN_MAX_INDEXED + 1];  // by number of indexed dims
+// by (NDIM, indexed dims); NDIM 0 = a model for any NDIM with that many indexed dims
+static RadiusModel g_models[KNN_MAXDIM + 1][KNN_MAX_INDEXED + 1];
+static const RadiusModel &radius_model(uint32_t d, uint32_t nd) {
+  const RadiusModel &m = g_models[d <= KNN_MAXDIM ? d : 0][nd];
+  return m.ok ? m : g_models[0][nd];
+}
 
 // Feature grid of the learned radius: width wf = (V (K+1) / N)^(1/nd), V = product of the
 // indexed extents (an extent of 0 counts as 1), doubled until the grid has <= 2^27 cells.
@@ -810,7 +815,7 @@ static void run_learned(Pipeline<T> &P, const knnjoin_params &p, knnjoin_stats &
   (void)p;
   hipStream_t s = P.s;
   const uint32_t n = P.n, nd = P.nd, K1 = P.K1;
-  const RadiusModel &M = g_models[nd];
+  const RadiusModel &M = radius_model(P.d, nd);
   if (!M.ok) throw Error(KNN_ERR_MODEL, "no learned-radius model registered for this number of indexed dims");
   // min/max of the indexed dims
   T *mm = P.template buf<T>(S_PART, 2 * KNN_MAX_INDEXED + minmax_scratch_elems(n, nd));
@@ -837,9 +842,10 @@ static void run_learned(Pipeline<T> &P, const knnjoin_params &p, knnjoin_stats &
   P.radius = radius;
 
   // ---- radius classes: the pending queries whose radius lies in the lowest occupied octave
-  // [R/2, R) are joined on a grid of cell width R/2 (a query's block then spans at most 5
-  // cells per indexed dim and about 2.4x the area of its ball); queries that fail double
-  // their radius, which moves them to a later octave.  One index build per occupied octave.
+  // [R/2, R) are joined on a grid matched to R (cell width R/2 for <= 2 indexed dims: a
+  // query's block then spans at most 5 cells per dim and about 2.4x the area of its ball;
+  // R above); queries that fail double their radius, which moves them to a later octave.
+  // One index build per occupied octave.
   unsigned long long *ctr = P.template buf<unsigned long long>(S_CTR, 64);
   fill_u32(reinterpret_cast<uint32_t *>(ctr), 0u, 16, s);
   uint64_t *maxr = reinterpret_cast<uint64_t *>(ctr + 8);
@@ -888,7 +894,9 @@ static void run_learned(Pipeline<T> &P, const knnjoin_params &p, knnjoin_stats &
       }
       R = R * (1.0 + 1e-6);
     }
-    T w = (T)(R / 2.0);
+    // cell width R/2 (blocks of <= 5 cells per dim) with <= 2 indexed dims; above, R (<= 3 cells
+    // per dim), since every extra cell of reach multiplies the rows of a block
+    T w = (T)(nd <= 2 ? R / 2.0 : R);
     if (!(w > (T)0)) w = F.wf;
     for (;;) {  // keep the grid within 2^30 cells
       typename GridOf<T>::type gt;
@@ -1210,16 +1218,23 @@ int knnjoin_stage_join_pass(const float *points, uint32_t n, uint32_t ndim, uint
 // ------------------------------------------------------------------ learned-radius entries
 extern "C" {
 
-int knnjoin_set_radius_model(uint32_t nd, const float *weights, uint32_t nweights, float margin) {
+int knnjoin_set_radius_model_dims(uint32_t ndim, uint32_t nd, const float *weights, uint32_t nweights,
+                                  float margin) {
   return knn::guarded([&]() -> int {
     using namespace knn;
     if (nd == 0 || nd > KNN_MAX_INDEXED) throw Error(KNN_ERR_ARG, "nd must be in 1..8");
+    if (ndim > KNN_MAXDIM || (ndim != 0 && ndim < nd)) throw Error(KNN_ERR_ARG, "ndim must be 0 or in nd..16");
     if (!weights || nweights != (uint32_t)MLP_NW) throw Error(KNN_ERR_MODEL, "radius model must have 5057 weights");
-    g_models[nd].w.assign(weights, weights + MLP_NW);
-    g_models[nd].margin = margin;
-    g_models[nd].ok = true;
+    RadiusModel &m = g_models[ndim][nd];
+    m.w.assign(weights, weights + MLP_NW);
+    m.margin = margin;
+    m.ok = true;
     return KNN_OK;
   });
+}
+
+int knnjoin_set_radius_model(uint32_t nd, const float *weights, uint32_t nweights, float margin) {
+  return knnjoin_set_radius_model_dims(0, nd, weights, nweights, margin);
 }
 
 int knnjoin_radius_features(const float *points, uint32_t n, uint32_t ndim, uint32_t k, uint32_t nd,
@@ -1265,7 +1280,7 @@ int knnjoin_radius_features(const float *points, uint32_t n, uint32_t ndim, uint
     float *fd = features ? P.buf<float>(S_TRAIN, (size_t)n * MLP_IN) : nullptr;
     float *rd = nullptr;
     float *Wd = nullptr;
-    const RadiusModel &M = g_models[nd];
+    const RadiusModel &M = radius_model(ndim, nd);
     if (radius_out) {
       if (!M.ok) throw Error(KNN_ERR_MODEL, "no radius model registered for nd");
       rd = P.buf<float>(S_RADIUS, n);

@@ -159,26 +159,34 @@ __device__ __forceinline__ void tile_setup(const JoinArgs &a, const G &g, const 
 }
 
 // Calls f(pBeg, pEnd) for each row of the block [lo, hi] holding points (the arguments and the
-// calls are uniform across the wave when lo/hi are).
-template <typename G, typename F>
-__device__ __forceinline__ void for_each_range(const JoinArgs &a, const G &g, const uint32_t *lo,
-                                               const uint32_t *hi, F &&f) {
+// calls are uniform across the wave when lo/hi are).  narrow(cur, l0, h0) may shrink a row's
+// dim-0 range [l0, h0] (initially [lo[0], hi[0]]) or drop the row (returns false); it must
+// return wave-uniform results.
+template <typename G, typename R, typename F>
+__device__ __forceinline__ void for_each_row(const JoinArgs &a, const G &g, const uint32_t *lo,
+                                             const uint32_t *hi, R &&narrow, F &&f) {
   uint32_t cur[KNN_MAX_INDEXED];
   for (uint32_t j = 0; j < g.nd; j++) cur[j] = lo[j];
   for (;;) {
-    uint64_t rowBase = g.keyBase;
-    for (uint32_t j = 1; j < g.nd; j++) rowBase += (uint64_t)cur[j] * g.mult[j];
-    uint32_t pBeg, pEnd;
-    if (a.denseStart) {
-      pBeg = a.denseStart[rowBase + lo[0]];
-      pEnd = a.denseStart[rowBase + hi[0] + 1];
-    } else {
-      const uint32_t cbeg = lower_bound_u64(a.cellKeys, a.ncell, rowBase + lo[0]);
-      const uint32_t cend = upper_bound_u64(a.cellKeys, a.ncell, rowBase + hi[0]);
-      pBeg = a.cellStart[cbeg];
-      pEnd = a.cellStart[cend];
+    uint32_t l0 = lo[0], h0 = hi[0];
+    if (narrow(cur, l0, h0)) {
+      uint64_t rowBase = g.keyBase;
+      for (uint32_t j = 1; j < g.nd; j++) rowBase += (uint64_t)cur[j] * g.mult[j];
+      uint32_t pBeg, pEnd;
+      if (a.denseStart) {
+        pBeg = a.denseStart[rowBase + l0];
+        pEnd = a.denseStart[rowBase + h0 + 1];
+      } else {
+        const uint32_t cbeg = lower_bound_u64(a.cellKeys, a.ncell, rowBase + l0);
+        const uint32_t cend = upper_bound_u64(a.cellKeys, a.ncell, rowBase + h0);
+        pBeg = a.cellStart[cbeg];
+        pEnd = a.cellStart[cend];
+      }
+      // the range is wave-uniform (so are lo/hi): scalar registers, scalar loop control
+      pBeg = (uint32_t)__builtin_amdgcn_readfirstlane((int)pBeg);
+      pEnd = (uint32_t)__builtin_amdgcn_readfirstlane((int)pEnd);
+      if (pEnd > pBeg) f(pBeg, pEnd);
     }
-    if (pEnd > pBeg) f(pBeg, pEnd);
     uint32_t j = 1;
     while (j < g.nd) {
       if (cur[j] < hi[j]) {
@@ -190,6 +198,55 @@ __device__ __forceinline__ void for_each_range(const JoinArgs &a, const G &g, co
     }
     if (j >= g.nd) break;
   }
+}
+
+template <typename G, typename F>
+__device__ __forceinline__ void for_each_range(const JoinArgs &a, const G &g, const uint32_t *lo,
+                                               const uint32_t *hi, F &&f) {
+  for_each_row(a, g, lo, hi, [](const uint32_t *, uint32_t &, uint32_t &) { return true; }, f);
+}
+
+template <int D>
+__device__ __forceinline__ double pick(const double *v, uint32_t j) {
+  double r = v[0];
+#pragma unroll
+  for (int jj = 1; jj < D; jj++)
+    if ((uint32_t)jj == j) r = v[jj];
+  return r;
+}
+
+// Chord of a ball in one row of a block (used with 3 or more indexed dims, where it drops the
+// many rows of a block's corners; with 2, the per-row work costs more than the cells saved).  A point of the row's cells lies at least `gap` from
+// the query in dims 1..nd-1 (gap = distance to the row's cell slab, each slab widened by
+// 2^-18 of its coordinates for the rounding of cell_coord), so a point within sqrt(r2) of the
+// query has |x0 - q0| <= sqrt(r2 - gap^2): only the dim-0 cells of that chord can hold one,
+// and no cell of the row when gap^2 > r2.  Per lane (inactive lanes contribute nothing); the
+// row range becomes the wave's union, clipped to [l0, h0].
+template <typename T, int D, typename G>
+__device__ __forceinline__ bool row_chord(const G &g, const uint32_t *cur, const double *qd, double r2,
+                                          bool active, uint32_t &l0, uint32_t &h0) {
+  double g2 = 0.0;
+  const double e = (double)g.eps;
+  for (uint32_t j = 1; j < g.nd; j++) {
+    const double mn = (double)g.minArr[j];
+    const double clo = mn + (double)cur[j] * e, chi = clo + e;
+    const double pad = 0x1p-18 * (fabs(mn) + fabs(chi) + e);
+    const double q = pick<D>(qd, j);
+    const double gap = fmax(0.0, fmax((clo - pad) - q, q - (chi + pad)));
+    g2 += gap * gap;
+  }
+  uint32_t l = 0xffffffffu, u = 0u;
+  if (active && g2 <= r2) {
+    const double h = sqrt(r2 - g2) * (1.0 + 0x1p-20);
+    const T xa = next_down((T)(qd[0] - h)), xb = next_up((T)(qd[0] + h));
+    l = cell_coord<T>(xa, (T)g.minArr[0], (T)g.eps, g.nCells[0]);
+    u = cell_coord<T>(xb, (T)g.minArr[0], (T)g.eps, g.nCells[0]);
+  }
+  l = wave_min_u32(l);
+  u = wave_max_u32(u);
+  l0 = l > l0 ? l : l0;
+  h0 = u < h0 ? u : h0;
+  return l0 <= h0;
 }
 
 __device__ __forceinline__ uint32_t next_tile(uint32_t *counter, uint32_t lane) {
@@ -258,8 +315,9 @@ namespace {
 
 template <typename T>
 __device__ __forceinline__ uint32_t bucket_of(T s, T scale) {
-  const uint32_t b = (uint32_t)(s * scale);
-  return b < (uint32_t)NB ? b : (uint32_t)NB - 1;
+  // s * scale >= 0; clamped below NB before the conversion (NB - 0.5 truncates to NB - 1)
+  const T t = s * scale;
+  return (uint32_t)(t < (T)NB - (T)0.5 ? t : (T)NB - (T)0.5);
 }
 
 // LDS: hist[NB/2][64] u32, two 16-bit bucket counters per word (per-lane histogram
@@ -301,24 +359,31 @@ __global__ __launch_bounds__(64) void k_range_select(JoinArgs a, G g0) {
     uint32_t inF = 0;
 
     // ---- pass 1 (lane = query): histogram of s over [0, F] for all queries of the tile
-    for_each_range(a, g, lo, hi, [&](uint32_t pBeg, uint32_t pEnd) {
+    // (rows narrowed to the chords of radius sqrt(F (1 + 2^-15)), which holds every point
+    // with s <= F)
+    const double rF2 = (double)L.F * (1.0 + 0x1p-15);
+    auto narrow1 = [&](const uint32_t *cur, uint32_t &l0, uint32_t &h0) {
+      return g.nd < 3 ? true : row_chord<T, D, G>(g, cur, L.qd, rF2, L.active, l0, h0);
+    };
+    for_each_row(a, g, lo, hi, narrow1, [&](uint32_t pBeg, uint32_t pEnd) {
       tileCand += pEnd - pBeg;
       for (uint32_t base = pBeg; base < pEnd; base += 64) {
         const uint32_t pp = base + lane < pEnd ? base + lane : pBeg;
         T cv[D];
 #pragma unroll
         for (int j = 0; j < D; j++) cv[j] = xs[(size_t)j * n + pp];
-        const uint32_t m = (pEnd - base) < 64u ? (pEnd - base) : 64u;
+        const uint32_t m = (uint32_t)__builtin_amdgcn_readfirstlane((int)((pEnd - base) < 64u ? (pEnd - base) : 64u));
         for (uint32_t jj = 0; jj < m; jj++) {
           T c[D];
 #pragma unroll
           for (int j = 0; j < D; j++) c[j] = readlane<T>(cv[j], (int)jj);
           const T s = Num<T>::template sq<D>(c, L.qc);
-          if (L.active && s <= L.F) {
-            const uint32_t b = bucket_of<T>(s, scale);
-            atomicAdd(&hist[hword(b, lane)], 1u << hshift(b));
-            inF++;
-          }
+          // branch-free: lanes out of range add 0 (nearly every candidate is in range for
+          // some lane of the wave, so the LDS add would be issued anyway)
+          const bool in = L.active && s <= L.F;
+          const uint32_t b = bucket_of<T>(s, scale);
+          atomicAdd(&hist[hword(b, lane)], in ? (1u << hshift(b)) : 0u);
+          inF += in ? 1u : 0u;
         }
       }
     });
@@ -382,7 +447,11 @@ __global__ __launch_bounds__(64) void k_range_select(JoinArgs a, G g0) {
         shi[j] = cell_coord<T>(next_up(xj + ri), (T)g.minArr[j], eps, g.nCells[j]);
       }
       bool ovf = false;
-      for_each_range(a, g, slo, shi, [&](uint32_t pBeg, uint32_t pEnd) {
+      const double ri2 = (double)ri * (double)ri * (1.0 + 0x1p-15);
+      auto narrow2 = [&](const uint32_t *cur, uint32_t &l0, uint32_t &h0) {
+        return g.nd < 3 ? true : row_chord<T, D, G>(g, cur, qdi, ri2, true, l0, h0);
+      };
+      for_each_row(a, g, slo, shi, narrow2, [&](uint32_t pBeg, uint32_t pEnd) {
         for (uint32_t base = pBeg; base < pEnd; base += 64) {
           const uint32_t p = base + lane;
           if (p < pEnd) {
@@ -568,7 +637,11 @@ __global__ __launch_bounds__(64) void k_range_heap(JoinArgs a, G g0) {
     T filt = L.F;
     uint64_t tileCand = 0;
 
-    for_each_range(a, g, lo, hi, [&](uint32_t pBeg, uint32_t pEnd) {
+    const double rF2 = (double)L.F * (1.0 + 0x1p-15);
+    auto narrowH = [&](const uint32_t *cur, uint32_t &l0, uint32_t &h0) {
+      return g.nd < 3 ? true : row_chord<T, D, G>(g, cur, L.qd, rF2, L.active, l0, h0);
+    };
+    for_each_row(a, g, lo, hi, narrowH, [&](uint32_t pBeg, uint32_t pEnd) {
       tileCand += pEnd - pBeg;
       for (uint32_t base = pBeg; base < pEnd; base += 64) {
         const uint32_t pp = base + lane < pEnd ? base + lane : pBeg;

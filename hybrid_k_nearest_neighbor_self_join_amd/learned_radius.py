@@ -6,13 +6,18 @@ from exact joins computed by this library (plain schedule) on seeded synthetic s
 and training run on the GPU through the C ABI (knnjoin_radius_features, knnjoin_mlp_train).
 The model only sets the radius a query is first tried at; rows stay exact whatever it predicts.
 
-    python -m hybrid_k_nearest_neighbor_self_join_amd.learned_radius train [--nd 2] [--out DIR]
+    python -m hybrid_k_nearest_neighbor_self_join_amd.learned_radius train [--d 2] [--nd 2] [--out DIR]
+
+A model is specific to the number of indexed dims `nd` and, for data with more dims than are
+indexed, to NDIM `d`: files radius_mlp_nd{nd}.npz (2-D data, registered for any NDIM) and
+radius_mlp_d{d}_nd{nd}.npz (registered for NDIM = d only).
 """
 from __future__ import annotations
 
 import argparse
 import json
 import os
+import re
 import time
 
 import numpy as np
@@ -24,21 +29,26 @@ WEIGHTS_DIR = os.path.join(PKG, "weights")
 MLP_NW = _lib.MLP_NW
 
 
-def weights_path(nd: int, directory: str = WEIGHTS_DIR) -> str:
-    return os.path.join(directory, f"radius_mlp_nd{nd}.npz")
+def _stem(nd: int, d: int) -> str:
+    return f"radius_mlp_nd{nd}" if d == nd == 2 else f"radius_mlp_d{d}_nd{nd}"
 
 
-def raw_path(nd: int, directory: str = WEIGHTS_DIR) -> str:
+def weights_path(nd: int, directory: str = WEIGHTS_DIR, d: int | None = None) -> str:
+    return os.path.join(directory, _stem(nd, nd if d is None else d) + ".npz")
+
+
+def raw_path(nd: int, directory: str = WEIGHTS_DIR, d: int | None = None) -> str:
     """Model file of the C++ command line (`knnjoin_main --learned FILE`): the 5057 weights
     then the margin, little-endian float32."""
-    return os.path.join(directory, f"radius_mlp_nd{nd}.bin")
+    return os.path.join(directory, _stem(nd, nd if d is None else d) + ".bin")
 
 
-def save(nd: int, w: np.ndarray, margin: float, info: dict, directory: str = WEIGHTS_DIR):
+def save(nd: int, w: np.ndarray, margin: float, info: dict, directory: str = WEIGHTS_DIR,
+         d: int | None = None):
     os.makedirs(directory, exist_ok=True)
-    np.savez(weights_path(nd, directory), weights=w.astype(np.float32), margin=np.float32(margin),
+    np.savez(weights_path(nd, directory, d), weights=w.astype(np.float32), margin=np.float32(margin),
              info=np.array(json.dumps(info)))
-    with open(raw_path(nd, directory), "wb") as f:
+    with open(raw_path(nd, directory, d), "wb") as f:
         f.write(np.concatenate([w.astype("<f4"), np.array([margin], "<f4")]).tobytes())
 
 
@@ -66,23 +76,25 @@ def training_pairs(points: np.ndarray, k: int, nd: int, max_queries: int, seed: 
     return feats[sel], y
 
 
-def default_sets(nd: int, n: int, seed: int):
-    """Seeded training point sets for `nd` indexed dims (uniform and clustered mixes)."""
+def default_sets(nd: int, n: int, seed: int, d: int | None = None):
+    """Seeded training point sets of `d` dims (default nd): for 2-D, the uniform box and
+    three uniform/clustered mixes; above, the uniform and the skewed (exponential) cube."""
+    d = nd if d is None else d
     out = []
-    if nd <= 2:
+    if d == 2:
         out.append(synthetic.uniform_box(n, seed=seed + 1))
         out.append(synthetic.mixed_box(n, seed=seed + 2))
         out.append(synthetic.mixed_box(n, seed=seed + 3, frac_uniform=0.5, nclusters=16, sigma=1.0))
         out.append(synthetic.mixed_box(n, seed=seed + 4, frac_uniform=0.9, nclusters=128, sigma=0.25))
     else:
-        out.append(synthetic.unit_cube(n, nd, seed=seed + 1))
-        out.append(synthetic.skewed_cube(n, nd, seed=seed + 2))
+        out.append(synthetic.unit_cube(n, d, seed=seed + 1))
+        out.append(synthetic.skewed_cube(n, d, seed=seed + 2))
     return out
 
 
-def build_dataset(nd: int, n: int, ks, per_set: int, seed: int, log=print):
+def build_dataset(nd: int, n: int, ks, per_set: int, seed: int, log=print, d: int | None = None):
     X, Y = [], []
-    for si, pts in enumerate(default_sets(nd, n, seed)):
+    for si, pts in enumerate(default_sets(nd, n, seed, d)):
         for k in ks:
             t0 = time.time()
             f, y = training_pairs(pts, k, nd, per_set, seed + 100 * si + k)
@@ -99,8 +111,9 @@ def calibrate_margin(w: np.ndarray, X: np.ndarray, Y: np.ndarray, quantile: floa
 
 def train(nd: int = 2, n: int = 2_000_000, ks=(8, 16, 32, 64, 128), per_set: int = 200_000,
           epochs: int = 30, batch: int = 4096, lr: float = 2e-3, seed: int = 20261202,
-          quantile: float = 0.97, out_dir: str = WEIGHTS_DIR, log=print) -> dict:
-    X, Y = build_dataset(nd, n, ks, per_set, seed, log)
+          quantile: float = 0.97, out_dir: str = WEIGHTS_DIR, log=print, d: int | None = None) -> dict:
+    d = nd if d is None else d
+    X, Y = build_dataset(nd, n, ks, per_set, seed, log, d)
     perm = np.random.default_rng(seed).permutation(len(Y))
     nval = max(1, len(Y) // 10)
     val, tr = perm[:nval], perm[nval:]
@@ -113,19 +126,19 @@ def train(nd: int = 2, n: int = 2_000_000, ks=(8, 16, 32, 64, 128), per_set: int
     base_val = float(np.mean((Y[val] - Y[tr].mean()) ** 2))
     margin = calibrate_margin(w, X[tr], Y[tr], quantile)
     cover = float(np.mean(Y[val] - pred_val <= margin))
-    info = {"nd": nd, "pairs": int(len(Y)), "epochs": epochs, "batch": batch, "lr": lr,
+    info = {"d": d, "nd": nd, "pairs": int(len(Y)), "epochs": epochs, "batch": batch, "lr": lr,
             "seed": seed, "quantile": quantile, "margin": margin, "val_mse": mse_val,
             "val_mse_constant_predictor": base_val, "val_coverage": cover,
             "train_seconds": t_train, "loss_per_epoch": [float(v) for v in losses],
-            "training_sets": f"{len(default_sets(nd, 16, seed))} seeded sets of {n} points, k in {list(ks)}"}
-    save(nd, w, margin, info, out_dir)
-    log(f"[train] nd={nd}: val mse {mse_val:.4f} (constant predictor {base_val:.4f}), "
+            "training_sets": f"{len(default_sets(nd, 16, seed, d))} seeded {d}-D sets of {n} points, k in {list(ks)}"}
+    save(nd, w, margin, info, out_dir, d)
+    log(f"[train] d={d} nd={nd}: val mse {mse_val:.4f} (constant predictor {base_val:.4f}), "
         f"margin {margin:.3f} covers {cover:.3f} of validation queries; {t_train:.1f} s")
     return info
 
 
-def load(nd: int, directory: str = WEIGHTS_DIR):
-    path = weights_path(nd, directory)
+def load(nd: int, directory: str = WEIGHTS_DIR, d: int | None = None):
+    path = weights_path(nd, directory, d)
     if not os.path.exists(path):
         return None
     z = np.load(path, allow_pickle=False)
@@ -133,29 +146,42 @@ def load(nd: int, directory: str = WEIGHTS_DIR):
 
 
 def register_default_models(directory: str = WEIGHTS_DIR) -> list:
-    """Registers every weights file found for nd = 1..8; returns the nd values loaded."""
+    """Registers every weights file in `directory`; returns the (d, nd) pairs loaded (d = 0:
+    the model applies to any NDIM with nd indexed dims)."""
     done = []
-    for nd in range(1, 9):
-        m = load(nd, directory)
-        if m is not None:
-            _lib.set_radius_model(nd, m[0], m[1])
-            done.append(nd)
+    if not os.path.isdir(directory):
+        return done
+    for name in sorted(os.listdir(directory)):
+        m = re.fullmatch(r"radius_mlp_(?:d(\d+)_)?nd(\d+)\.npz", name)
+        if not m:
+            continue
+        nd = int(m.group(2))
+        d = int(m.group(1)) if m.group(1) else 0
+        w = load(nd, directory, d if d else None)
+        if w is None:
+            continue
+        _lib.set_radius_model(nd, w[0], w[1], ndim=d)
+        done.append((d, nd))
     return done
+
+
+def has_model(done: list, d: int, nd: int) -> bool:
+    return (d, nd) in done or (0, nd) in done
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("cmd", choices=["train"])
-    ap.add_argument("--nd", type=int, nargs="+", default=[2])
+    ap.add_argument("--nd", type=int, default=2)
+    ap.add_argument("--d", type=int, default=None, help="NDIM of the training data (default nd)")
     ap.add_argument("--n", type=int, default=2_000_000)
     ap.add_argument("--epochs", type=int, default=30)
     ap.add_argument("--per-set", type=int, default=200_000)
     ap.add_argument("--out", default=WEIGHTS_DIR)
     args = ap.parse_args()
-    for nd in args.nd:
-        ks = (8, 16, 32, 64, 128) if nd <= 2 else (16, 32, 64)
-        n = args.n if nd <= 2 else min(args.n, 1_000_000)
-        train(nd=nd, n=n, ks=ks, per_set=args.per_set, epochs=args.epochs, out_dir=args.out)
+    d = args.nd if args.d is None else args.d
+    ks = (8, 16, 32, 64, 128) if d == 2 else (16, 32, 64)
+    train(nd=args.nd, n=args.n, ks=ks, per_set=args.per_set, epochs=args.epochs, out_dir=args.out, d=d)
 
 
 if __name__ == "__main__":

@@ -151,8 +151,13 @@ int knnjoin_stage_join_pass(const float *points, uint32_t n, uint32_t ndim, uint
  * w3[64], b3 (5057 floats).  Radius = wf * exp(mlp(features) + margin).
  * ------------------------------------------------------------------------------------- */
 
-/* Registers the model used for data with `nd` indexed dims. */
+/* Registers the model used for data with `nd` indexed dims (any NDIM). */
 int knnjoin_set_radius_model(uint32_t nd, const float *weights, uint32_t nweights, float margin);
+
+/* Registers the model used for NDIM-dimensional data with `nd` indexed dims; it takes
+ * precedence over a model registered for `nd` alone (ndim 0 = that any-NDIM model). */
+int knnjoin_set_radius_model_dims(uint32_t ndim, uint32_t nd, const float *weights, uint32_t nweights,
+                                  float margin);
 
 /* The 12 features of every point (n x 12, row-major) as the learned schedule computes them
  * (after the dimension reorder when `reorder`), the feature-grid width wf, and (when a model

@@ -56,7 +56,9 @@ def test_batched_bruteforce_sample(oracle):
     ids, dists, st = batched.knnjoin_batched([s for s, _ in sets], [k for _, k in sets])
     for i, (pts, k) in enumerate(sets):
         q = np.random.default_rng(i).choice(len(pts), 200, replace=False)
-        bi, bd = oracle.bruteforce(pts, k, queries=q)
+        perm, _ = oracle.reorder(pts)
+        # distances are summed in the join's (variance) dimension order
+        bi, bd = oracle.bruteforce(np.ascontiguousarray(pts[:, perm]), k, queries=q)
         assert np.array_equal(ids[i][q], bi)
         assert np.array_equal(bits(dists[i][q]), bits(bd))
 

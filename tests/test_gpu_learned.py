@@ -94,7 +94,7 @@ def test_missing_model_is_an_error():
 
 def test_trained_model_gains_over_plain_schedule(oracle):
     loaded = learned_radius.register_default_models()
-    if 2 not in loaded:
+    if not learned_radius.has_model(loaded, 2, 2):
         pytest.skip("no trained 2-D model file in the package")
     pts = synthetic.mixed_box(2_000_000, seed=0xFEED)
     ids_p, d_p, st_p = _lib.knnjoin_ex(pts, 64, beta=0.025)
