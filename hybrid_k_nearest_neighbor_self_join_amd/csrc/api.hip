@@ -16,6 +16,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <stdexcept>
@@ -325,6 +326,38 @@ static void sort_perm_desc(const T *var, uint32_t d, uint32_t *perm) {
     }
     perm[j + 1] = p;
   }
+}
+
+// Interval of the variance that main.cu:2537-2562 computes in T arithmetic (sequential sums
+// over the m sampled rows, then / (n * T(1/100))), from exact-ish FP64 moments: with u the
+// unit roundoff of T and g = (m + 4) u (error bound of recursive summation), a = S / (n/100):
+//   |avg_T - a| <= da = (g + 4u) * sum|x| / (n/100),
+//   sum (x - avg_T)^2 = Q + 2 (a - avg_T) C + m (a - avg_T)^2, Q = sum (x - a)^2, C = sum (x - a),
+//   var_T in [(Q - 2 da |C|)(1 - g)(1 - 4u), (Q + 2 da |C| + m da^2)(1 + g)(1 + 4u)] / (n/100).
+// When the intervals of all dims are pairwise disjoint, sorting the FP64 centres descending
+// gives the order the T values give (strictly ordered, so the stable sort's tie rule is moot).
+template <typename T>
+static bool order_from_intervals(const double *mom, uint32_t n, uint32_t d, uint32_t *perm) {
+  const double u = sizeof(T) == 4 ? 0x1p-24 : 0x1p-53;
+  const double m = (double)(((uint64_t)n + 99) / 100);
+  const double g = (m + 4.0) * u * 1.01;
+  const double den = (double)n / 100.0;
+  double lo[KNN_MAXDIM], hi[KNN_MAXDIM], mid[KNN_MAXDIM];
+  for (uint32_t i = 0; i < d; i++) {
+    const double A = mom[4 * i + 1], Q = mom[4 * i + 2], C = mom[4 * i + 3];
+    const double da = (g + 4.0 * u) * A / den;
+    const double slack = 1e-9 * (Q + A * A / m);  // FP64 rounding of the moments themselves
+    lo[i] = (Q - 2.0 * da * std::fabs(C) - slack) * (1.0 - g) * (1.0 - 4.0 * u) / den;
+    hi[i] = (Q + 2.0 * da * std::fabs(C) + m * da * da + slack) * (1.0 + g) * (1.0 + 4.0 * u) / den;
+    mid[i] = Q / den;
+    if (!(lo[i] == lo[i]) || !(hi[i] == hi[i])) return false;
+  }
+  for (uint32_t i = 0; i < d; i++)
+    for (uint32_t j = i + 1; j < d; j++)
+      if (!(lo[i] > hi[j] || lo[j] > hi[i])) return false;
+  for (uint32_t i = 0; i < d; i++) perm[i] = i;
+  std::stable_sort(perm, perm + d, [&](uint32_t a, uint32_t b) { return mid[a] > mid[b]; });
+  return true;
 }
 
 struct EpsResult {
@@ -663,13 +696,22 @@ static int run_join(const T *pts_in, uint32_t n, uint32_t d, uint32_t k, const k
   tPh.start();
   uint32_t perm[KNN_MAXDIM];
   for (uint32_t j = 0; j < KNN_MAXDIM; j++) perm[j] = j;
-  if (p.reorder) {
-    T *var = P.template buf<T>(S_SMALL, 64 * 1024 / sizeof(T));
-    variance_rows<T>(dpts, n, d, var, s);
-    T hv[KNN_MAXDIM];
-    HIP_CHECK(hipMemcpyAsync(hv, var, sizeof(T) * d, hipMemcpyDeviceToHost, s));
+  if (p.reorder && d > 1) {
+    // the order the sequential T variances give, decided from FP64 error intervals when they
+    // separate every pair of dims, else from the sequential sums themselves
+    double *mom = P.template buf<double>(S_SMALL, 64 * 1024 / sizeof(double));
+    variance_moments<T>(dpts, n, d, mom, s);
+    double hm[4 * KNN_MAXDIM];
+    HIP_CHECK(hipMemcpyAsync(hm, mom, sizeof(double) * 4 * d, hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipStreamSynchronize(s));
-    sort_perm_desc<T>(hv, d, perm);
+    if (!order_from_intervals<T>(hm, n, d, perm)) {
+      T *var = P.template buf<T>(S_SMALL, 64 * 1024 / sizeof(T));
+      variance_rows<T>(dpts, n, d, var, s);
+      T hv[KNN_MAXDIM];
+      HIP_CHECK(hipMemcpyAsync(hv, var, sizeof(T) * d, hipMemcpyDeviceToHost, s));
+      HIP_CHECK(hipStreamSynchronize(s));
+      sort_perm_desc<T>(hv, d, perm);
+    }
   }
   P.x = P.template buf<T>(S_X, (size_t)n * d);
   permute_to_soa<T>(dpts, n, d, perm, P.x, s);
@@ -806,6 +848,17 @@ static void run_plain(Pipeline<T> &P, double beta, knnjoin_stats &stats, uint8_t
   stats.kernel_launches = launches;
 }
 
+// Cells per radius-class upper bound R in the learned schedule's grids: 2 with <= 2 indexed
+// dims, 1 above; KNN_LEARNED_GRID_DIV (1..8) overrides it for tuning.
+static double learned_grid_div(uint32_t nd) {
+  static const double env = []() {
+    const char *v = std::getenv("KNN_LEARNED_GRID_DIV");
+    const double x = v ? std::atof(v) : 0.0;
+    return (x >= 1.0 && x <= 8.0) ? x : 0.0;
+  }();
+  return env > 0.0 ? env : (nd <= 2 ? 2.0 : 1.0);
+}
+
 // The learned schedule: per-query initial radius from the MLP, then passes over radius
 // octaves (each on a grid matched to its radii) with the expansion loop (radius x2 for the
 // queries that fail) until every query has its K+1 neighbours.
@@ -872,7 +925,9 @@ static void run_learned(Pipeline<T> &P, const knnjoin_params &p, knnjoin_stats &
     }
     if (pending == 0) break;
     if (passes > 200) throw Error(KNN_ERR_NO_PROGRESS, "learned schedule: queries still failing after 200 passes");
-    const int cHi = std::min(c + 3, 63);
+    // octave classes (4 quarter-octave bins) with <= 2 indexed dims; single bins above, where
+    // a cell up to 2x the radius would cost up to 2^nd in block volume
+    const int cHi = std::min(c + (nd <= 2 ? 3 : 0), 63);
     uint32_t nq = 0;
     for (int b = c; b <= cHi; b++) nq += h[b];
     double R;
@@ -896,7 +951,7 @@ static void run_learned(Pipeline<T> &P, const knnjoin_params &p, knnjoin_stats &
     }
     // cell width R/2 (blocks of <= 5 cells per dim) with <= 2 indexed dims; above, R (<= 3 cells
     // per dim), since every extra cell of reach multiplies the rows of a block
-    T w = (T)(nd <= 2 ? R / 2.0 : R);
+    T w = (T)(R / learned_grid_div(nd));
     if (!(w > (T)0)) w = F.wf;
     for (;;) {  // keep the grid within 2^30 cells
       typename GridOf<T>::type gt;
@@ -929,7 +984,8 @@ static void run_learned(Pipeline<T> &P, const knnjoin_params &p, knnjoin_stats &
       bytesAtMax = pr.bytes_total - prevBytes;
     }
     prevBytes = pr.bytes_total;
-    grow_class_radius(radius, act, failed, n, (T)2, s);
+    // growth on failure: one class (an octave; a quarter octave with >= 3 indexed dims)
+    grow_class_radius(radius, act, failed, n, nd <= 2 ? (T)2 : (T)1.189207115002721, s);
     joinMs += tPh.stop();
     if (P.verbose)
       printf("[knn] learned pass %u octave %d grid %.9g queries %u failed %u tiles %u cells %u span %u overflow %u kernel %.3f ms\n",
@@ -1424,7 +1480,8 @@ struct BatchGroup {
   uint64_t *cellKeys = nullptr;
   unsigned long long *cand = nullptr;
   uint4 *tiles = nullptr;
-  uint8_t *failed = nullptr, *overflow = nullptr;
+  uint8_t *failed = nullptr, *overflow = nullptr, *bin = nullptr, *actq = nullptr;
+  uint32_t *setMinBin = nullptr;
   SetDesc *sets = nullptr;
   int32_t *outIds = nullptr;
   void *heap = nullptr;
@@ -1538,6 +1595,9 @@ static void batch_build(BatchPlan &B, uint32_t nsets, const uint32_t *n, const u
     G.cand = B.alloc<unsigned long long>(4);
     G.failed = B.alloc<uint8_t>(N);
     G.overflow = B.alloc<uint8_t>(N);
+    G.bin = B.alloc<uint8_t>(N);
+    G.actq = B.alloc<uint8_t>(N);
+    G.setMinBin = B.alloc<uint32_t>(G.nsets);
     G.sets = B.alloc<SetDesc>(G.nsets);
     G.outIds = B.alloc<int32_t>(G.outElems);
     G.outDists = B.alloc<float>(G.outElems);
@@ -1584,10 +1644,9 @@ static JoinArgs batch_args(const BatchPlan &B, const BatchGroup &G) {
   return a;
 }
 
-// One radius pass of a group: per-set grids at the current widths, the group index, tiles
-// of the failed (initially all) queries, the select kernel, the heap kernel for staging
-// overflows, and radius growth of the queries that still failed.
-static void batch_pass(const BatchPlan &B, BatchGroup &G, hipStream_t s) {
+// Index of a group at the sets' current widths: per-set grids, cell keys, radix sort, cell
+// table, cell-sorted coordinates, dense cell-start table.
+static void batch_index(BatchGroup &G, hipStream_t s) {
   const uint32_t N = G.N;
   batch_grids(G.sets, G.nsets, G.mnmx, G.width, G.bound, G.status, s);
   batch_cell_keys(G.x, N, G.sets, G.setOf, G.keys, G.ids, s);
@@ -1599,10 +1658,19 @@ static void batch_pass(const BatchPlan &B, BatchGroup &G, hipStream_t s) {
   fill_u32(G.dense, 0u, G.bound + 1, s);
   batch_dense_counts(G.cellKeys, G.cellStart, G.cpos + N, N, G.dense, s);
   scan_exclusive_u32(G.dense, G.dense, (uint32_t)G.bound, G.scratch, s);
+}
+
+// One radius pass of a group: every set's lowest pending octave, grids matched to it, tiles of
+// those queries, the select kernel, the heap kernel for staging overflows, and radius growth
+// of the queries that failed; G.ctr[3] = queries still pending.
+static void batch_pass(const BatchPlan &B, BatchGroup &G, hipStream_t s) {
+  const uint32_t N = G.N;
+  batch_class(G.sets, G.nsets, G.setOf, G.failed, G.bin, N, G.setMinBin, G.width, G.actq, s);
+  batch_index(G, s);
 
   const JoinArgs a = batch_args(B, G);
   const GridParams g0{};
-  batch_flags_u8(G.ids, G.failed, N, G.aflags, s);
+  batch_flags_u8(G.ids, G.actq, N, G.aflags, s);
   scan_exclusive_u32(G.aflags, G.apos, N, G.scratch, s);
   compact_active(G.ids, G.aflags, G.apos, N, G.act, s);
   batch_tiles(G.cellStart, G.apos, G.cpos + N, N, B.tq, G.ntiles, G.tileOff, G.scratch, G.tiles, s);
@@ -1617,7 +1685,7 @@ static void batch_pass(const BatchPlan &B, BatchGroup &G, hipStream_t s) {
   range_topk<float>(a, g0, s);
   fill_u8(G.overflow, 0, N, s);
 
-  batch_grow(G.setOf, G.failed, N, G.radius, G.setMaxBits, G.width, G.nsets, s);
+  batch_grow_class(G.sets, G.setOf, G.actq, G.failed, N, G.radius, G.bin, G.ctr + 3, s);
 }
 
 static void batch_body(const BatchPlan &B, hipStream_t s) {
@@ -1625,9 +1693,11 @@ static void batch_body(const BatchPlan &B, hipStream_t s) {
     batch_variance_perm(G.xin, G.sets, G.nsets, G.d, B.reorder, G.var, G.perm, s);
     batch_permute(G.xin, G.N, G.d, G.setOf, G.perm, G.x, s);
     batch_minmax(G.x, G.N, G.sets, G.nsets, G.mnmx, s);
-    batch_init_radius(G.sets, G.nsets, G.mnmx, G.width, G.setOf, G.N, G.radius, G.failed, s);
+    batch_init_width(G.sets, G.nsets, G.mnmx, G.width, G.N, G.failed, s);
     fill_u8(G.overflow, 0, G.N, s);
     fill_u32(reinterpret_cast<uint32_t *>(G.cand), 0u, 8, s);
+    batch_index(G, s);
+    batch_density_radius(G.x, G.N, G.sets, G.setOf, G.dense, G.radius, G.bin, s);
     for (uint32_t p = 0; p < B.passes; p++) batch_pass(B, G, s);
   }
 }
@@ -1659,7 +1729,7 @@ static int run_batched(uint32_t nsets, const float *const *points, const uint32_
   if (!grid) grid = 2048;
   if (tq > 64) throw Error(KNN_ERR_ARG, "tile_queries must be <= 64");
   if (grid > kBatchMaxGrid) throw Error(KNN_ERR_ARG, "grid_blocks must be <= 8192");
-  const uint32_t passes = p.graph_passes ? p.graph_passes : 3;
+  const uint32_t passes = p.graph_passes ? p.graph_passes : 6;
   Timer tAll(s);
   auto w0 = std::chrono::steady_clock::now();
   if (B.sig != sig) {
@@ -1713,8 +1783,8 @@ static int run_batched(uint32_t nsets, const float *const *points, const uint32_
   for (;;) {
     bool any = false;
     for (BatchGroup &G : B.groups) {
-      uint32_t f = 0;
-      HIP_CHECK(hipMemcpyAsync(&f, G.ctr + 1, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+      uint32_t f = 0;  // queries still pending after the last pass
+      HIP_CHECK(hipMemcpyAsync(&f, G.ctr + 3, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
       HIP_CHECK(hipStreamSynchronize(s));
       if (f) {
         any = true;
@@ -1722,7 +1792,7 @@ static int run_batched(uint32_t nsets, const float *const *points, const uint32_
       }
     }
     if (!any) break;
-    if (++eager > 64) throw Error(KNN_ERR_NO_PROGRESS, "batched join: queries still failing after 64 extra passes");
+    if (++eager > 200) throw Error(KNN_ERR_NO_PROGRESS, "batched join: queries still failing after 200 extra passes");
   }
   stats.ms_device = tAll.stop();
   for (BatchGroup &G : B.groups) {

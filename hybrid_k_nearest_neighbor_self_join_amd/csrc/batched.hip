@@ -8,12 +8,14 @@
 //   one radix sort / cell table / dense start table for the whole group,
 //   tiles of <= TQ queries per cell (a cell belongs to one set), and the range kernels of
 //   range.hip with per-tile set descriptors (grid, K+1, row offset).
-// Radius schedule: every query starts at its set's r0 (the radius whose ball holds about
-// 3 (K+1) points at the set's mean density); failed queries double their radius and the
-// set's grid is rebuilt with cell width = its largest failed radius, so a query's block is
-// always at most 3 cells per indexed dim.  Rows are exact whatever the schedule (a query is
-// accepted only when its K+1 nearest are provably inside its block) and equal the rows of a
-// single-set join of the same set.
+// Radius schedule (per query, any schedule gives the same rows): each set first gets a grid of
+// width wf (about K+1 points per cell at its mean density); a query's initial radius is the
+// one whose ball holds about 2 (K+1) points at the density of its own and the two adjacent
+// cells along dim 0.  Then passes over radius octaves: in each pass every set joins its
+// pending queries of its lowest occupied octave [R/2, R) on a grid matched to R (width R/2
+// with <= 2 indexed dims, R above); failed queries double their radius (a later octave).
+// Rows are exact whatever the radii (a query is accepted only when its K+1 nearest are
+// provably inside its block) and equal the rows of a single-set join of the same set.
 //
 // Every kernel reads its sizes (cells, tiles) from device memory, so a pass is a fixed
 // sequence of launches: the whole join (input copy, reorder, grids, indexes, P passes) is
@@ -125,9 +127,9 @@ __device__ double unit_ball_volume(uint32_t nd) {
   return v;
 }
 
-// Initial radius per set: the ball holding about 3 (K+1) points at the set's mean density
-// over its bounding box; degenerate boxes fall back to the largest extent / 8 (or 1).
-__global__ void k_b_init_radius(SetDesc *sets, uint32_t nsets, const float *mnmx, float *width) {
+// Per set: wf = (V (K+1) / n)^(1/nd) over its bounding box (degenerate boxes: largest extent / 8,
+// or 1); the first grid uses it.  All queries pending.
+__global__ void k_b_init_width(SetDesc *sets, uint32_t nsets, const float *mnmx, float *width) {
   const uint32_t si = blockIdx.x * blockDim.x + threadIdx.x;
   if (si >= nsets) return;
   SetDesc &sd = sets[si];
@@ -137,22 +139,126 @@ __global__ void k_b_init_radius(SetDesc *sets, uint32_t nsets, const float *mnmx
     vol *= e;
     ext = e > ext ? e : ext;
   }
-  double r = 0.0;
-  if (vol > 0.0 && sd.n > 0)
-    r = pow(3.0 * sd.k1 * vol / ((double)sd.n * unit_ball_volume(sd.nd)), 1.0 / sd.nd);
-  if (!(r > 0.0) || !(r < 3.0e38)) r = ext > 0.0 ? ext / 8.0 : 1.0;
-  float rf = (float)r;
-  if (!(rf > 0.0f)) rf = 1.0f;
-  sd.r0 = rf;
-  width[si] = rf;
+  double w = 0.0;
+  if (vol > 0.0 && sd.n > 0) w = pow((double)sd.k1 * vol / (double)sd.n, 1.0 / sd.nd);
+  if (!(w > 0.0) || !(w < 3.0e38)) w = ext > 0.0 ? ext / 8.0 : 1.0;
+  float wf = (float)w;
+  if (!(wf > 0.0f)) wf = 1.0f;
+  sd.r0 = wf;
+  width[si] = wf;
 }
 
-__global__ void k_b_fill_radius(const SetDesc *sets, const uint32_t *setOf, uint32_t N, float *radius,
-                                uint8_t *failed) {
+__global__ void k_b_pending(uint32_t N, uint8_t *failed) {
   const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= N) return;
-  radius[p] = sets[setOf[p]].r0;
-  failed[p] = 1;
+  if (p < N) failed[p] = 1;
+}
+
+// quarter-octave bin of r relative to wf: floor(4 log2(r / wf)) + 32 in [0, 63]
+__device__ __forceinline__ int bin_of(float r, float wf) {
+  const float x = 4.0f * log2f(r / wf);
+  if (!(x >= -32.0f)) return 0;
+  if (x >= 31.0f) return 63;
+  const int b = (int)floorf(x) + 32;
+  return b < 0 ? 0 : (b > 63 ? 63 : b);
+}
+
+// Initial radius of every query from the points of its own and the two adjacent cells along
+// dim 0 on the wf grid: density rho = count / volume, r = (2 (K+1) / (V_nd rho))^(1/nd),
+// kept within [wf / 64, 64 wf].
+__global__ void k_b_density_radius(const float *__restrict__ x, uint32_t N, const SetDesc *sets,
+                                   const uint32_t *__restrict__ setOf, const uint32_t *__restrict__ dense,
+                                   float *radius, uint8_t *bin) {
+  const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= N) return;
+  const SetDesc &sd = sets[setOf[q]];
+  const GridParams &g = sd.g;
+  uint64_t row = g.keyBase;
+  uint32_t c0 = 0;
+  for (uint32_t j = 0; j < g.nd; j++) {
+    const uint32_t c = cell_coord<float>(x[(size_t)j * N + q], g.minArr[j], g.eps, g.nCells[j]);
+    if (j == 0) c0 = c; else row += (uint64_t)c * g.mult[j];
+  }
+  const uint32_t lo = c0 > 0 ? c0 - 1 : 0;
+  const uint32_t hi = c0 + 1 < g.nCells[0] ? c0 + 1 : g.nCells[0] - 1;
+  const uint32_t cnt = dense[row + hi + 1] - dense[row + lo];
+  const double wf = (double)sd.r0;
+  const double vol = (double)(hi - lo + 1) * pow((double)g.eps, (double)g.nd);
+  const double rho = (double)(cnt > 0 ? cnt : 1u) / vol;
+  double r = pow(2.0 * sd.k1 / (unit_ball_volume(g.nd) * rho), 1.0 / g.nd);
+  r = fmin(fmax(r, wf / 64.0), wf * 64.0);
+  radius[q] = (float)r;
+  bin[q] = (uint8_t)bin_of((float)r, sd.r0);
+}
+
+// per set: the lowest radius bin among its pending queries (wave-aggregated; 64 = none)
+__global__ void k_b_min_bin(const uint32_t *__restrict__ setOf, const uint8_t *failed, const uint8_t *bin,
+                            uint32_t N, uint32_t *setMinBin) {
+  const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool pend = q < N && failed[q];
+  uint32_t key = pend ? (setOf[q] << 8) | bin[q] : 0xffffffffu;
+  bool mine = pend;
+  for (;;) {
+    const uint64_t live = __ballot(mine);
+    if (!live) break;
+    const int leader = __ffsll((long long)live) - 1;
+    const uint32_t ls = (uint32_t)__shfl((int)(key >> 8), leader, 64);
+    uint32_t b = (mine && (key >> 8) == ls) ? (key & 0xffu) : 0xffu;
+    for (int o = 32; o > 0; o >>= 1) {
+      const uint32_t t = (uint32_t)__shfl_xor((int)b, o, 64);
+      b = t < b ? t : b;
+    }
+    if ((int)(threadIdx.x & 63) == leader) atomicMin(&setMinBin[ls], b);
+    if ((key >> 8) == ls) mine = false;
+  }
+}
+
+// per set: the pass's radius class (octave [c, c + 3] with <= 2 indexed dims, bin c above)
+// and grid width R / div (R = the class's upper radius); sets without pending queries keep
+// their grid
+__global__ void k_b_class_width(const SetDesc *sets, uint32_t nsets, const uint32_t *setMinBin, float *width) {
+  const uint32_t si = blockIdx.x * blockDim.x + threadIdx.x;
+  if (si >= nsets) return;
+  const uint32_t c = setMinBin[si];
+  if (c >= 64) return;
+  const double span = sets[si].nd <= 2 ? 4.0 : 1.0;  // bins per class
+  const double R = (double)sets[si].r0 * exp2(((double)c + span - 32.0) / 4.0);
+  const double div = sets[si].nd <= 2 ? 2.0 : 1.0;
+  width[si] = (float)(R / div);
+}
+
+// act[q] = pending with a bin in its set's class
+__global__ void k_b_class_flags(const SetDesc *sets, const uint32_t *__restrict__ setOf, const uint8_t *failed,
+                                const uint8_t *bin, const uint32_t *setMinBin, uint32_t N, uint8_t *act) {
+  const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= N) return;
+  bool a = false;
+  if (failed[q]) {
+    const uint32_t si = setOf[q];
+    const uint32_t c = setMinBin[si];
+    a = bin[q] >= c && bin[q] <= c + (sets[si].nd <= 2 ? 3u : 0u);
+  }
+  act[q] = a ? 1 : 0;
+}
+
+// queries of the pass that failed: radius into the next class; pending count into *pending
+__global__ void k_b_grow_class(const SetDesc *sets, const uint32_t *__restrict__ setOf, const uint8_t *act,
+                               const uint8_t *failed, uint32_t N, float *radius, uint8_t *bin,
+                               uint32_t *pending) {
+  const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+  bool pend = false;
+  if (q < N) {
+    pend = failed[q] != 0;
+    if (pend && act[q]) {
+      // one class further: an octave, a quarter octave with >= 3 indexed dims
+      const float r = radius[q] * (sets[setOf[q]].nd <= 2 ? 2.0f : 1.18920712f);
+      radius[q] = r;
+      const int b = bin_of(r, sets[setOf[q]].r0);
+      // at least one bin further, so that every failure moves the query to a later class
+      bin[q] = (uint8_t)(b > (int)bin[q] ? b : ((int)bin[q] + 1 < 64 ? (int)bin[q] + 1 : 63));
+    }
+  }
+  const uint64_t m = __ballot(pend);
+  if ((threadIdx.x & 63) == 0 && m) atomicAdd(pending, (uint32_t)__popcll(m));
 }
 
 // Grid of every set at its requested width (main.cu:2465-2486 arithmetic: lo = min - w,
@@ -269,27 +375,6 @@ __global__ void k_b_write_tiles(const uint32_t *cellStart, const uint32_t *pos, 
     tiles[t] = make_uint4(c, a, (a1 - a < tq) ? a1 - a : tq, a == a0 ? 1u : 0u);
 }
 
-// failed queries: radius *= 2 and the set's largest failed radius (float bits, atomicMax)
-__global__ void k_b_grow(const uint32_t *setOf, const uint8_t *failed, uint32_t N, float *radius,
-                         uint32_t *setMaxBits) {
-  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= N || !failed[p]) return;
-  const float r = radius[p] * 2.0f;
-  radius[p] = r;
-  atomicMax(&setMaxBits[setOf[p]], __float_as_uint(r));
-}
-
-// next width of each set: its largest failed radius when it has failures (else unchanged)
-__global__ void k_b_next_width(const uint32_t *setMaxBits, uint32_t nsets, float *width) {
-  const uint32_t si = blockIdx.x * blockDim.x + threadIdx.x;
-  if (si >= nsets) return;
-  const uint32_t b = setMaxBits[si];
-  if (b) {
-    const float r = __uint_as_float(b);
-    width[si] = r > width[si] ? r : width[si];
-  }
-}
-
 }  // namespace
 
 // ------------------------------------------------------------------ host launchers
@@ -306,10 +391,26 @@ void batch_minmax(const float *x, uint32_t N, const SetDesc *sets, uint32_t nset
                   hipStream_t s) {
   if (nsets) k_b_minmax<<<nsets, 256, 0, s>>>(x, N, sets, mnmx);
 }
-void batch_init_radius(SetDesc *sets, uint32_t nsets, const float *mnmx, float *width,
-                       const uint32_t *setOf, uint32_t N, float *radius, uint8_t *failed, hipStream_t s) {
-  k_b_init_radius<<<nblk(nsets), BLK, 0, s>>>(sets, nsets, mnmx, width);
-  if (N) k_b_fill_radius<<<nblk(N), BLK, 0, s>>>(sets, setOf, N, radius, failed);
+void batch_init_width(SetDesc *sets, uint32_t nsets, const float *mnmx, float *width, uint32_t N,
+                      uint8_t *failed, hipStream_t s) {
+  k_b_init_width<<<nblk(nsets), BLK, 0, s>>>(sets, nsets, mnmx, width);
+  if (N) k_b_pending<<<nblk(N), BLK, 0, s>>>(N, failed);
+}
+void batch_density_radius(const float *x, uint32_t N, const SetDesc *sets, const uint32_t *setOf,
+                          const uint32_t *dense, float *radius, uint8_t *bin, hipStream_t s) {
+  if (N) k_b_density_radius<<<nblk(N), BLK, 0, s>>>(x, N, sets, setOf, dense, radius, bin);
+}
+void batch_class(const SetDesc *sets, uint32_t nsets, const uint32_t *setOf, const uint8_t *failed,
+                 const uint8_t *bin, uint32_t N, uint32_t *setMinBin, float *width, uint8_t *act, hipStream_t s) {
+  fill_u32(setMinBin, 64u, nsets, s);
+  if (N) k_b_min_bin<<<nblk(N), BLK, 0, s>>>(setOf, failed, bin, N, setMinBin);
+  k_b_class_width<<<nblk(nsets), BLK, 0, s>>>(sets, nsets, setMinBin, width);
+  if (N) k_b_class_flags<<<nblk(N), BLK, 0, s>>>(sets, setOf, failed, bin, setMinBin, N, act);
+}
+void batch_grow_class(const SetDesc *sets, const uint32_t *setOf, const uint8_t *act, const uint8_t *failed,
+                      uint32_t N, float *radius, uint8_t *bin, uint32_t *pending, hipStream_t s) {
+  fill_u32(pending, 0u, 1, s);
+  if (N) k_b_grow_class<<<nblk(N), BLK, 0, s>>>(sets, setOf, act, failed, N, radius, bin, pending);
 }
 void batch_grids(SetDesc *sets, uint32_t nsets, const float *mnmx, float *width, uint64_t bound,
                  uint32_t *status, hipStream_t s) {
@@ -333,11 +434,4 @@ void batch_tiles(const uint32_t *cellStart, const uint32_t *pos, const uint32_t 
   scan_exclusive_u32(ntiles, tileOff, N + 1, scratch, s);
   if (N) k_b_write_tiles<<<nblk(N), BLK, 0, s>>>(cellStart, pos, tileOff, ncell, N, tq, tiles);
 }
-void batch_grow(const uint32_t *setOf, const uint8_t *failed, uint32_t N, float *radius,
-                uint32_t *setMaxBits, float *width, uint32_t nsets, hipStream_t s) {
-  fill_u32(setMaxBits, 0u, nsets, s);
-  if (N) k_b_grow<<<nblk(N), BLK, 0, s>>>(setOf, failed, N, radius, setMaxBits);
-  k_b_next_width<<<nblk(nsets), BLK, 0, s>>>(setMaxBits, nsets, width);
-}
-
 }  // namespace knn

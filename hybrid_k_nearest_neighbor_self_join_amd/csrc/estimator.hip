@@ -41,7 +41,24 @@ __device__ T sequential_pass(const T *__restrict__ pts, uint32_t n, uint32_t d, 
     if (nb < m) stage_chunk<T>(pts, d, i, nb, (uint32_t)(m - nb < VAR_CHUNK ? m - nb : VAR_CHUNK), buf[cur ^ 1]);
     if (threadIdx.x == 0) {
       const T *b = buf[cur];
-      for (uint32_t t = 0; t < cnt; t++) {
+      // 8 LDS reads issued ahead of 8 dependent adds (same order of additions)
+      uint32_t t = 0;
+      for (; t + 8 <= cnt; t += 8) {
+        T v[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) v[u] = b[t + u];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+          if (SQ) {
+            T df = v[u] - avg;
+            T sq = df * df;
+            sum += sq;
+          } else {
+            sum += v[u];
+          }
+        }
+      }
+      for (; t < cnt; t++) {
         if (SQ) {
           T df = b[t] - avg;
           T sq = df * df;
@@ -70,6 +87,37 @@ __global__ __launch_bounds__(256) void k_variance(const T *__restrict__ pts, uin
   const T avg = shAvg;
   const T sum2 = sequential_pass<T, true>(pts, n, d, i, avg, buf);
   if (threadIdx.x == 0) var[i] = sum2 / denom;
+}
+
+// Moments of the sampled rows (0, 100, ...) per dim in FP64, for the error intervals of the
+// sequential T-arithmetic variance: mom[4*i + {0,1,2,3}] = {sum x, sum |x|, sum (x-a)^2,
+// sum (x-a)} with a = sum x / (n/100) taken from a first launch (second launch, phase 1).
+template <typename T>
+__global__ __launch_bounds__(256) void k_var_moments(const T *__restrict__ pts, uint32_t n, uint32_t d,
+                                                     int phase, double *mom) {
+  const uint32_t i = blockIdx.y;
+  const uint64_t m = ((uint64_t)n + 99) / 100;
+  double a0 = 0.0, a1 = 0.0;
+  const double avg = phase ? mom[4 * i] / ((double)n / 100.0) : 0.0;
+  for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < m; r += (uint64_t)gridDim.x * blockDim.x) {
+    const double x = (double)pts[r * 100 * (uint64_t)d + i];
+    if (phase == 0) {
+      a0 += x;
+      a1 += fabs(x);
+    } else {
+      const double y = x - avg;
+      a0 += y * y;
+      a1 += y;
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    a0 += __shfl_xor(a0, o, 64);
+    a1 += __shfl_xor(a1, o, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    atomicAdd(&mom[4 * i + 2 * phase], a0);
+    atomicAdd(&mom[4 * i + 2 * phase + 1], a1);
+  }
 }
 
 struct Perm {
@@ -173,6 +221,16 @@ void variance_rows(const T *pts, uint32_t n, uint32_t d, T *var, hipStream_t s) 
 }
 
 template <typename T>
+void variance_moments(const T *pts, uint32_t n, uint32_t d, double *mom, hipStream_t s) {
+  fill_u32(reinterpret_cast<uint32_t *>(mom), 0u, (size_t)8 * d, s);
+  const uint64_t m = ((uint64_t)n + 99) / 100;
+  uint32_t bx = (uint32_t)((m + 255) / 256);
+  bx = bx < 64 ? (bx ? bx : 1) : 64;
+  k_var_moments<T><<<dim3(bx, d), 256, 0, s>>>(pts, n, d, 0, mom);
+  k_var_moments<T><<<dim3(bx, d), 256, 0, s>>>(pts, n, d, 1, mom);
+}
+
+template <typename T>
 void permute_to_soa(const T *pts, uint32_t n, uint32_t d, const uint32_t *perm_host, T *out,
                     hipStream_t s) {
   Perm p;
@@ -203,6 +261,7 @@ void kdist_hist_soa(const T *x, uint32_t n, uint32_t d, uint32_t offset, uint32_
 
 #define INST(T)                                                                            \
   template void variance_rows<T>(const T *, uint32_t, uint32_t, T *, hipStream_t);          \
+  template void variance_moments<T>(const T *, uint32_t, uint32_t, double *, hipStream_t);  \
   template void permute_to_soa<T>(const T *, uint32_t, uint32_t, const uint32_t *, T *,     \
                                   hipStream_t);                                             \
   template void avg_dist_soa<T>(const T *, uint32_t, uint32_t, double *, uint64_t *,        \

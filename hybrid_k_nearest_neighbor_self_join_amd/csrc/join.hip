@@ -133,24 +133,23 @@ __global__ void k_prefilter(const T *__restrict__ x, uint32_t n, G g, const T *r
 }
 
 // sum over cells of cellMax (candidate points of each active query cell's block) and the
-// number of active query cells; out[0] += points, out[1] += cells
-__global__ void k_cell_bytes(const uint32_t *cellMax, uint32_t ncell, unsigned long long *out) {
-  __shared__ unsigned long long sp[256], sc[256];
-  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-  const uint32_t v = c < ncell ? cellMax[c] : 0u;
-  sp[threadIdx.x] = v;
-  sc[threadIdx.x] = v ? 1ull : 0ull;
-  __syncthreads();
-  for (int o = 128; o > 0; o >>= 1) {
-    if ((int)threadIdx.x < o) {
-      sp[threadIdx.x] += sp[threadIdx.x + o];
-      sc[threadIdx.x] += sc[threadIdx.x + o];
-    }
-    __syncthreads();
+// number of active query cells; out[0] += points, out[1] += cells.  Grid-stride per thread,
+// then one wave reduction and one atomic per wave.
+__global__ __launch_bounds__(256) void k_cell_bytes(const uint32_t *cellMax, uint32_t ncell,
+                                                    unsigned long long *out) {
+  unsigned long long sp = 0, sc = 0;
+  for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < ncell; c += gridDim.x * blockDim.x) {
+    const uint32_t v = cellMax[c];
+    sp += v;
+    sc += v ? 1ull : 0ull;
   }
-  if (threadIdx.x == 0) {
-    atomicAdd(out, sp[0]);
-    atomicAdd(out + 1, sc[0]);
+  for (int o = 32; o > 0; o >>= 1) {
+    sp += __shfl_xor(sp, o, 64);
+    sc += __shfl_xor(sc, o, 64);
+  }
+  if ((threadIdx.x & 63) == 0 && (sp || sc)) {
+    atomicAdd(out, sp);
+    atomicAdd(out + 1, sc);
   }
 }
 
@@ -195,17 +194,57 @@ __global__ void k_write_tiles(const uint32_t *cellStart, const uint32_t *pos,
 
 // ------------------------------------------------------------------ checksum
 constexpr uint32_t SUM_CHUNK = 4096;
-__global__ void k_sum_chunks(const double *rowSum, uint32_t n, double *partial) {
-  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-  const uint64_t lo = (uint64_t)c * SUM_CHUNK;
+// one block per chunk of SUM_CHUNK rows: coalesced copy to LDS, then lane 0 adds in row order
+__global__ __launch_bounds__(256) void k_sum_chunks(const double *rowSum, uint32_t n, double *partial) {
+  __shared__ double buf[SUM_CHUNK];
+  const uint64_t lo = (uint64_t)blockIdx.x * SUM_CHUNK;
   if (lo >= n) return;
-  const uint64_t hi = lo + SUM_CHUNK < n ? lo + SUM_CHUNK : n;
-  double s = 0.0;
-  for (uint64_t r = lo; r < hi; r++) s += rowSum[r];
-  partial[c] = s;
+  const uint32_t cnt = (uint32_t)(lo + SUM_CHUNK < n ? SUM_CHUNK : n - lo);
+  for (uint32_t i = threadIdx.x; i < cnt; i += blockDim.x) buf[i] = rowSum[lo + i];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s = 0.0;
+    uint32_t r = 0;
+    for (; r + 8 <= cnt; r += 8) {
+      double v[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) v[u] = buf[r + u];
+#pragma unroll
+      for (int u = 0; u < 8; u++) s += v[u];
+    }
+    for (; r < cnt; r++) s += buf[r];
+    partial[blockIdx.x] = s;
+  }
 }
+// Per-row sums in entry order.  A block stages ROWS consecutive rows (one contiguous range of
+// the table) in LDS with coalesced loads, then thread r sums row r from LDS.
+constexpr uint32_t SUM_LDS_BYTES = 32768;
 template <typename T>
-__global__ void k_row_sums(const T *__restrict__ dists, uint32_t n, uint32_t k1, double *rowSum) {
+__global__ __launch_bounds__(256) void k_row_sums(const T *__restrict__ dists, uint32_t n, uint32_t k1,
+                                                  uint32_t rowsPerBlock, double *rowSum) {
+  __shared__ T buf[SUM_LDS_BYTES / sizeof(T)];
+  const uint64_t r0 = (uint64_t)blockIdx.x * rowsPerBlock;
+  if (r0 >= n) return;
+  const uint32_t rows = (uint32_t)((n - r0) < rowsPerBlock ? (n - r0) : rowsPerBlock);
+  const T *src = dists + r0 * k1;
+  const uint32_t cnt = rows * k1;
+  for (uint32_t i = threadIdx.x; i < cnt; i += blockDim.x) buf[i] = src[i];
+  __syncthreads();
+  for (uint32_t r = threadIdx.x; r < rows; r += blockDim.x) {
+    const T *row = buf + (size_t)r * k1;
+    double s = 0.0;
+    for (uint32_t e = 0; e < k1; e++) {
+      const T d = row[e];
+      const T sq = d * d;
+      s += (double)sq;
+    }
+    rowSum[r0 + r] = s;
+  }
+}
+
+// fallback for rows longer than the LDS stage
+template <typename T>
+__global__ void k_row_sums_direct(const T *__restrict__ dists, uint32_t n, uint32_t k1, double *rowSum) {
   const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= n) return;
   const T *row = dists + (size_t)r * k1;
@@ -279,7 +318,10 @@ void prefilter(const T *x, uint32_t n, const typename GridOf<T>::type &g, const 
         x, n, g, radius, dense, sortedIds, activeSrc, failed, k1, cellOf, cellMax, flags);
 }
 void cell_bytes(const uint32_t *cellMax, uint32_t ncell, unsigned long long *out, hipStream_t s) {
-  if (ncell) k_cell_bytes<<<(ncell + 255) / 256, 256, 0, s>>>(cellMax, ncell, out);
+  if (ncell) {
+    const uint32_t blocks = (ncell + 255) / 256;
+    k_cell_bytes<<<blocks < 2048 ? blocks : 2048, 256, 0, s>>>(cellMax, ncell, out);
+  }
 }
 void cell_of(const uint32_t *flags, const uint32_t *pos, uint32_t n, uint32_t *cellOf, hipStream_t s) {
   if (n) k_cell_of<<<(n + 255) / 256, 256, 0, s>>>(flags, pos, n, cellOf);
@@ -304,12 +346,19 @@ void write_tiles(const uint32_t *cellStart, const uint32_t *pos, const uint32_t 
 
 template <typename T>
 void row_sums(const T *dists, uint32_t n, uint32_t k1, double *rowSum, hipStream_t s) {
-  if (n) k_row_sums<T><<<(n + 255) / 256, 256, 0, s>>>(dists, n, k1, rowSum);
+  if (!n) return;
+  const uint32_t rpb = (uint32_t)(SUM_LDS_BYTES / sizeof(T) / k1);
+  if (rpb >= 1) {
+    const uint32_t rows = rpb < 256 ? rpb : 256;
+    k_row_sums<T><<<(n + rows - 1) / rows, 256, 0, s>>>(dists, n, k1, rows, rowSum);
+  } else {
+    k_row_sums_direct<T><<<(n + 255) / 256, 256, 0, s>>>(dists, n, k1, rowSum);
+  }
 }
 
 void sum_rows(const double *rowSum, uint32_t n, double *partial, double *out, hipStream_t s) {
   const uint32_t m = (n + SUM_CHUNK - 1) / SUM_CHUNK;
-  if (m) k_sum_chunks<<<(m + 63) / 64, 64, 0, s>>>(rowSum, n, partial);
+  if (m) k_sum_chunks<<<m, 256, 0, s>>>(rowSum, n, partial);
   k_sum_final<<<1, 64, 0, s>>>(partial, m, out);
 }
 

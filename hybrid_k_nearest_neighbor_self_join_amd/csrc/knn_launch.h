@@ -39,6 +39,10 @@ void fill_u32(uint32_t *a, uint32_t v, size_t n, hipStream_t s);
 // of the row-major input.  var[d] (device).
 template <typename T>
 void variance_rows(const T *pts, uint32_t n, uint32_t d, T *var, hipStream_t s);
+// FP64 moments of the same sampled rows, 4 per dim: sum x, sum |x|, sum (x-a)^2, sum (x-a)
+// with a = sum x / (n/100); mom holds 4 d doubles (zeroed here)
+template <typename T>
+void variance_moments(const T *pts, uint32_t n, uint32_t d, double *mom, hipStream_t s);
 // out[j*n + i] = pts[i*d + perm[j]]  (row-major -> reordered dim-major)
 template <typename T>
 void permute_to_soa(const T *pts, uint32_t n, uint32_t d, const uint32_t *perm_host, T *out,
@@ -205,8 +209,18 @@ void batch_permute(const float *xin, uint32_t N, uint32_t d, const uint32_t *set
                    float *x, hipStream_t s);
 void batch_minmax(const float *x, uint32_t N, const SetDesc *sets, uint32_t nsets, float *mnmx,
                   hipStream_t s);
-void batch_init_radius(SetDesc *sets, uint32_t nsets, const float *mnmx, float *width,
-                       const uint32_t *setOf, uint32_t N, float *radius, uint8_t *failed, hipStream_t s);
+void batch_init_width(SetDesc *sets, uint32_t nsets, const float *mnmx, float *width, uint32_t N,
+                      uint8_t *failed, hipStream_t s);
+// initial per-query radius from the density around each query on the wf grid (dense table of
+// the current index); bin = quarter-octave of radius / wf
+void batch_density_radius(const float *x, uint32_t N, const SetDesc *sets, const uint32_t *setOf,
+                          const uint32_t *dense, float *radius, uint8_t *bin, hipStream_t s);
+// per set: lowest pending octave, the pass's grid width, and the pass's active flags
+void batch_class(const SetDesc *sets, uint32_t nsets, const uint32_t *setOf, const uint8_t *failed,
+                 const uint8_t *bin, uint32_t N, uint32_t *setMinBin, float *width, uint8_t *act, hipStream_t s);
+// radius * 2 for the pass's failed queries; *pending = queries still pending
+void batch_grow_class(const SetDesc *sets, const uint32_t *setOf, const uint8_t *act, const uint8_t *failed,
+                      uint32_t N, float *radius, uint8_t *bin, uint32_t *pending, hipStream_t s);
 void batch_grids(SetDesc *sets, uint32_t nsets, const float *mnmx, float *width, uint64_t bound,
                  uint32_t *status, hipStream_t s);
 void batch_cell_keys(const float *x, uint32_t N, const SetDesc *sets, const uint32_t *setOf,
@@ -219,8 +233,7 @@ void batch_flags_u8(const uint32_t *sortedIds, const uint8_t *f, uint32_t N, uin
 void batch_tiles(const uint32_t *cellStart, const uint32_t *pos, const uint32_t *ncell, uint32_t N,
                  uint32_t tq, uint32_t *ntiles, uint32_t *tileOff, uint32_t *scratch, uint4 *tiles,
                  hipStream_t s);
-void batch_grow(const uint32_t *setOf, const uint8_t *failed, uint32_t N, float *radius,
-                uint32_t *setMaxBits, float *width, uint32_t nsets, hipStream_t s);
+
 
 // deterministic sum of rowSum over the rows (chunks of 4096 rows, then chunk sums): out[0]
 void sum_rows(const double *rowSum, uint32_t n, double *partial, double *out, hipStream_t s);

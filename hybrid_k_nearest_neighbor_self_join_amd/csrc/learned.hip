@@ -156,8 +156,24 @@ __global__ __launch_bounds__(256) void k_radius_bins(const T *radius, const uint
   __shared__ uint32_t lh[64];
   if (threadIdx.x < 64) lh[threadIdx.x] = 0;
   __syncthreads();
-  const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q < n && pending[q]) atomicAdd(&lh[radius_bin<T>(radius[q], wf)], 1u);
+  // grid-stride (a bounded number of blocks, so few global atomics per bin); within a wave,
+  // one LDS add per distinct bin (a few), not one per query
+  const uint32_t stride = gridDim.x * blockDim.x;
+  const uint32_t rounds = (uint32_t)(((uint64_t)n + stride - 1) / stride);
+  for (uint32_t it = 0; it < rounds; it++) {
+    const uint64_t q = (uint64_t)it * stride + blockIdx.x * blockDim.x + threadIdx.x;
+    const int b = (q < n && pending[q]) ? radius_bin<T>(radius[q], wf) : -1;
+    bool mine = b >= 0;
+    for (;;) {
+      const uint64_t live = __ballot(mine);
+      if (!live) break;
+      const int leader = __ffsll((long long)live) - 1;
+      const int bb = __shfl(b, leader, 64);
+      const uint64_t same = __ballot(mine && b == bb);
+      if ((int)(threadIdx.x & 63) == leader) atomicAdd(&lh[bb], (uint32_t)__popcll(same));
+      if (b == bb) mine = false;
+    }
+  }
   __syncthreads();
   if (threadIdx.x < 64 && lh[threadIdx.x]) atomicAdd(&hist[threadIdx.x], lh[threadIdx.x]);
 }
@@ -328,7 +344,10 @@ void radius_kernel(const T *x, uint32_t n, const typename GridOf<T>::type &g, co
 
 template <typename T>
 void radius_bins(const T *radius, const uint8_t *pending, uint32_t n, float wf, uint32_t *hist, hipStream_t s) {
-  if (n) k_radius_bins<T><<<(n + 255) / 256, 256, 0, s>>>(radius, pending, n, wf, hist);
+  if (n) {
+    const uint32_t blocks = (n + 255) / 256;
+    k_radius_bins<T><<<blocks < 1024 ? blocks : 1024, 256, 0, s>>>(radius, pending, n, wf, hist);
+  }
 }
 template <typename T>
 void radius_class_flags(const T *radius, const uint8_t *pending, uint32_t n, float wf, uint32_t lo,

@@ -305,6 +305,110 @@ __device__ __forceinline__ void tile_accounting(const JoinArgs &a, const uint4 &
   }
 }
 
+// row_chord for one lane's own row (no wave reduction): narrows [l0, h0] to the chord of the
+// ball of squared radius r2 around qd; false when the row lies outside the ball
+template <typename T, int D, typename G>
+__device__ __forceinline__ bool row_chord_lane(const G &g, const uint32_t *cur, const double *qd, double r2,
+                                               uint32_t &l0, uint32_t &h0) {
+  double g2 = 0.0;
+  const double e = (double)g.eps;
+  for (uint32_t j = 1; j < g.nd; j++) {
+    const double mn = (double)g.minArr[j];
+    const double clo = mn + (double)cur[j] * e, chi = clo + e;
+    const double pad = 0x1p-18 * (fabs(mn) + fabs(chi) + e);
+    const double q = pick<D>(qd, j);
+    const double gap = fmax(0.0, fmax((clo - pad) - q, q - (chi + pad)));
+    g2 += gap * gap;
+  }
+  if (!(g2 <= r2)) return false;
+  const double h = sqrt(r2 - g2) * (1.0 + 0x1p-20);
+  const T xa = next_down((T)(qd[0] - h)), xb = next_up((T)(qd[0] + h));
+  const uint32_t l = cell_coord<T>(xa, (T)g.minArr[0], (T)g.eps, g.nCells[0]);
+  const uint32_t u = cell_coord<T>(xb, (T)g.minArr[0], (T)g.eps, g.nCells[0]);
+  l0 = l > l0 ? l : l0;
+  h0 = u < h0 ? u : h0;
+  return l0 <= h0;
+}
+
+// LDS hand-off between the lanes of a wave.  The range kernels run one wave per workgroup,
+// so this is all the synchronisation they need; __syncthreads would also wait for every
+// outstanding global load and store (the row writes included) at each call.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
+// Calls f(p) for every sorted position p of the block [lo, hi] (rows with nothing within
+// sqrt(r2) of qd dropped when chord is set), 64 positions per round, lane = position.  The
+// rows' ranges are looked up in parallel (lane = row, 64 rows per batch) and concatenated
+// through a wave prefix sum, so a round mixes rows and every lane has a candidate.
+// rowB / rowX: 64-entry LDS arrays of the wave.  Wave-uniform arguments.
+template <typename T, int D, typename G, typename F>
+__device__ __forceinline__ void for_each_point_flat(const JoinArgs &a, const G &g, const uint32_t *lo,
+                                                    const uint32_t *hi, bool chord, const double *qd,
+                                                    double r2, uint32_t lane, uint32_t *rowB,
+                                                    uint32_t *rowX, F &&f) {
+  uint64_t R = 1;  // rows of the block (spans are <= the grid's cells, so R <= its cell count)
+  for (uint32_t j = 1; j < g.nd; j++) R *= (uint64_t)(hi[j] - lo[j] + 1);
+  for (uint64_t rb = 0; rb < R; rb += 64) {
+    const uint64_t rr = rb + lane;
+    uint32_t pB = 0, len = 0;
+    if (rr < R) {
+      uint32_t cur[KNN_MAX_INDEXED];
+      uint64_t t = rr;
+      uint64_t rowBase = g.keyBase;
+      for (uint32_t j = 1; j < g.nd; j++) {
+        const uint32_t span = hi[j] - lo[j] + 1;
+        if (t < span) {
+          cur[j] = lo[j] + (uint32_t)t;
+          t = 0;
+        } else {
+          cur[j] = lo[j] + (uint32_t)(t % span);
+          t /= span;
+        }
+        rowBase += (uint64_t)cur[j] * g.mult[j];
+      }
+      uint32_t l0 = lo[0], h0 = hi[0];
+      const bool keep = chord ? row_chord_lane<T, D, G>(g, cur, qd, r2, l0, h0) : true;
+      if (keep) {
+        uint32_t pE;
+        if (a.denseStart) {
+          pB = a.denseStart[rowBase + l0];
+          pE = a.denseStart[rowBase + h0 + 1];
+        } else {
+          const uint32_t cbeg = lower_bound_u64(a.cellKeys, a.ncell, rowBase + l0);
+          const uint32_t cend = upper_bound_u64(a.cellKeys, a.ncell, rowBase + h0);
+          pB = a.cellStart[cbeg];
+          pE = a.cellStart[cend];
+        }
+        len = pE - pB;
+      }
+    }
+    uint32_t inc = len;
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t t = (uint32_t)__shfl_up((int)inc, o, 64);
+      if (lane >= (uint32_t)o) inc += t;
+    }
+    const uint32_t total = (uint32_t)__builtin_amdgcn_readlane((int)inc, 63);
+    rowB[lane] = pB;
+    rowX[lane] = inc - len;  // exclusive offset, non-decreasing over the lanes
+    wave_lds_sync();
+    for (uint32_t f0 = 0; f0 < total; f0 += 64) {
+      const uint32_t fi = f0 + lane;
+      if (fi < total) {
+        // last row whose exclusive offset is <= fi (rows of length 0 share the next offset)
+        uint32_t l = 0, h = 64;
+        while (h - l > 1) {
+          const uint32_t mid = (l + h) >> 1;
+          if (rowX[mid] <= fi) l = mid; else h = mid;
+        }
+        f(rowB[l] + (fi - rowX[l]));
+      }
+    }
+    wave_lds_sync();
+  }
+}
+
 // ====================================================================== select kernel
 constexpr int NB = 64;
 constexpr uint32_t SELECT_MAX_GRID = 2048;
@@ -321,8 +425,8 @@ __device__ __forceinline__ uint32_t bucket_of(T s, T scale) {
 }
 
 // LDS: hist[NB/2][64] u32, two 16-bit bucket counters per word (per-lane histogram
-// columns, then slot cursors), followed by the per-query staging arrays key[SM] (double),
-// id[SM] (u32), s[SM] (T).  A lane with 65536 or more candidates inside F goes to the heap
+// columns, then slot offsets), the collecting query's cursors qh[NB/2], then the per-query
+// staging arrays key[SM] (double), id[SM] (u32), s[SM] (T).  A lane with 65536 or more candidates inside F goes to the heap
 // kernel, so no 16-bit counter or cursor can carry into its neighbour.
 __device__ __forceinline__ uint32_t hword(uint32_t b, uint32_t col) { return (b >> 1) * 64 + col; }
 __device__ __forceinline__ uint32_t hshift(uint32_t b) { return (b & 1u) << 4; }
@@ -331,7 +435,12 @@ __global__ __launch_bounds__(64) void k_range_select(JoinArgs a, G g0) {
   extern __shared__ __align__(16) unsigned char smem[];
   uint32_t *hist = reinterpret_cast<uint32_t *>(smem);
   const uint32_t SM = a.smax;
-  double *skey = reinterpret_cast<double *>(smem + (NB / 2) * 64 * sizeof(uint32_t));
+  // the collecting query's slot cursors, copied out of its histogram column: in the column
+  // (word stride 64) every lane's atomic would hit the same LDS bank
+  uint32_t *qh = hist + (NB / 2) * 64;
+  uint32_t *rowB = qh + NB / 2;
+  uint32_t *rowX = rowB + 64;
+  double *skey = reinterpret_cast<double *>(smem + ((NB / 2) * 64 + NB / 2 + 128) * sizeof(uint32_t));
   uint32_t *sid = reinterpret_cast<uint32_t *>(skey + SM);
   T *ss = reinterpret_cast<T *>(sid + SM + (SM & 1));
   const uint32_t lane = threadIdx.x;
@@ -352,7 +461,7 @@ __global__ __launch_bounds__(64) void k_range_select(JoinArgs a, G g0) {
     uint64_t cellId;
     tile_setup<T, D, G>(a, g, tl, lane, L, lo, hi, ownLo, ownHi, cellId);
     const T scale = (T)NB / L.F;
-    __syncthreads();  // the previous tile's readers of hist/staging are done
+    wave_lds_sync();  // the previous tile's readers of hist/staging are done
 #pragma unroll 8
     for (int w = 0; w < NB / 2; w++) hist[w * 64 + lane] = 0u;
     uint64_t tileCand = 0;
@@ -415,7 +524,7 @@ __global__ __launch_bounds__(64) void k_range_select(JoinArgs a, G g0) {
     const T edge = go ? ((T)(bstar + 1) / scale) * Num<T>::margin : (T)0;
     bool okMine = false, ovfMine = preOvf;
     const uint64_t goMask = __ballot(go);
-    __syncthreads();
+    wave_lds_sync();
 
     // ---- pass 2 (wave = query): for each query, scan its own sub-block (radius sqrt(edge)),
     // stage the candidates of buckets <= b* (plus the margin) in LDS in bucket order, rank
@@ -446,41 +555,35 @@ __global__ __launch_bounds__(64) void k_range_select(JoinArgs a, G g0) {
         slo[j] = cell_coord<T>(next_down(xj - ri), (T)g.minArr[j], eps, g.nCells[j]);
         shi[j] = cell_coord<T>(next_up(xj + ri), (T)g.minArr[j], eps, g.nCells[j]);
       }
+      if (lane < (uint32_t)(NB / 2)) qh[lane] = hist[lane * 64 + i];
+      wave_lds_sync();
       bool ovf = false;
       const double ri2 = (double)ri * (double)ri * (1.0 + 0x1p-15);
-      auto narrow2 = [&](const uint32_t *cur, uint32_t &l0, uint32_t &h0) {
-        return g.nd < 3 ? true : row_chord<T, D, G>(g, cur, qdi, ri2, true, l0, h0);
-      };
-      for_each_row(a, g, slo, shi, narrow2, [&](uint32_t pBeg, uint32_t pEnd) {
-        for (uint32_t base = pBeg; base < pEnd; base += 64) {
-          const uint32_t p = base + lane;
-          if (p < pEnd) {
-            T c[D];
+      for_each_point_flat<T, D, G>(a, g, slo, shi, g.nd >= 3, qdi, ri2, lane, rowB, rowX, [&](uint32_t p) {
+        T c[D];
 #pragma unroll
-            for (int j = 0; j < D; j++) c[j] = xs[(size_t)j * n + p];
-            const T s = Num<T>::template sq<D>(c, qi);
-            if (s <= Fi) {
-              uint32_t b = bucket_of<T>(s, sci);
-              if (b <= bsi || s <= edi) {
-                b = b < bsi ? b : bsi;
-                const uint32_t slot = (atomicAdd(&hist[hword(b, i)], 1u << hshift(b)) >> hshift(b)) & 0xffffu;
-                if (slot < SM) {
-                  skey[slot] = fp64_key<D, T>(c, qdi);
-                  sid[slot] = a.sortedIds[p];
-                  ss[slot] = s;
-                } else {
-                  ovf = true;
-                }
-              }
+        for (int j = 0; j < D; j++) c[j] = xs[(size_t)j * n + p];
+        const T s = Num<T>::template sq<D>(c, qi);
+        if (s <= Fi) {
+          uint32_t b = bucket_of<T>(s, sci);
+          if (b <= bsi || s <= edi) {
+            b = b < bsi ? b : bsi;
+            const uint32_t slot = (atomicAdd(&qh[b >> 1], 1u << hshift(b)) >> hshift(b)) & 0xffffu;
+            if (slot < SM) {
+              skey[slot] = fp64_key<D, T>(c, qdi);
+              sid[slot] = a.sortedIds[p];
+              ss[slot] = s;
+            } else {
+              ovf = true;
             }
           }
         }
       });
-      __syncthreads();
-      const uint32_t m = (hist[hword(bsi, i)] >> hshift(bsi)) & 0xffffu;
+      wave_lds_sync();
+      const uint32_t m = (qh[bsi >> 1] >> hshift(bsi)) & 0xffffu;
       if (__ballot(ovf) || m > SM) {
         if (lane == i) ovfMine = true;
-        __syncthreads();
+        wave_lds_sync();
         continue;
       }
       // rank: items of buckets <= b-2 all precede an item of bucket b, items of buckets >= b+2
@@ -498,8 +601,8 @@ __global__ __launch_bounds__(64) void k_range_select(JoinArgs a, G g0) {
           const uint32_t it = sid[tI];
           uint32_t b = bucket_of<T>(ss[tI], sci);
           b = b < bsi ? b : bsi;
-          const uint32_t w0 = b >= 2 ? (hist[hword(b - 2, i)] >> hshift(b - 2)) & 0xffffu : 0u;
-          const uint32_t w1 = b + 1 < bsi ? (hist[hword(b + 1, i)] >> hshift(b + 1)) & 0xffffu : m;
+          const uint32_t w0 = b >= 2 ? (qh[(b - 2) >> 1] >> hshift(b - 2)) & 0xffffu : 0u;
+          const uint32_t w1 = b + 1 < bsi ? (qh[(b + 1) >> 1] >> hshift(b + 1)) & 0xffffu : m;
           uint32_t r = w0;
           for (uint32_t v = w0; v < w1; v++) r += kless(skey[v], sid[v], kt, it) ? 1u : 0u;
           rk[u] = r;
@@ -519,7 +622,7 @@ __global__ __launch_bounds__(64) void k_range_select(JoinArgs a, G g0) {
         }
       }
       if (lane == i) okMine = okq;
-      __syncthreads();
+      wave_lds_sync();
     }
 
     ovfMine = ovfMine || (L.active && big);
@@ -728,7 +831,7 @@ uint32_t range_select_grid(uint32_t ntiles) {
 
 template <typename T>
 size_t range_select_lds_bytes(uint32_t smax) {
-  return (size_t)(NB / 2) * 64 * sizeof(uint32_t) + (size_t)smax * 8 +
+  return ((size_t)(NB / 2) * 64 + NB / 2 + 128) * sizeof(uint32_t) + (size_t)smax * 8 +
          (size_t)(smax + (smax & 1)) * 4 + (size_t)smax * sizeof(T);
 }
 
