@@ -390,8 +390,35 @@ __device__ __forceinline__ void for_each_point_flat(const JoinArgs &a, const G &
       if (lane >= (uint32_t)o) inc += t;
     }
     const uint32_t total = (uint32_t)__builtin_amdgcn_readlane((int)inc, 63);
+    const uint32_t excl = inc - len;  // exclusive offset, non-decreasing over the lanes
+    const uint64_t left = R - rb;
+    if (left <= 8) {
+      // few rows (the usual case with <= 2 indexed dims): offsets and starts in scalar
+      // registers, a candidate's row found by compares
+      uint32_t X[8], Bs[8];
+#pragma unroll
+      for (int k = 0; k < 8; k++) {
+        X[k] = (uint32_t)__builtin_amdgcn_readlane((int)excl, k);
+        Bs[k] = (uint32_t)__builtin_amdgcn_readlane((int)pB, k);
+      }
+      const uint32_t nr = (uint32_t)left;
+      for (uint32_t f0 = 0; f0 < total; f0 += 64) {
+        const uint32_t fi = f0 + lane;
+        if (fi < total) {
+          uint32_t b = Bs[0], x = X[0];
+#pragma unroll
+          for (int k = 1; k < 8; k++)
+            if ((uint32_t)k < nr && fi >= X[k]) {
+              b = Bs[k];
+              x = X[k];
+            }
+          f(b + (fi - x));
+        }
+      }
+      continue;
+    }
     rowB[lane] = pB;
-    rowX[lane] = inc - len;  // exclusive offset, non-decreasing over the lanes
+    rowX[lane] = excl;
     wave_lds_sync();
     for (uint32_t f0 = 0; f0 < total; f0 += 64) {
       const uint32_t fi = f0 + lane;
@@ -563,6 +590,7 @@ __global__ __launch_bounds__(64) void k_range_select(JoinArgs a, G g0) {
         T c[D];
 #pragma unroll
         for (int j = 0; j < D; j++) c[j] = xs[(size_t)j * n + p];
+        const uint32_t cid = a.sortedIds[p];  // with the coordinates: one memory round trip
         const T s = Num<T>::template sq<D>(c, qi);
         if (s <= Fi) {
           uint32_t b = bucket_of<T>(s, sci);
@@ -571,7 +599,7 @@ __global__ __launch_bounds__(64) void k_range_select(JoinArgs a, G g0) {
             const uint32_t slot = (atomicAdd(&qh[b >> 1], 1u << hshift(b)) >> hshift(b)) & 0xffffu;
             if (slot < SM) {
               skey[slot] = fp64_key<D, T>(c, qdi);
-              sid[slot] = a.sortedIds[p];
+              sid[slot] = cid;
               ss[slot] = s;
             } else {
               ovf = true;
@@ -604,7 +632,14 @@ __global__ __launch_bounds__(64) void k_range_select(JoinArgs a, G g0) {
           const uint32_t w0 = b >= 2 ? (qh[(b - 2) >> 1] >> hshift(b - 2)) & 0xffffu : 0u;
           const uint32_t w1 = b + 1 < bsi ? (qh[(b + 1) >> 1] >> hshift(b + 1)) & 0xffffu : m;
           uint32_t r = w0;
-          for (uint32_t v = w0; v < w1; v++) r += kless(skey[v], sid[v], kt, it) ? 1u : 0u;
+          uint32_t v = w0;
+          for (; v + 4 <= w1; v += 4) {  // 4 independent LDS reads per wait
+            const double k0 = skey[v], k1 = skey[v + 1], k2 = skey[v + 2], k3 = skey[v + 3];
+            const uint32_t i0 = sid[v], i1 = sid[v + 1], i2 = sid[v + 2], i3 = sid[v + 3];
+            r += (kless(k0, i0, kt, it) ? 1u : 0u) + (kless(k1, i1, kt, it) ? 1u : 0u) +
+                 (kless(k2, i2, kt, it) ? 1u : 0u) + (kless(k3, i3, kt, it) ? 1u : 0u);
+          }
+          for (; v < w1; v++) r += kless(skey[v], sid[v], kt, it) ? 1u : 0u;
           rk[u] = r;
           if (r == K1 - 1 && kt <= Tqi) okHere = true;
         }
