@@ -34,7 +34,7 @@ EXPORTED_SYMBOLS = [
     "knnjoin_set_radius_model", "knnjoin_set_radius_model_dims", "knnjoin_radius_features",
     "knnjoin_mlp_forward",
     "knnjoin_mlp_grad", "knnjoin_mlp_train", "knnjoin_batch_default_params", "knnjoin_batched",
-    "knnjoin_batch_autotune",
+    "knnjoin_batch_autotune", "knnjoin_outlier_scores",
 ]
 MLP_NW = 5057
 
@@ -146,6 +146,8 @@ def _setup(L):
     L.knnjoin_mlp_grad.restype = i32
     L.knnjoin_mlp_train.argtypes = [vp, vp, vp, u32, u32, u32, flt, u64, vp]
     L.knnjoin_mlp_train.restype = i32
+    L.knnjoin_outlier_scores.argtypes = [vp, vp, u32, u32, i32, vp, vp, vp, vp]
+    L.knnjoin_outlier_scores.restype = i32
 
 
 def p(a) -> ctypes.c_void_p | None:
@@ -287,3 +289,15 @@ def mlp_train(weights: np.ndarray, X: np.ndarray, Y: np.ndarray, epochs: int, ba
     losses = np.zeros(epochs, np.float32)
     check(lib().knnjoin_mlp_train(p(w), p(X), p(Y), len(X), epochs, batch, lr, seed, p(losses)))
     return w, losses
+
+
+def outlier_scores(ids: np.ndarray, dists: np.ndarray):
+    """Outlier scores of an N x (K+1) table (host arrays): (dist_sum float32, dist_rank,
+    indegree, indegree_rank uint32); see knnjoin_outlier_scores."""
+    ids = np.ascontiguousarray(ids, np.int32)
+    dists = np.ascontiguousarray(dists, np.float32)
+    n, k1 = ids.shape
+    ds = np.zeros(n, np.float32)
+    dr, dg, gr = (np.zeros(n, np.uint32) for _ in range(3))
+    check(lib().knnjoin_outlier_scores(p(ids), p(dists), n, k1, 0, p(ds), p(dr), p(dg), p(gr)))
+    return ds, dr, dg, gr

@@ -1271,6 +1271,48 @@ int knnjoin_stage_join_pass(const float *points, uint32_t n, uint32_t ndim, uint
 
 }  // extern "C"
 
+// ------------------------------------------------------------------ outlier scores
+extern "C" {
+
+int knnjoin_outlier_scores(const int32_t *ids, const float *dists, uint32_t n, uint32_t k1,
+                           int device_ptrs, float *dist_sum, uint32_t *dist_rank, uint32_t *indegree,
+                           uint32_t *indegree_rank) {
+  return knn::guarded([&]() -> int {
+    using namespace knn;
+    if (n == 0) return KNN_OK;
+    if (!ids || !dists || !dist_sum || !dist_rank || !indegree || !indegree_rank || k1 == 0)
+      throw Error(KNN_ERR_ARG, "bad outlier-score arguments");
+    hipStream_t s = lib_stream();
+    const size_t tab = (size_t)n * k1;
+    const int32_t *dIds = ids;
+    const float *dD = dists;
+    if (!device_ptrs) {
+      int32_t *a = reinterpret_cast<int32_t *>(g_cache.get(S_OUTIDS, sizeof(int32_t) * tab));
+      float *b = reinterpret_cast<float *>(g_cache.get(S_OUTDISTS, sizeof(float) * tab));
+      HIP_CHECK(hipMemcpyAsync(a, ids, sizeof(int32_t) * tab, hipMemcpyHostToDevice, s));
+      HIP_CHECK(hipMemcpyAsync(b, dists, sizeof(float) * tab, hipMemcpyHostToDevice, s));
+      dIds = a;
+      dD = b;
+    }
+    const size_t sw = radix_scratch_words(n) + 1024;
+    uint32_t *w = reinterpret_cast<uint32_t *>(g_cache.get(S_TRAIN, sizeof(uint32_t) * ((size_t)8 * n + sw)));
+    float *ds = reinterpret_cast<float *>(w);
+    uint32_t *dr = w + n, *dg = w + 2 * (size_t)n, *gr = w + 3 * (size_t)n;
+    uint32_t *k0 = w + 4 * (size_t)n, *v0 = w + 5 * (size_t)n, *k1t = w + 6 * (size_t)n, *v1t = w + 7 * (size_t)n;
+    uint32_t *scr = w + 8 * (size_t)n;
+    outlier_scores(dIds, dD, n, k1, ds, dr, dg, gr, k0, v0, k1t, v1t, scr, s);
+    const hipMemcpyKind kout = device_ptrs ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+    HIP_CHECK(hipMemcpyAsync(dist_sum, ds, sizeof(float) * n, kout, s));
+    HIP_CHECK(hipMemcpyAsync(dist_rank, dr, sizeof(uint32_t) * n, kout, s));
+    HIP_CHECK(hipMemcpyAsync(indegree, dg, sizeof(uint32_t) * n, kout, s));
+    HIP_CHECK(hipMemcpyAsync(indegree_rank, gr, sizeof(uint32_t) * n, kout, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    return KNN_OK;
+  });
+}
+
+}  // extern "C"
+
 // ------------------------------------------------------------------ learned-radius entries
 extern "C" {
 
@@ -1729,7 +1771,7 @@ static int run_batched(uint32_t nsets, const float *const *points, const uint32_
   if (!grid) grid = 2048;
   if (tq > 64) throw Error(KNN_ERR_ARG, "tile_queries must be <= 64");
   if (grid > kBatchMaxGrid) throw Error(KNN_ERR_ARG, "grid_blocks must be <= 8192");
-  const uint32_t passes = p.graph_passes ? p.graph_passes : 6;
+  const uint32_t passes = p.graph_passes ? p.graph_passes : 8;
   Timer tAll(s);
   auto w0 = std::chrono::steady_clock::now();
   if (B.sig != sig) {

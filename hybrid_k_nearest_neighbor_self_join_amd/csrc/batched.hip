@@ -12,8 +12,9 @@
 // width wf (about K+1 points per cell at its mean density); a query's initial radius is the
 // one whose ball holds about 2 (K+1) points at the density of its own and the two adjacent
 // cells along dim 0.  Then passes over radius octaves: in each pass every set joins its
-// pending queries of its lowest occupied octave [R/2, R) on a grid matched to R (width R/2
-// with <= 2 indexed dims, R above); failed queries double their radius (a later octave).
+// pending queries of its lowest occupied radius class (an octave [R/2, R) with <= 2 indexed
+// dims, half an octave above) on a grid matched to R (width R/2 with <= 2 indexed dims, R
+// above); failed queries move their radius one class up.
 // Rows are exact whatever the radii (a query is accepted only when its K+1 nearest are
 // provably inside its block) and equal the rows of a single-set join of the same set.
 //
@@ -212,7 +213,8 @@ __global__ void k_b_min_bin(const uint32_t *__restrict__ setOf, const uint8_t *f
   }
 }
 
-// per set: the pass's radius class (octave [c, c + 3] with <= 2 indexed dims, bin c above)
+// per set: the pass's radius class (octave [c, c + 3] with <= 2 indexed dims, half octave
+// [c, c + 1] above)
 // and grid width R / div (R = the class's upper radius); sets without pending queries keep
 // their grid
 __global__ void k_b_class_width(const SetDesc *sets, uint32_t nsets, const uint32_t *setMinBin, float *width) {
@@ -220,7 +222,7 @@ __global__ void k_b_class_width(const SetDesc *sets, uint32_t nsets, const uint3
   if (si >= nsets) return;
   const uint32_t c = setMinBin[si];
   if (c >= 64) return;
-  const double span = sets[si].nd <= 2 ? 4.0 : 1.0;  // bins per class
+  const double span = sets[si].nd <= 2 ? 4.0 : 2.0;  // bins per class
   const double R = (double)sets[si].r0 * exp2(((double)c + span - 32.0) / 4.0);
   const double div = sets[si].nd <= 2 ? 2.0 : 1.0;
   width[si] = (float)(R / div);
@@ -235,7 +237,7 @@ __global__ void k_b_class_flags(const SetDesc *sets, const uint32_t *__restrict_
   if (failed[q]) {
     const uint32_t si = setOf[q];
     const uint32_t c = setMinBin[si];
-    a = bin[q] >= c && bin[q] <= c + (sets[si].nd <= 2 ? 3u : 0u);
+    a = bin[q] >= c && bin[q] <= c + (sets[si].nd <= 2 ? 3u : 1u);
   }
   act[q] = a ? 1 : 0;
 }
@@ -249,8 +251,8 @@ __global__ void k_b_grow_class(const SetDesc *sets, const uint32_t *__restrict__
   if (q < N) {
     pend = failed[q] != 0;
     if (pend && act[q]) {
-      // one class further: an octave, a quarter octave with >= 3 indexed dims
-      const float r = radius[q] * (sets[setOf[q]].nd <= 2 ? 2.0f : 1.18920712f);
+      // one class further: an octave, half an octave with >= 3 indexed dims
+      const float r = radius[q] * (sets[setOf[q]].nd <= 2 ? 2.0f : 1.41421356f);
       radius[q] = r;
       const int b = bin_of(r, sets[setOf[q]].r0);
       // at least one bin further, so that every failure moves the query to a later class

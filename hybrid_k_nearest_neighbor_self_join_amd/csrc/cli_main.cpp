@@ -8,8 +8,10 @@
 // in the reference).  N = floor(values / DIM); trailing values are dropped.
 //
 // Output: the reference's console lines ("[Verification] Total distance (without square
-// root): ..."), one line appended to gpu_stats.txt (main.cu:470-489), and with --print-table
-// the files KNN_out.txt and KNN_out_distances.txt (printNeighborTable, main.cu:911-949).
+// root): ..."), one line appended to gpu_stats.txt (main.cu:470-489), with --print-table
+// the files KNN_out.txt and KNN_out_distances.txt (printNeighborTable, main.cu:911-949), and
+// with --outlier-scores the file KNN_outlier_scores.txt (printOutlierScores,
+// main.cu:952-1067; float32 only).
 //
 // Options (the reference sets these at compile time in params.h):
 //   --indexed-dims N   NUMINDEXEDDIM (default min(DIM, 6))
@@ -18,6 +20,7 @@
 //   --learned FILE     learned per-query radius: FILE holds 5057 little-endian float32
 //                      weights followed by the float32 margin (written by learned_radius.py)
 //   --print-table      write KNN_out.txt / KNN_out_distances.txt
+//   --outlier-scores   write KNN_outlier_scores.txt
 //   --parse-only       read the file and print N and a digest of the values (no GPU use)
 #include <cinttypes>
 #include <cstdio>
@@ -82,6 +85,23 @@ void write_table(unsigned n, unsigned k1, const int32_t *ids, const T *d) {
   }
 }
 
+// main.cu:1050-1058: per point, mean distance (row sum / (K+1 - 1)), its rank, in-degree, its rank
+void write_outlier_scores(unsigned n, unsigned k1, const int32_t *ids, const float *d) {
+  std::vector<float> sum(n);
+  std::vector<uint32_t> dr(n), deg(n), gr(n);
+  if (knnjoin_outlier_scores(ids, d, n, k1, 0, sum.data(), dr.data(), deg.data(), gr.data()) != KNN_OK) {
+    std::fprintf(stderr, "outlier scores: %s\n", knnjoin_last_error());
+    return;
+  }
+  const char *fname = "KNN_outlier_scores.txt";
+  std::printf("\nOutputting outlier scores to: %s\n", fname);
+  std::ofstream o(fname, std::ios::out);
+  o << "#data point (line is the point id), col0: mean distance between point and its k neighbors,";
+  o << "col1: outlier ranking for col0, col2: the number of times the point appears in another set (in-degree), col3: the ranking for col2 \n";
+  for (size_t i = 0; i < n; i++)
+    o << sum[i] / ((k1 * 1.0) - 1.0) << ", " << dr[i] << ", " << deg[i] << ", " << gr[i] << std::endl;
+}
+
 bool load_model(const char *path, uint32_t nd) {
   std::ifstream f(path, std::ios::binary);
   if (!f) {
@@ -104,7 +124,7 @@ bool load_model(const char *path, uint32_t nd) {
 
 template <typename T>
 int run(const char *fname, unsigned ndim, unsigned k, knnjoin_params &p, bool printTable,
-        bool parseOnly, const char *dtypeName) {
+        bool outlierScores, bool parseOnly, const char *dtypeName) {
   std::vector<T> vals;
   if (!import_values<T>(fname, vals)) {
     std::fprintf(stderr, "cannot open %s\n", fname);
@@ -136,6 +156,12 @@ int run(const char *fname, unsigned ndim, unsigned k, knnjoin_params &p, bool pr
   std::printf("\nTotal time (device pipeline): %f", st.ms_total / 1e3);
   std::printf("\n[Verification] Total distance (without square root): %f\n", st.checksum);
   if (printTable) write_table<T>((unsigned)n, k1, ids.data(), d.data());
+  if (outlierScores) {
+    if (sizeof(T) == 4)
+      write_outlier_scores((unsigned)n, k1, ids.data(), reinterpret_cast<const float *>(d.data()));
+    else
+      std::fprintf(stderr, "--outlier-scores needs float32 data\n");
+  }
   std::ofstream gs("gpu_stats.txt", std::ios::app);
   gs << st.ms_total / 1e3 << ", " << fname << ", KNN: " << k << ", Eps bucket: " << st.eps_bucket
      << ", Total dist: " << std::setprecision(9) << st.checksum << ", "
@@ -152,7 +178,7 @@ int main(int argc, char **argv) {
   std::vector<const char *> pos;
   knnjoin_params p;
   knnjoin_default_params(&p);
-  bool dbl = false, printTable = false, parseOnly = false;
+  bool dbl = false, printTable = false, parseOnly = false, outlierScores = false;
   const char *model = nullptr;
   for (int i = 1; i < argc; i++) {
     std::string a = argv[i];
@@ -168,6 +194,7 @@ int main(int argc, char **argv) {
     else if (a == "--double") dbl = true;
     else if (a == "--learned") model = need("--learned");
     else if (a == "--print-table") printTable = true;
+    else if (a == "--outlier-scores") outlierScores = true;
     else if (a == "--parse-only") parseOnly = true;
     else pos.push_back(argv[i]);
   }
@@ -187,6 +214,6 @@ int main(int argc, char **argv) {
     if (!load_model(model, nd)) return 1;
     p.mode = KNN_MODE_LEARNED;
   }
-  return dbl ? run<double>(pos[0], ndim, k, p, printTable, parseOnly, "double")
-             : run<float>(pos[0], ndim, k, p, printTable, parseOnly, "float");
+  return dbl ? run<double>(pos[0], ndim, k, p, printTable, outlierScores, parseOnly, "double")
+             : run<float>(pos[0], ndim, k, p, printTable, outlierScores, parseOnly, "float");
 }

@@ -235,6 +235,14 @@ void batch_tiles(const uint32_t *cellStart, const uint32_t *pos, const uint32_t 
                  hipStream_t s);
 
 
+// ------------------------------------------------------------- outlier scores (outlier.hip)
+// main.cu:952-1067 over an N x k1 table: FP32 row sums of the distances (entry order) and
+// their rank (descending), in-degree of every point and its rank (ascending); equal values
+// ranked by point id.  keys/vals/keysTmp/valsTmp: n words each; scratch: radix_scratch_words(n).
+void outlier_scores(const int32_t *ids, const float *dists, uint32_t n, uint32_t k1, float *distSum,
+                    uint32_t *distRank, uint32_t *indeg, uint32_t *indegRank, uint32_t *keys,
+                    uint32_t *vals, uint32_t *keysTmp, uint32_t *valsTmp, uint32_t *scratch, hipStream_t s);
+
 // deterministic sum of rowSum over the rows (chunks of 4096 rows, then chunk sums): out[0]
 void sum_rows(const double *rowSum, uint32_t n, double *partial, double *out, hipStream_t s);
 

@@ -145,6 +145,19 @@ int knnjoin_stage_join_pass(const float *points, uint32_t n, uint32_t ndim, uint
                             uint32_t *num_failed);
 
 /* ---------------------------------------------------------------------------------------
+ * Outlier scores of a neighbour table (printOutlierScores, main.cu:952-1067; written by the
+ * reference's CLI when PRINTOUTLIERSCORES is set, params.h:22).  ids/dists: n rows of k1
+ * entries (host, or device with device_ptrs).  Per point: dist_sum = FP32 sum of its row's
+ * distances in entry order (the reference prints dist_sum / (k1 - 1)), dist_rank = its position
+ * ordered by dist_sum descending, indegree = rows (its own included) it appears in,
+ * indegree_rank = its position ordered by indegree ascending.  The reference's std::sort
+ * leaves equal values in unspecified order; here they are ordered by point id.
+ * ------------------------------------------------------------------------------------- */
+int knnjoin_outlier_scores(const int32_t *ids, const float *dists, uint32_t n, uint32_t k1,
+                           int device_ptrs, float *dist_sum, uint32_t *dist_rank, uint32_t *indegree,
+                           uint32_t *indegree_rank);
+
+/* ---------------------------------------------------------------------------------------
  * Learned per-query initial radius (no counterpart in the reference, whose only schedule is
  * the BETA estimate + growth loop of main.cu:305-430; used by KNN_MODE_LEARNED).
  * MLP 12 -> 64 -> 64 -> 1, ReLU, FP32; flat weights W1[64][12], b1[64], W2[64][64], b2[64],
@@ -190,7 +203,7 @@ typedef struct {
   int32_t reorder;           /* 1 = reorder each set's dims by variance */
   int32_t device_ptrs;       /* 1 = set points and outputs are device pointers */
   int32_t use_graph;         /* 1 = capture the join in a hipGraph, replay for equal shapes */
-  uint32_t graph_passes;     /* radius passes inside the graph (0 = 6); later passes run eagerly */
+  uint32_t graph_passes;     /* radius passes inside the graph (0 = 8); later passes run eagerly */
   uint32_t tile_queries;     /* queries per tile, <= 64 (0 = autotuned value or 64) */
   uint32_t grid_blocks;      /* persistent range-kernel blocks, <= 8192 (0 = autotuned or 2048) */
   int32_t verbose;

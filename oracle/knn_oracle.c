@@ -425,3 +425,47 @@ void orc_radius_features(const float *pts, uint32_t n, uint32_t d, uint32_t k, u
   free(cnt);
   free(cc);
 }
+
+
+/* ---- outlier scores (main.cu:952-1067) ---- */
+static const float *g_sort_sum;
+static const uint32_t *g_sort_deg;
+static int cmp_sum_desc(const void *a, const void *b) {
+  const uint32_t i = *(const uint32_t *)a, j = *(const uint32_t *)b;
+  const float x = g_sort_sum[i], y = g_sort_sum[j];
+  const int xn = x != x, yn = y != y;
+  if (xn != yn) return xn - yn; /* NaN last */
+  if (!xn && x > y) return -1;
+  if (!xn && x < y) return 1;
+  return (i > j) - (i < j);
+}
+static int cmp_deg_asc(const void *a, const void *b) {
+  const uint32_t i = *(const uint32_t *)a, j = *(const uint32_t *)b;
+  if (g_sort_deg[i] != g_sort_deg[j]) return g_sort_deg[i] < g_sort_deg[j] ? -1 : 1;
+  return (i > j) - (i < j);
+}
+
+void orc_outlier_scores(const int32_t *ids, const float *dists, uint32_t n, uint32_t k1,
+                        float *dist_sum, uint32_t *dist_rank, uint32_t *indegree,
+                        uint32_t *indegree_rank) {
+  uint32_t *order = (uint32_t *)malloc(sizeof(uint32_t) * (n ? n : 1));
+  for (uint32_t i = 0; i < n; i++) {
+    float s = 0.0f;
+    for (uint32_t e = 0; e < k1; e++) s += dists[(size_t)i * k1 + e];
+    dist_sum[i] = s;
+    indegree[i] = 0;
+  }
+  for (uint64_t t = 0; t < (uint64_t)n * k1; t++) {
+    const int32_t id = ids[t];
+    if (id >= 0 && (uint32_t)id < n) indegree[id]++;
+  }
+  for (uint32_t i = 0; i < n; i++) order[i] = i;
+  g_sort_sum = dist_sum;
+  qsort(order, n, sizeof(uint32_t), cmp_sum_desc);
+  for (uint32_t p = 0; p < n; p++) dist_rank[order[p]] = p;
+  for (uint32_t i = 0; i < n; i++) order[i] = i;
+  g_sort_deg = indegree;
+  qsort(order, n, sizeof(uint32_t), cmp_deg_asc);
+  for (uint32_t p = 0; p < n; p++) indegree_rank[order[p]] = p;
+  free(order);
+}

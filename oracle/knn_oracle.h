@@ -75,6 +75,14 @@ void orc_bruteforce_f32(const float *pts, uint32_t n, uint32_t d, uint32_t k,
                         const uint32_t *queries, uint32_t nq, int nthreads, int32_t *outIds,
                         float *outDists);
 double orc_checksum_f32(const float *dists, uint64_t rows, uint32_t k1);
+
+/* printOutlierScores, main.cu:952-1067: FP32 row sums of the distances in entry order and
+ * their rank by sum descending; in-degree of every point (its own row included) and its
+ * rank ascending.  Equal values are ranked by point id (the reference's std::sort leaves
+ * them in unspecified order). */
+void orc_outlier_scores(const int32_t *ids, const float *dists, uint32_t n, uint32_t k1,
+                        float *dist_sum, uint32_t *dist_rank, uint32_t *indegree,
+                        uint32_t *indegree_rank);
 int orc_knnjoin_subset_f32(const float *pts, uint32_t n, uint32_t d, uint32_t k, uint32_t nd,
                            double beta, int nthreads, const uint32_t *subset, uint32_t nsub,
                            int32_t *outIds, float *outDists, orc_stats *st);

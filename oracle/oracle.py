@@ -87,6 +87,8 @@ def _setup(L):
     L.orc_bruteforce_f32.argtypes = [vp, u32, u32, u32, vp, u32, i32, vp, vp]
     L.orc_checksum_f32.argtypes = [vp, u64, u32]
     L.orc_checksum_f32.restype = dbl
+    L.orc_outlier_scores.argtypes = [vp, vp, u32, u32, vp, vp, vp, vp]
+    L.orc_outlier_scores.restype = None
     L.orc_reorder_f64.argtypes = [vp, u32, u32, vp, vp]
     L.orc_estimate_eps_f64.argtypes = [vp, u32, u32, u32, dbl, vp, ctypes.POINTER(EpsInfo)]
     L.orc_estimate_eps_f64.restype = i32
@@ -225,6 +227,17 @@ def bruteforce(pts: np.ndarray, k: int, queries: np.ndarray | None = None,
 def checksum(dists: np.ndarray) -> float:
     dists = np.ascontiguousarray(dists, np.float32)
     return float(lib().orc_checksum_f32(_p(dists), dists.shape[0], dists.shape[1]))
+
+
+def outlier_scores(ids: np.ndarray, dists: np.ndarray):
+    """(dist_sum, dist_rank, indegree, indegree_rank) of an N x (K+1) table."""
+    ids = np.ascontiguousarray(ids, np.int32)
+    dists = np.ascontiguousarray(dists, np.float32)
+    n, k1 = ids.shape
+    ds = np.zeros(n, np.float32)
+    dr, dg, gr = (np.zeros(n, np.uint32) for _ in range(3))
+    lib().orc_outlier_scores(_p(ids), _p(dists), n, k1, _p(ds), _p(dr), _p(dg), _p(gr))
+    return ds, dr, dg, gr
 
 
 def join_pass(pts_r: np.ndarray, nd: int, k: int, eps: float, queries: np.ndarray,

@@ -120,3 +120,23 @@ def test_cli_learned_model_file(tmp_path, oracle):
     lines = (tmp_path / "KNN_out.txt").read_text().splitlines()[1:]
     got = np.array([[int(t) for t in ln.split(", ") if t] for ln in lines])
     assert np.array_equal(got, ids)
+
+
+@pytest.mark.gpu
+@needs_cli
+def test_cli_outlier_scores_file(tmp_path, oracle):
+    pts = synthetic.uniform_box(5000, seed=7)
+    path = tmp_path / "pts.csv"
+    np.savetxt(path, pts, delimiter=",", fmt="%.9g")
+    pts = parse_like_reference(path.read_text()).reshape(-1, 2)
+    r = run_cli([str(path), "2", "6", "--outlier-scores"], tmp_path)
+    assert r.returncode == 0, r.stdout + r.stderr
+    rc, ids, d, _ = oracle.knnjoin(pts, 6, nd=2, beta=0.05)
+    s, dr, deg, gr = oracle.outlier_scores(ids, d)
+    lines = (tmp_path / "KNN_outlier_scores.txt").read_text().splitlines()
+    assert lines[0].startswith("#data point (line is the point id), col0: mean distance")
+    rows = [ln.split(", ") for ln in lines[1:]]
+    assert len(rows) == len(pts)
+    assert np.allclose([float(x[0]) for x in rows], s.astype(np.float64) / 6.0, rtol=1e-5)
+    assert [int(x[1]) for x in rows] == list(dr) and [int(x[2]) for x in rows] == list(deg)
+    assert [int(x[3]) for x in rows] == list(gr)
