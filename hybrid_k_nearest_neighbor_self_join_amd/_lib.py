@@ -18,7 +18,7 @@ LIB_PATH = os.path.join(PKG, "libgpuknnselfjoin.so")
 KNN_OK, KNN_ERR_ARG, KNN_ERR_TOO_FEW, KNN_ERR_EPS_ZERO, KNN_ERR_NO_PROGRESS = 0, 1, 2, 3, 4
 KNN_ERR_HIP, KNN_ERR_NOMEM, KNN_ERR_NO_DEVICE, KNN_ERR_MODEL = 5, 6, 7, 8
 KNN_FLOAT32, KNN_FLOAT64 = 0, 1
-KNN_MODE_BETA, KNN_MODE_LEARNED = 0, 1
+KNN_MODE_BETA, KNN_MODE_LEARNED, KNN_MODE_BRUTE = 0, 1, 2
 
 STATUS_NAMES = {
     0: "KNN_OK", 1: "KNN_ERR_ARG", 2: "KNN_ERR_TOO_FEW", 3: "KNN_ERR_EPS_ZERO",
@@ -48,6 +48,7 @@ class KnnParams(ctypes.Structure):
         ("verbose", ctypes.c_int32),
         ("device_ptrs", ctypes.c_int32),
         ("stream", ctypes.c_void_p),
+        ("brute_fallback", ctypes.c_int32),
     ]
 
 
@@ -74,7 +75,7 @@ class KnnStats(ctypes.Structure):
         ("algo_bytes_max", ctypes.c_uint64),
         ("overflow_queries", ctypes.c_uint64),
         ("eps_bucket", ctypes.c_uint32),
-        ("reserved", ctypes.c_uint32),
+        ("brute_queries", ctypes.c_uint32),
     ]
 
     def as_dict(self) -> dict:
@@ -174,7 +175,8 @@ def default_params(**kw) -> KnnParams:
 
 # ------------------------------------------------------------------ convenience wrappers
 def knnjoin_ex(points: np.ndarray, k: int, *, num_indexed_dims: int = 0, beta: float = 0.05,
-               mode: int = KNN_MODE_BETA, reorder: int = 1, verbose: int = 0):
+               mode: int = KNN_MODE_BETA, reorder: int = 1, verbose: int = 0,
+               brute_fallback: int = -1):
     """Whole join through knnjoin_ex on host arrays.  Returns (ids[N,K+1], dists[N,K+1], stats)."""
     pts = np.ascontiguousarray(points)
     if pts.dtype not in (np.float32, np.float64):
@@ -184,7 +186,7 @@ def knnjoin_ex(points: np.ndarray, k: int, *, num_indexed_dims: int = 0, beta: f
     ids = np.full((n, k + 1), -1, np.int32)
     dists = np.zeros((n, k + 1), pts.dtype)
     prm = default_params(num_indexed_dims=num_indexed_dims, beta=beta, mode=mode,
-                         reorder=reorder, verbose=verbose)
+                         reorder=reorder, verbose=verbose, brute_fallback=brute_fallback)
     st = KnnStats()
     rc = lib().knnjoin_ex(p(pts), dt, n, d, k, ctypes.byref(prm), p(ids), p(dists),
                           ctypes.byref(st))

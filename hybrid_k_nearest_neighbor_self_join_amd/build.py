@@ -23,7 +23,7 @@ LIB = os.path.join(PKG, "libgpuknnselfjoin.so")
 CLI = os.path.join(PKG, "knnjoin_main")
 
 SOURCES = ["prims.hip", "estimator.hip", "join.hip", "range.hip", "learned.hip", "batched.hip",
-           "outlier.hip", "api.hip"]
+           "outlier.hip", "brute.hip", "api.hip"]
 ARCH = os.environ.get("KNN_OFFLOAD_ARCH", "gfx950")
 
 HIP_FLAGS = [

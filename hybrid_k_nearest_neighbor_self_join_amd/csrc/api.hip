@@ -46,7 +46,7 @@ enum Slot {
   S_PTS, S_X, S_XS, S_KEYS, S_KEYS_TMP, S_IDS, S_IDS_TMP, S_CELLKEYS, S_CELLSTART, S_FLAGS, S_POS,
   S_ACT, S_NTILES, S_TILEOFF, S_TILES, S_FAILED, S_ROWSUM, S_OUTIDS, S_OUTDISTS, S_SCRATCH,
   S_HEAP, S_SMALL, S_PART, S_CTR, S_CSUM, S_HOSTFLAGS, S_OVERFLOW, S_STAGE, S_DENSE, S_BFLAGS,
-  S_BPOS, S_BSTART, S_CELLOF, S_CELLMAX, S_FCNT, S_RADIUS, S_RHIST, S_MODEL, S_TRAIN, S_OCC, S_ACTQ, S_NSLOTS
+  S_BPOS, S_BSTART, S_CELLOF, S_CELLMAX, S_FCNT, S_RADIUS, S_RHIST, S_MODEL, S_TRAIN, S_OCC, S_ACTQ, S_BRUTE_PK, S_BRUTE_PI, S_BLIST, S_NSLOTS
 };
 
 struct Cache {
@@ -229,6 +229,7 @@ struct Pipeline {
   uint32_t *scratch = nullptr;  // u32 words
   size_t scratchWords = 0;
   bool verbose = false;
+  bool brute = false;  // brute-force fallback enabled
 
   template <typename V>
   V *buf(Slot sl, size_t count) { return reinterpret_cast<V *>(g_cache.get(sl, sizeof(V) * count)); }
@@ -635,6 +636,84 @@ template <typename T>
 static void run_learned(Pipeline<T> &P, const knnjoin_params &p, knnjoin_stats &stats,
                         uint8_t *failed, int32_t *dIds, T *dDists, Timer &tPh, Timer &tK);
 
+// ------------------------------------------------------------------ brute force
+// When the brute-force kernel takes over, from measurements on MI355X (2-D, float32): it
+// evaluates about 1.5e12 (query, point) pairs/s at K = 64 and 2e12-3e12 at K = 8 (less below
+// about 50k points, where a launch is latency-bound), while one more index build and range
+// pass costs about 0.25 ms + 0.3 ns per point (10M points: a 755-query tail took 5 ms more by
+// brute force than by the learned schedule's last pass).  With w = max(1, (K+1)/9):
+//   tail:  pending queries x N x w <= 7.5e8 + 1000 N  (10M points, K = 64: <= 149 queries)
+//   whole: N^2 w <= 2^30                              (N <= 32k at K = 8, 12k at K = 64)
+static bool brute_tail(uint64_t pending, uint64_t n, uint32_t k1) {
+  const double w = std::max(1.0, (double)k1 / 9.0);
+  return (double)pending * (double)n * w <= 7.5e8 + 1000.0 * (double)n;
+}
+static bool brute_whole(uint64_t n, uint32_t k1) {
+  const double w = std::max(1.0, (double)k1 / 9.0);
+  return (double)n * (double)n * w <= 1073741824.0;
+}
+
+// checkIndexSelectivity (main.cu:2687-2716): a grid of width eps prunes nothing when no
+// indexed dim has more than 5 cells (the 3 a query searches plus the 2 of the eps margins).
+template <typename T>
+static bool no_selectivity(const T *mn, const T *mx, uint32_t nd, T eps) {
+  if (!(eps > (T)0)) return false;
+  for (uint32_t j = 0; j < nd; j++) {
+    const T lo = mn[j] - eps, hi = mx[j] + eps;
+    const T c = std::ceil((hi - lo) / eps);
+    if (!(c <= (T)5)) return false;
+  }
+  return true;
+}
+
+// Exact rows of the queries with sel[q] != 0 (all when sel is null) by the brute-force kernel
+// (distanceTableNDBruteForce, GPU.cu:1903-2930); failed[q] is cleared for each.  Returns the
+// number of queries.
+template <typename T>
+static uint32_t run_brute(Pipeline<T> &P, const uint8_t *sel, uint8_t *failed, int32_t *dIds, T *dDists,
+                          double &ms, Timer &tK) {
+  hipStream_t s = P.s;
+  const uint32_t n = P.n;
+  const uint32_t *list = nullptr;
+  uint32_t nq = n;
+  if (sel) {
+    uint32_t *flags = P.template buf<uint32_t>(S_FLAGS, n + 1);
+    uint32_t *pos = P.template buf<uint32_t>(S_POS, n + 1);
+    uint32_t *l = P.template buf<uint32_t>(S_BLIST, n);
+    select_list(sel, n, flags, pos, l, P.scratch, s);
+    HIP_CHECK(hipMemcpyAsync(&nq, pos + n, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    list = l;
+  }
+  if (nq == 0) return 0;
+  const BrutePlan pl = brute_plan(n, nq, P.K1);
+  if (pl.qw == 0) throw Error(KNN_ERR_ARG, "brute force: K+1 must be <= 2048");
+  BruteArgs a{};
+  a.x = P.x;
+  a.n = n;
+  a.d = P.d;
+  a.k1 = P.K1;
+  a.queries = list;
+  a.nq = nq;
+  a.qw = pl.qw;
+  a.chunk = pl.chunk;
+  a.nchunks = pl.nchunks;
+  if (pl.partialEntries) {
+    a.partKey = P.template buf<double>(S_BRUTE_PK, pl.partialEntries);
+    a.partId = P.template buf<uint32_t>(S_BRUTE_PI, pl.partialEntries);
+  }
+  a.outIds = dIds;
+  a.outDists = dDists;
+  a.failed = failed;
+  tK.start();
+  brute_join<T>(a, s);
+  ms += tK.stop();
+  if (P.verbose)
+    printf("[knn] brute force: queries %u chunks %u x %u points, %u queries per wave\n", nq, pl.nchunks,
+           pl.chunk, pl.qw);
+  return nq;
+}
+
 template <typename T>
 static int run_join(const T *pts_in, uint32_t n, uint32_t d, uint32_t k, const knnjoin_params &p,
                     int32_t *outIds, T *outDists, knnjoin_stats *st) {
@@ -662,6 +741,13 @@ static int run_join(const T *pts_in, uint32_t n, uint32_t d, uint32_t k, const k
   P.k = k;
   P.K1 = K1;
   P.verbose = p.verbose != 0;
+  P.brute = p.brute_fallback < 0 ? p.mode == KNN_MODE_LEARNED : p.brute_fallback != 0;
+  if (p.mode != KNN_MODE_BETA && p.mode != KNN_MODE_LEARNED && p.mode != KNN_MODE_BRUTE)
+    throw Error(KNN_ERR_ARG, "unknown radius mode");
+  if ((p.mode == KNN_MODE_BRUTE || P.brute) && K1 > BRUTE_MAX_K1) {
+    if (p.mode == KNN_MODE_BRUTE) throw Error(KNN_ERR_ARG, "KNN_MODE_BRUTE: K+1 must be <= 2048");
+    P.brute = false;
+  }
   P.scratchWords = std::max(scan_scratch_words(n + 1), radix_scratch_words(n)) + 1024;
   P.scratch = P.template buf<uint32_t>(S_SCRATCH, P.scratchWords);
 
@@ -718,7 +804,17 @@ static int run_join(const T *pts_in, uint32_t n, uint32_t d, uint32_t k, const k
   stats.ms_reorder = tPh.stop();
   for (uint32_t j = 0; j < 16; j++) stats.perm[j] = j < d ? perm[j] : 0;
 
-  if (p.mode == KNN_MODE_LEARNED) {
+  if (p.mode == KNN_MODE_LEARNED && !radius_model(d, nd).ok)
+    throw Error(KNN_ERR_MODEL, "no learned-radius model registered for this number of indexed dims");
+  if (p.mode == KNN_MODE_BRUTE || (P.brute && brute_whole(n, K1))) {
+    // the whole join without the index
+    tPh.start();
+    double bms = 0;
+    stats.brute_queries = run_brute<T>(P, nullptr, nullptr, dIds, dDists, bms, tK);
+    stats.ms_join = tPh.stop();
+    stats.iterations = 1;
+    stats.kernel_launches = 1;
+  } else if (p.mode == KNN_MODE_LEARNED) {
     run_learned<T>(P, p, stats, failed, dIds, dDists, tPh, tK);
   } else {
     run_plain<T>(P, beta, stats, failed, dIds, dDists, tPh, tK);
@@ -832,6 +928,16 @@ static void run_plain(Pipeline<T> &P, double beta, knnjoin_stats &stats, uint8_t
     nq = nfail;
     numIter++;
     first = false;
+    if (nq != 0 && P.brute &&
+        (brute_tail(nq, n, K1) || no_selectivity<T>(hmn, hmx, nd, grow_eps(eps, eps0)))) {
+      tPh.start();
+      double bms = 0;
+      stats.brute_queries += run_brute<T>(P, failed, failed, dIds, dDists, bms, tK);
+      joinMs += tPh.stop();
+      launches++;
+      numIter++;
+      nq = 0;
+    }
   }
   stats.ms_join = joinMs;
   stats.ms_range_kernel = kMs;
@@ -925,6 +1031,17 @@ static void run_learned(Pipeline<T> &P, const knnjoin_params &p, knnjoin_stats &
     }
     if (pending == 0) break;
     if (passes > 200) throw Error(KNN_ERR_NO_PROGRESS, "learned schedule: queries still failing after 200 passes");
+    if (P.brute && (brute_tail(pending, n, K1) ||
+                    no_selectivity<T>(hmn, hmx, nd, (T)((double)wfF * std::exp2((c - 32) / 4.0))))) {
+      // few pending queries, or even the smallest pending radius (the lower edge of the lowest
+      // occupied bin) leaves the index nothing to prune: brute-force them together
+      tPh.start();
+      double bms = 0;
+      stats.brute_queries += run_brute<T>(P, failed, failed, dIds, dDists, bms, tK);
+      joinMs += tPh.stop();
+      passes++;
+      break;
+    }
     // octave classes (4 quarter-octave bins) with <= 2 indexed dims; single bins above, where
     // a cell up to 2x the radius would cost up to 2^nd in block volume
     const int cHi = std::min(c + (nd <= 2 ? 3 : 0), 63);
@@ -1052,6 +1169,7 @@ void knnjoin_default_params(knnjoin_params *p) {
   p->beta = 0.05;  // params.h:58
   p->mode = KNN_MODE_BETA;
   p->reorder = 1;  // params.h:35
+  p->brute_fallback = -1;
 }
 
 int knnjoin_ex(const void *points, int dtype, uint32_t n, uint32_t ndim, uint32_t k,

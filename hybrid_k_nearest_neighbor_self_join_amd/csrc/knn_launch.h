@@ -243,6 +243,36 @@ void outlier_scores(const int32_t *ids, const float *dists, uint32_t n, uint32_t
                     uint32_t *distRank, uint32_t *indeg, uint32_t *indegRank, uint32_t *keys,
                     uint32_t *vals, uint32_t *keysTmp, uint32_t *valsTmp, uint32_t *scratch, hipStream_t s);
 
+// ------------------------------------------------------------- brute force (brute.hip)
+// Exact top-(K+1) of a query list against every point, no index (the reference's
+// no-selectivity path, GPU.cu:1903-2930 / kernel.cu:902-975).  Rows are bit-identical to the
+// range kernels' rows for the same query.
+constexpr uint32_t BRUTE_MAX_K1 = 2048;
+struct BruteArgs {
+  const void *x;            // reordered dim-major points x[j*n + i]
+  uint32_t n, d, k1;
+  const uint32_t *queries;  // query ids, or null = 0..nq-1
+  uint32_t nq;
+  uint32_t qw, chunk, nchunks;  // from brute_plan
+  double *partKey;          // nq * nchunks * k1 entries each when nchunks > 1
+  uint32_t *partId;
+  int32_t *outIds;
+  void *outDists;
+  uint8_t *failed;          // optional: failed[q] = 0 for every row written
+};
+// qw == 0: not possible (K+1 above BRUTE_MAX_K1 or too many blocks)
+struct BrutePlan {
+  uint32_t qw, chunk, nchunks;
+  uint64_t partialEntries;
+};
+BrutePlan brute_plan(uint32_t n, uint32_t nq, uint32_t k1);
+// list[0..count) = ids i with sel[i] != 0, ascending; count in pos[n].  flags/pos: n + 1 words,
+// scratch: scan_scratch_words(n)
+void select_list(const uint8_t *sel, uint32_t n, uint32_t *flags, uint32_t *pos, uint32_t *list,
+                 uint32_t *scratch, hipStream_t s);
+template <typename T>
+void brute_join(const BruteArgs &a, hipStream_t s);
+
 // deterministic sum of rowSum over the rows (chunks of 4096 rows, then chunk sums): out[0]
 void sum_rows(const double *rowSum, uint32_t n, double *partial, double *out, hipStream_t s);
 

@@ -42,7 +42,9 @@ enum { KNN_FLOAT32 = 0, KNN_FLOAT64 = 1 };
 /* radius schedule */
 enum {
   KNN_MODE_BETA = 0,    /* plain method: BETA-driven eps0 + failed-query loop (main.cu:305-430) */
-  KNN_MODE_LEARNED = 1  /* learned per-query initial radius, same loop as fallback */
+  KNN_MODE_LEARNED = 1, /* learned per-query initial radius, same loop as fallback */
+  KNN_MODE_BRUTE = 2    /* no index: every query against every point (distanceTableNDBruteForce,
+                           GPU.cu:1903-2930); K+1 <= 2048 */
 };
 
 typedef struct {
@@ -53,6 +55,13 @@ typedef struct {
   int32_t verbose;           /* 1 = print phase timings to stdout */
   int32_t device_ptrs;       /* 1 = points/outputs are device pointers (HBM-resident) */
   void *stream;              /* hipStream_t for device_ptrs mode; NULL = library stream */
+  int32_t brute_fallback;    /* brute-force the queries the index cannot serve cheaply: when no
+                                indexed dim keeps more than 5 cells at the next radius
+                                (checkIndexSelectivity, main.cu:2687-2716), when the failed
+                                queries x N x w is at most 7.5e8 + 1000 N, or the whole join
+                                when N^2 w <= 2^30, w = max(1, (K+1)/9).  1 on, 0 off (the reference's
+                                loop), -1 (default) = on for KNN_MODE_LEARNED, off for
+                                KNN_MODE_BETA.  Rows are identical either way. */
 } knnjoin_params;
 
 typedef struct {
@@ -73,7 +82,7 @@ typedef struct {
   uint64_t algo_bytes_max;  /* algorithmic bytes of the longest range-kernel launch */
   uint64_t overflow_queries; /* queries re-run by the heap kernel (staging overflow) */
   uint32_t eps_bucket;      /* k-distance histogram bucket of eps0 (GPU.cu:3484-3527) */
-  uint32_t reserved;
+  uint32_t brute_queries;   /* queries whose rows the brute-force kernel wrote */
 } knnjoin_stats;
 
 /*

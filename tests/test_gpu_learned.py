@@ -71,7 +71,9 @@ def test_learned_schedule_exact_with_any_model(oracle, scale):
     # whatever the network predicts (constant, noise, wild), rows equal the oracle's
     pts = synthetic.mixed_box(120000, seed=21)
     _random_model(2, 99, scale)
-    ids, d, st = _lib.knnjoin_ex(pts, 32, mode=_lib.KNN_MODE_LEARNED)
+    # brute_fallback=0 keeps every query on the learned passes (the fallback is tested apart)
+    ids, d, st = _lib.knnjoin_ex(pts, 32, mode=_lib.KNN_MODE_LEARNED, brute_fallback=0)
+    assert st.brute_queries == 0
     rc, i2, d2, _ = oracle.knnjoin(pts, 32, nd=2, beta=0.025)
     assert rc == 0
     assert np.array_equal(ids, i2) and np.array_equal(bits(d), bits(d2))
@@ -80,7 +82,8 @@ def test_learned_schedule_exact_with_any_model(oracle, scale):
 def test_learned_schedule_exact_higher_dims(oracle):
     pts = synthetic.skewed_cube(50000, 4, seed=3)
     _random_model(4, 5, 0.5)
-    ids, d, st = _lib.knnjoin_ex(pts, 16, num_indexed_dims=4, mode=_lib.KNN_MODE_LEARNED)
+    ids, d, st = _lib.knnjoin_ex(pts, 16, num_indexed_dims=4, mode=_lib.KNN_MODE_LEARNED,
+                                 brute_fallback=0)
     rc, i2, d2, _ = oracle.knnjoin(pts, 16, nd=4, beta=0.025)
     assert np.array_equal(ids, i2) and np.array_equal(bits(d), bits(d2))
 
