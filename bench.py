@@ -192,6 +192,53 @@ def suite_batched(args):
     }), flush=True)
 
 
+def suite_brute(args):
+    """Brute-force path: whole joins by KNN_MODE_BRUTE against the grid schedule (2-D uniform,
+    k 8 and 64), and a dense cluster with 20 far outliers, where the plain loop needs hundreds
+    of index builds, with the brute-force fallback off and on."""
+    import torch  # noqa: F401
+
+    from hybrid_k_nearest_neighbor_self_join_amd import _lib, learned_radius, synthetic
+
+    learned_radius.register_default_models()
+
+    def best(pts, k, **kw):
+        _lib.knnjoin_ex(pts, k, **kw)
+        runs = [_lib.knnjoin_ex(pts, k, **kw) for _ in range(max(args.steps, 3))]
+        return min(runs, key=lambda r: r[2].ms_total)
+
+    for n in (20_000, 100_000, 300_000):
+        pts = synthetic.uniform_box(n, seed=0x5EED0200 + n)
+        for k in (8, 64):
+            ib, db, sb = best(pts, k, mode=_lib.KNN_MODE_BRUTE)
+            ip, dp, sp = best(pts, k, beta=0.025, brute_fallback=0)
+            print(json.dumps({
+                "suite": "brute", "case": "whole_join", "n": n, "d": 2, "k": k,
+                "brute_ms": sb.ms_total, "brute_kernel_ms": sb.ms_join,
+                "pairs_per_s": n * n / (sb.ms_join / 1e3), "plain_ms": sp.ms_total,
+                "identical_rows": bool(np.array_equal(ib, ip) and np.array_equal(db.view(np.uint32), dp.view(np.uint32))),
+            }), flush=True)
+    rng = np.random.default_rng(0x5EED0300)
+    pts = np.concatenate([rng.normal(0.0, 1.0, (60_000, 2)), rng.uniform(30.0, 40.0, (20, 2))]).astype(np.float32)
+    res = {}
+    for tag, kw in [("plain_no_fallback", dict(beta=0.025, brute_fallback=0)),
+                    ("plain_fallback", dict(beta=0.025, brute_fallback=1)),
+                    ("learned_no_fallback", dict(mode=_lib.KNN_MODE_LEARNED, brute_fallback=0)),
+                    ("learned_fallback", dict(mode=_lib.KNN_MODE_LEARNED, brute_fallback=1))]:
+        if "learned" in tag and not learned_radius.has_model(learned_radius.register_default_models(), 2, 2):
+            continue
+        i, d, st = best(pts, 16, **kw)
+        res[tag] = (i, d, st)
+    base = res["plain_no_fallback"]
+    print(json.dumps({
+        "suite": "brute", "case": "cluster_60000_plus_20_far_outliers", "k": 16,
+        **{t: {"ms": r[2].ms_total, "iterations": r[2].iterations, "reindexes": r[2].reindexes,
+               "brute_queries": r[2].brute_queries} for t, r in res.items()},
+        "identical_rows": all(np.array_equal(r[0], base[0]) and np.array_equal(r[1].view(np.uint32), base[1].view(np.uint32))
+                              for r in res.values()),
+    }), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -205,8 +252,9 @@ def main():
                          "ships with the package, the learned schedule; value = the faster")
     ap.add_argument("--cpu-sample", type=int, default=1_000_000)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--suite", choices=["highdim", "batched"], default=None,
-                    help="instead of the headline line: the 4-D/6-D cases or the 256-set batch")
+    ap.add_argument("--suite", choices=["highdim", "batched", "brute"], default=None,
+                    help="instead of the headline line: the 4-D/6-D cases, the 256-set batch, or "
+                         "the brute-force path")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 5 device copies of 1 GiB (the reference kernel for scaling "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE readings to bytes)")
@@ -227,6 +275,8 @@ def main():
         return suite_highdim(args)
     if args.suite == "batched":
         return suite_batched(args)
+    if args.suite == "brute":
+        return suite_brute(args)
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (this build has no multi-GPU path); "
               f"--gpus {args.gpus} is not supported", file=sys.stderr)
