@@ -439,6 +439,9 @@ __device__ __forceinline__ void for_each_point_flat(const JoinArgs &a, const G &
 // ====================================================================== select kernel
 constexpr int NB = 64;
 constexpr uint32_t SELECT_MAX_GRID = 2048;
+// pass 2: sub-blocks of at most RMAX rows have their row ranges looked up for the whole tile
+// at once (lane = query) and their candidates loaded PR rounds of 64 at a time
+constexpr uint32_t RMAX = 6;
 }  // namespace
 template <typename T>
 size_t range_select_lds_bytes(uint32_t smax);
@@ -452,8 +455,10 @@ __device__ __forceinline__ uint32_t bucket_of(T s, T scale) {
 }
 
 // LDS: hist[NB/2][64] u32, two 16-bit bucket counters per word (per-lane histogram
-// columns, then slot offsets), the collecting query's cursors qh[NB/2], then the per-query
-// staging arrays key[SM] (double), id[SM] (u32), s[SM] (T).  A lane with 65536 or more candidates inside F goes to the heap
+// columns, then slot offsets), the collecting query's cursors qh[NB/2], rowB/rowX[64], the
+// tile's pass-2 row table prB/prX[RMAX][64] (first sorted position and exclusive offset of
+// each row of each query's sub-block), prT/prR[64] (candidates and rows per query), then the
+// per-query staging arrays key[SM] (double), id[SM] (u32), s[SM] (T).  A lane with 65536 or more candidates inside F goes to the heap
 // kernel, so no 16-bit counter or cursor can carry into its neighbour.
 __device__ __forceinline__ uint32_t hword(uint32_t b, uint32_t col) { return (b >> 1) * 64 + col; }
 __device__ __forceinline__ uint32_t hshift(uint32_t b) { return (b & 1u) << 4; }
@@ -467,7 +472,11 @@ __global__ __launch_bounds__(64) void k_range_select(JoinArgs a, G g0) {
   uint32_t *qh = hist + (NB / 2) * 64;
   uint32_t *rowB = qh + NB / 2;
   uint32_t *rowX = rowB + 64;
-  double *skey = reinterpret_cast<double *>(smem + ((NB / 2) * 64 + NB / 2 + 128) * sizeof(uint32_t));
+  uint32_t *prB = rowX + 64;
+  uint32_t *prX = prB + RMAX * 64;
+  uint32_t *prT = prX + RMAX * 64;
+  uint32_t *prR = prT + 64;
+  double *skey = reinterpret_cast<double *>(smem + ((NB / 2) * 64 + NB / 2 + 128 + RMAX * 128 + 128) * sizeof(uint32_t));
   uint32_t *sid = reinterpret_cast<uint32_t *>(skey + SM);
   T *ss = reinterpret_cast<T *>(sid + SM + (SM & 1));
   const uint32_t lane = threadIdx.x;
@@ -553,6 +562,65 @@ __global__ __launch_bounds__(64) void k_range_select(JoinArgs a, G g0) {
     const uint64_t goMask = __ballot(go);
     wave_lds_sync();
 
+    // ---- pass 2 set-up (lane = query): the rows of each query's collect sub-block (radius
+    // sqrt(edge), the same bounds the per-query path computes) and their point ranges, for
+    // the whole tile at once: one memory round trip per tile instead of two per query.
+    // Queries with more than RMAX rows, or grids without the dense start table, take the
+    // per-query path.
+    bool fastL = false;
+    if (go && a.denseStart) {
+      const T riL = (T)next_up((T)sqrt((double)edge * (1.0 + 0x1p-16)));
+      uint32_t sl[KNN_MAX_INDEXED], sh[KNN_MAX_INDEXED];
+      uint32_t R = 1;
+      for (uint32_t j = 0; j < g.nd; j++) {
+        T xj = L.qc[0];
+#pragma unroll
+        for (int jj = 1; jj < D; jj++)
+          if ((uint32_t)jj == j) xj = L.qc[jj];
+        sl[j] = cell_coord<T>(next_down(xj - riL), (T)g.minArr[j], eps, g.nCells[j]);
+        sh[j] = cell_coord<T>(next_up(xj + riL), (T)g.minArr[j], eps, g.nCells[j]);
+        if (j >= 1) {
+          const uint32_t sp = sh[j] - sl[j] + 1;
+          R = (R <= RMAX && sp <= RMAX) ? R * sp : RMAX + 1;
+        }
+      }
+      if (R <= RMAX) {
+        fastL = true;
+        uint32_t cur[KNN_MAX_INDEXED];
+        for (uint32_t j = 1; j < g.nd; j++) cur[j] = sl[j];
+        const double ri2L = (double)riL * (double)riL * (1.0 + 0x1p-15);
+        uint32_t acc = 0;
+#pragma unroll
+        for (uint32_t r = 0; r < RMAX; r++) {
+          if (r < R) {
+            uint64_t rowBase = g.keyBase;
+            for (uint32_t j = 1; j < g.nd; j++) rowBase += (uint64_t)cur[j] * g.mult[j];
+            uint32_t l0 = sl[0], h0 = sh[0];
+            const bool keep = g.nd >= 3 ? row_chord_lane<T, D, G>(g, cur, L.qd, ri2L, l0, h0) : true;
+            uint32_t pB = 0, len = 0;
+            if (keep) {
+              pB = a.denseStart[rowBase + l0];
+              len = a.denseStart[rowBase + h0 + 1] - pB;
+            }
+            prB[r * 64 + lane] = pB;
+            prX[r * 64 + lane] = acc;
+            acc += len;
+            for (uint32_t j = 1; j < g.nd; j++) {  // next row: dims 1.. as an odometer
+              if (cur[j] < sh[j]) {
+                cur[j]++;
+                break;
+              }
+              cur[j] = sl[j];
+            }
+          }
+        }
+        prT[lane] = acc;
+        prR[lane] = R;
+      }
+    }
+    const uint64_t fastMask = __ballot(fastL);
+    wave_lds_sync();
+
     // ---- pass 2 (wave = query): for each query, scan its own sub-block (radius sqrt(edge)),
     // stage the candidates of buckets <= b* (plus the margin) in LDS in bucket order, rank
     // them exactly, write the row coalesced
@@ -571,26 +639,10 @@ __global__ __launch_bounds__(64) void k_range_select(JoinArgs a, G g0) {
       const double Tqi = readlane<double>(L.Tq, (int)i);
       const uint32_t bsi = (uint32_t)__builtin_amdgcn_readlane((int)bstar, (int)i);
       const uint32_t qid = (uint32_t)__builtin_amdgcn_readlane((int)L.q, (int)i);
-      // every staged candidate has key <= edge (1 + 2^-19): per-dim reach sqrt of that
-      const T ri = (T)next_up((T)sqrt((double)edi * (1.0 + 0x1p-16)));
-      uint32_t slo[KNN_MAX_INDEXED], shi[KNN_MAX_INDEXED];
-      for (uint32_t j = 0; j < g.nd; j++) {
-        T xj = qi[0];
-#pragma unroll
-        for (int jj = 1; jj < D; jj++)
-          if ((uint32_t)jj == j) xj = qi[jj];
-        slo[j] = cell_coord<T>(next_down(xj - ri), (T)g.minArr[j], eps, g.nCells[j]);
-        shi[j] = cell_coord<T>(next_up(xj + ri), (T)g.minArr[j], eps, g.nCells[j]);
-      }
       if (lane < (uint32_t)(NB / 2)) qh[lane] = hist[lane * 64 + i];
       wave_lds_sync();
       bool ovf = false;
-      const double ri2 = (double)ri * (double)ri * (1.0 + 0x1p-15);
-      for_each_point_flat<T, D, G>(a, g, slo, shi, g.nd >= 3, qdi, ri2, lane, rowB, rowX, [&](uint32_t p) {
-        T c[D];
-#pragma unroll
-        for (int j = 0; j < D; j++) c[j] = xs[(size_t)j * n + p];
-        const uint32_t cid = a.sortedIds[p];  // with the coordinates: one memory round trip
+      auto stage = [&](const T *c, uint32_t cid) {
         const T s = Num<T>::template sq<D>(c, qi);
         if (s <= Fi) {
           uint32_t b = bucket_of<T>(s, sci);
@@ -606,7 +658,64 @@ __global__ __launch_bounds__(64) void k_range_select(JoinArgs a, G g0) {
             }
           }
         }
-      });
+      };
+      if ((fastMask >> i) & 1ull) {
+        // rows from the tile's table; PR rounds of candidates loaded before any is staged
+        constexpr int PR = D <= 4 ? 4 : (D <= 8 ? 2 : 1);
+        const uint32_t Ri = prR[i], tot = prT[i];
+        uint32_t X[RMAX], Bs[RMAX];
+#pragma unroll
+        for (uint32_t k = 0; k < RMAX; k++) {
+          X[k] = prX[k * 64 + i];
+          Bs[k] = prB[k * 64 + i];
+        }
+        for (uint32_t f0 = 0; f0 < tot; f0 += 64u * PR) {
+          T cv[PR][D];
+          uint32_t cidv[PR];
+          bool val[PR];
+#pragma unroll
+          for (int u = 0; u < PR; u++) {
+            const uint32_t fi = f0 + 64u * u + lane;
+            val[u] = fi < tot;
+            if (val[u]) {
+              uint32_t b = Bs[0], x = X[0];
+#pragma unroll
+              for (uint32_t k = 1; k < RMAX; k++)
+                if (k < Ri && fi >= X[k]) {
+                  b = Bs[k];
+                  x = X[k];
+                }
+              const uint32_t p = b + (fi - x);
+#pragma unroll
+              for (int j = 0; j < D; j++) cv[u][j] = xs[(size_t)j * n + p];
+              cidv[u] = a.sortedIds[p];
+            }
+          }
+#pragma unroll
+          for (int u = 0; u < PR; u++)
+            if (val[u]) stage(cv[u], cidv[u]);
+        }
+      } else {
+        // every staged candidate has key <= edge (1 + 2^-19): per-dim reach sqrt of that
+        const T ri = (T)next_up((T)sqrt((double)edi * (1.0 + 0x1p-16)));
+        uint32_t slo[KNN_MAX_INDEXED], shi[KNN_MAX_INDEXED];
+        for (uint32_t j = 0; j < g.nd; j++) {
+          T xj = qi[0];
+#pragma unroll
+          for (int jj = 1; jj < D; jj++)
+            if ((uint32_t)jj == j) xj = qi[jj];
+          slo[j] = cell_coord<T>(next_down(xj - ri), (T)g.minArr[j], eps, g.nCells[j]);
+          shi[j] = cell_coord<T>(next_up(xj + ri), (T)g.minArr[j], eps, g.nCells[j]);
+        }
+        const double ri2 = (double)ri * (double)ri * (1.0 + 0x1p-15);
+        for_each_point_flat<T, D, G>(a, g, slo, shi, g.nd >= 3, qdi, ri2, lane, rowB, rowX, [&](uint32_t p) {
+          T c[D];
+#pragma unroll
+          for (int j = 0; j < D; j++) c[j] = xs[(size_t)j * n + p];
+          const uint32_t cid = a.sortedIds[p];  // with the coordinates: one memory round trip
+          stage(c, cid);
+        });
+      }
       wave_lds_sync();
       const uint32_t m = (qh[bsi >> 1] >> hshift(bsi)) & 0xffffu;
       if (__ballot(ovf) || m > SM) {
@@ -866,7 +975,7 @@ uint32_t range_select_grid(uint32_t ntiles) {
 
 template <typename T>
 size_t range_select_lds_bytes(uint32_t smax) {
-  return ((size_t)(NB / 2) * 64 + NB / 2 + 128) * sizeof(uint32_t) + (size_t)smax * 8 +
+  return ((size_t)(NB / 2) * 64 + NB / 2 + 128 + RMAX * 128 + 128) * sizeof(uint32_t) + (size_t)smax * 8 +
          (size_t)(smax + (smax & 1)) * 4 + (size_t)smax * sizeof(T);
 }
 
