@@ -230,6 +230,8 @@ struct Pipeline {
   size_t scratchWords = 0;
   bool verbose = false;
   bool brute = false;  // brute-force fallback enabled
+  const T *rowMajor = nullptr;  // the row-major input, when x is its column permutation rowPerm
+  DimPerm rowPerm{};
 
   template <typename V>
   V *buf(Slot sl, size_t count) { return reinterpret_cast<V *>(g_cache.get(sl, sizeof(V) * count)); }
@@ -275,7 +277,10 @@ struct Pipeline {
     }
     sortedIds = ids;
     xs = buf<T>(S_XS, (size_t)n * d);
-    gather_soa<T>(x, sortedIds, n, d, xs, s);
+    if (rowMajor)
+      gather_rows<T>(rowMajor, sortedIds, n, d, rowPerm, xs, s);
+    else
+      gather_soa<T>(x, sortedIds, n, d, xs, s);
     sync();
     // dense cell-start table: O(1) row lookups when the grid is small enough
     denseStart = nullptr;
@@ -801,6 +806,8 @@ static int run_join(const T *pts_in, uint32_t n, uint32_t d, uint32_t k, const k
   }
   P.x = P.template buf<T>(S_X, (size_t)n * d);
   permute_to_soa<T>(dpts, n, d, perm, P.x, s);
+  P.rowMajor = dpts;
+  for (uint32_t j = 0; j < KNN_MAXDIM; j++) P.rowPerm.v[j] = j < d ? perm[j] : 0u;
   stats.ms_reorder = tPh.stop();
   for (uint32_t j = 0; j < 16; j++) stats.perm[j] = j < d ? perm[j] : 0;
 

@@ -53,6 +53,17 @@ __global__ void k_gather_soa(const T *__restrict__ x, const uint32_t *__restrict
   for (uint32_t j = 0; j < d; j++) xs[(size_t)j * n + p] = x[(size_t)j * n + i];
 }
 
+// xs[j*n + p] = in[ids[p]*d + perm[j]]: the same values as k_gather_soa from the permuted
+// SoA copy, read from the row-major input, whose d coordinates of a point share a cache line
+template <typename T>
+__global__ void k_gather_rows(const T *__restrict__ in, const uint32_t *__restrict__ ids, uint32_t n,
+                              uint32_t d, DimPerm perm, T *__restrict__ xs) {
+  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const T *row = in + (size_t)ids[p] * d;
+  for (uint32_t j = 0; j < d; j++) xs[(size_t)j * n + p] = row[perm.v[j]];
+}
+
 __global__ void k_dense_counts(const uint64_t *cellKeys, const uint32_t *cellStart,
                                uint32_t ncell, uint32_t *dense) {
   const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
@@ -295,6 +306,11 @@ template <typename T>
 void gather_soa(const T *x, const uint32_t *ids, uint32_t n, uint32_t d, T *xs, hipStream_t s) {
   if (n) k_gather_soa<T><<<(n + 255) / 256, 256, 0, s>>>(x, ids, n, d, xs);
 }
+template <typename T>
+void gather_rows(const T *in, const uint32_t *ids, uint32_t n, uint32_t d, const DimPerm &perm, T *xs,
+                 hipStream_t s) {
+  if (n) k_gather_rows<T><<<(n + 255) / 256, 256, 0, s>>>(in, ids, n, d, perm, xs);
+}
 void dense_cell_counts(const uint64_t *cellKeys, const uint32_t *cellStart, uint32_t ncell,
                        uint32_t *dense, hipStream_t s) {
   if (ncell) k_dense_counts<<<(ncell + 255) / 256, 256, 0, s>>>(cellKeys, cellStart, ncell, dense);
@@ -368,6 +384,8 @@ void sum_rows(const double *rowSum, uint32_t n, double *partial, double *out, hi
   template void cell_keys_u64<T>(const T *, uint32_t, const GridOf<T>::type &, uint64_t *,      \
                                  uint32_t *, hipStream_t);                                     \
   template void gather_soa<T>(const T *, const uint32_t *, uint32_t, uint32_t, T *, hipStream_t); \
+  template void gather_rows<T>(const T *, const uint32_t *, uint32_t, uint32_t, const DimPerm &, T *, \
+                               hipStream_t);                                                   \
   template void row_sums<T>(const T *, uint32_t, uint32_t, double *, hipStream_t);          \
   template void prefilter<T>(const T *, uint32_t, const GridOf<T>::type &, const T *,         \
                              const uint32_t *, const uint32_t *, const uint8_t *, uint8_t *,  \

@@ -78,6 +78,13 @@ void compact_cells_u64(const uint64_t *keys, const uint32_t *flags, const uint32
 // xs[j*n + p] = x[j*n + ids[p]]
 template <typename T>
 void gather_soa(const T *x, const uint32_t *ids, uint32_t n, uint32_t d, T *xs, hipStream_t s);
+// xs[j*n + p] = in[ids[p]*d + perm[j]] from the row-major input (one cache line per point)
+struct DimPerm {
+  uint32_t v[KNN_MAXDIM];
+};
+template <typename T>
+void gather_rows(const T *in, const uint32_t *ids, uint32_t n, uint32_t d, const DimPerm &perm, T *xs,
+                 hipStream_t s);
 
 // Dense cell-start table (totalCells + 1 entries): denseStart[id] = first sorted position of
 // linear cell id (number of points in cells < id).  counts are scattered from the compacted
