@@ -68,6 +68,34 @@ __device__ __forceinline__ float mlp_forward(const float *W, const float *f, flo
   return y;
 }
 
+// mlp_forward with W2 stored transposed (W2t[j][i] = W2[i][j]): the 64 weights a layer-1
+// unit feeds are contiguous, so they load 4 at a time.  Same operations in the same order.
+__device__ __forceinline__ float mlp_forward_t(const float *W, const float *f) {
+  const float *W1 = W, *b1 = W + 768, *W2t = W + 832, *b2 = W + 4928, *w3 = W + 4992;
+  float acc[MLP_H];
+#pragma unroll
+  for (int i = 0; i < MLP_H; i++) acc[i] = b2[i];
+  for (int j = 0; j < MLP_H; j++) {
+    float a = b1[j];
+#pragma unroll
+    for (int t = 0; t < MLP_IN; t++) a = fmaf(W1[j * MLP_IN + t], f[t], a);
+    const float h = relu(a);
+    const float4 *w = reinterpret_cast<const float4 *>(W2t + j * MLP_H);
+#pragma unroll
+    for (int i4 = 0; i4 < MLP_H / 4; i4++) {
+      const float4 v = w[i4];
+      acc[4 * i4 + 0] = fmaf(v.x, h, acc[4 * i4 + 0]);
+      acc[4 * i4 + 1] = fmaf(v.y, h, acc[4 * i4 + 1]);
+      acc[4 * i4 + 2] = fmaf(v.z, h, acc[4 * i4 + 2]);
+      acc[4 * i4 + 3] = fmaf(v.w, h, acc[4 * i4 + 3]);
+    }
+  }
+  float y = W[5056];
+#pragma unroll
+  for (int i = 0; i < MLP_H; i++) y = fmaf(w3[i], relu(acc[i]), y);
+  return y;
+}
+
 template <typename T, typename G>
 __global__ void k_feature_counts(const T *__restrict__ x, uint32_t n, G g, uint32_t *cnt) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -122,9 +150,17 @@ __global__ __launch_bounds__(256) void k_radius(const T *__restrict__ x, uint32_
                                                 float logdens, const float *__restrict__ Wg,
                                                 float margin, float wf, T *radius,
                                                 float *featOut, uint32_t *rhist) {
-  __shared__ float W[MLP_NW];
+  __shared__ __align__(16) float W[MLP_NW + 3];
   __shared__ uint32_t lh[64];
-  for (uint32_t i = threadIdx.x; i < MLP_NW; i += blockDim.x) W[i] = Wg ? Wg[i] : 0.0f;
+  // W2 (64 x 64 at 832) stored transposed for mlp_forward_t
+  for (uint32_t i = threadIdx.x; i < MLP_NW; i += blockDim.x) {
+    uint32_t dst = i;
+    if (i >= 832 && i < 4928) {
+      const uint32_t r = (i - 832) / MLP_H, c = (i - 832) % MLP_H;
+      dst = 832 + c * MLP_H + r;
+    }
+    W[dst] = Wg ? Wg[i] : 0.0f;
+  }
   if (threadIdx.x < 64) lh[threadIdx.x] = 0;
   __syncthreads();
   const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
@@ -134,7 +170,7 @@ __global__ __launch_bounds__(256) void k_radius(const T *__restrict__ x, uint32_
     if (featOut)
       for (int j = 0; j < MLP_IN; j++) featOut[(size_t)q * MLP_IN + j] = f[j];
     if (radius) {
-      const float y = mlp_forward(W, f, nullptr, nullptr);
+      const float y = mlp_forward_t(W, f);
       // r = wf e^(y + margin), the exponent kept within [-8, 14] (about wf/3000 .. 1.2e6 wf)
       // so that any weights give a finite radius the expansion loop can start from
       float e = y + margin;

@@ -61,6 +61,20 @@ def test_features_match_oracle(oracle, gen, nd, k):
     assert np.array_equal(np.round(np.expm1(f[:, :10])), np.round(np.expm1(f2[:, :10])))
 
 
+def test_radius_inference_matches_oracle_mlp(oracle):
+    # the join's radius kernel (W2 held transposed in LDS) against the oracle's MLP on the same
+    # features: same fmaf order, so y agrees bitwise; exp may differ in the last place
+    pts = synthetic.mixed_box(50000, seed=12)
+    w = learned_radius.init_weights(17) * 0.5
+    margin = 0.25
+    _lib.set_radius_model(2, w, margin)
+    f, wf, r = _lib.radius_features(pts, 16, nd=2, want_radius=True)
+    y = oracle.mlp_forward(w, f)
+    e = np.clip(y + np.float32(margin), np.float32(-8.0), np.float32(14.0))  # float32, as on the GPU
+    want = np.float32(wf) * np.exp(e.astype(np.float64)).astype(np.float32)
+    assert np.allclose(r, want, rtol=4e-7, atol=0)
+
+
 def _random_model(nd, seed, scale):
     w = learned_radius.init_weights(seed) * scale
     _lib.set_radius_model(nd, w, 0.0)
