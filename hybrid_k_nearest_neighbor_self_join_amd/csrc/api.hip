@@ -453,6 +453,20 @@ struct Timer {
   }
 };
 
+// Range kernel of a pass: the collect kernel (one pass over each query's own block) with 5 or
+// more indexed dims, else the two-pass select kernel, whose first pass reads each candidate
+// once for all queries of a tile.  Measured on MI355X, 5M points, K = 32 (learned / plain
+// schedule, ms): 6 indexed dims, uniform 225 / 264 against 317 / 302, skewed 339 / 409
+// against 567 / 582; 4 indexed dims, uniform 63 / 98 against 59 / 66; 2 of 4 dims indexed
+// 2475 against 1504 (learned): queries share most of their candidates with the rest of their
+// tile there.  KNN_RANGE_KERNEL=select|collect overrides the choice for tuning.
+static bool use_collect(uint32_t nd, uint32_t k1) {
+  const char *v = std::getenv("KNN_RANGE_KERNEL");  // read per pass: tests switch it in-process
+  const int env = !v ? -1 : (std::strcmp(v, "collect") == 0 ? 1 : (std::strcmp(v, "select") == 0 ? 0 : -1));
+  if (k1 > BRUTE_MAX_K1) return false;
+  return env >= 0 ? env == 1 : nd >= 5;
+}
+
 // Builds tiles of <= 64 active queries per cell: activeFailed[q] != 0 selects q (all when
 // null).  Returns the tile count; act/tiles point into the cache.
 template <typename T>
@@ -570,6 +584,14 @@ PassResult Pipeline<T>::pass(const uint8_t *activeFailed, uint32_t nq, uint8_t *
   a.overflowCount = overflowCount;
   PassResult r{};
   r.ntiles = NT;
+  if (use_collect(nd, K1)) {
+    tK.start();
+    range_collect<T>(a, g, s);
+    r.kernel_ms = tK.stop();
+    r.noverflow = 0;
+    finish(r, counters, ctr, prefailed);
+    return r;
+  }
   if (!(a.smax > K1 && range_select_lds_bytes<T>(a.smax) <= 65536)) {
     // K+1 too large for LDS staging: the heap kernel takes every query of the pass
     size_t hs = range_heap_scratch_bytes<T>(NT, K1);
@@ -961,7 +983,7 @@ static void run_plain(Pipeline<T> &P, double beta, knnjoin_stats &stats, uint8_t
   stats.kernel_launches = launches;
 }
 
-// Cells per radius-class upper bound R in the learned schedule's grids: 2 with <= 2 indexed
+// Cells per radius-class upper bound R in the learned schedule's grids: 3 with <= 2 indexed
 // dims, 1 above; KNN_LEARNED_GRID_DIV (1..8) overrides it for tuning.
 static double learned_grid_div(uint32_t nd) {
   static const double env = []() {
@@ -969,7 +991,21 @@ static double learned_grid_div(uint32_t nd) {
     const double x = v ? std::atof(v) : 0.0;
     return (x >= 1.0 && x <= 8.0) ? x : 0.0;
   }();
-  return env > 0.0 ? env : (nd <= 2 ? 2.0 : 1.0);
+  return env > 0.0 ? env : (nd <= 2 ? 3.0 : 1.0);
+}
+
+// Quarter-octave bins per radius class: 8 (two octaves) with <= 2 indexed dims, 1 above.
+// Measured on MI355X, 2-D mixed/uniform sets of 1M-10M points, K = 8..128: two-octave classes
+// on cells of R/3 beat one-octave classes on cells of R/2 everywhere (10M mixed, K = 64:
+// 41.2 against 47.3 ms; K = 16: 44.0 against 61.5 ms), since each class costs an index
+// build over all N points.  KNN_LEARNED_CLASS_BINS (1..16) overrides it for tuning.
+static int learned_class_bins(uint32_t nd) {
+  static const int env = []() {
+    const char *v = std::getenv("KNN_LEARNED_CLASS_BINS");
+    const int x = v ? std::atoi(v) : 0;
+    return (x >= 1 && x <= 16) ? x : 0;
+  }();
+  return env > 0 ? env : (nd <= 2 ? 8 : 1);
 }
 
 // The learned schedule: per-query initial radius from the MLP, then passes over radius
@@ -1007,11 +1043,11 @@ static void run_learned(Pipeline<T> &P, const knnjoin_params &p, knnjoin_stats &
   stats.ms_eps = tPh.stop();
   P.radius = radius;
 
-  // ---- radius classes: the pending queries whose radius lies in the lowest occupied octave
-  // [R/2, R) are joined on a grid matched to R (cell width R/2 for <= 2 indexed dims: a
-  // query's block then spans at most 5 cells per dim and about 2.4x the area of its ball;
-  // R above); queries that fail double their radius, which moves them to a later octave.
-  // One index build per occupied octave.
+  // ---- radius classes: the pending queries whose radius lies in the lowest occupied class
+  // of bins [c, c + learned_class_bins) (two octaves [R/4, R) with <= 2 indexed dims, a
+  // quarter octave above) are joined on a grid matched to R (cell width R / learned_grid_div);
+  // queries that fail grow their radius (x2, or x2^(1/4) with 3 or more indexed dims), which
+  // moves them to a later class.  One index build per occupied class.
   unsigned long long *ctr = P.template buf<unsigned long long>(S_CTR, 64);
   fill_u32(reinterpret_cast<uint32_t *>(ctr), 0u, 16, s);
   uint64_t *maxr = reinterpret_cast<uint64_t *>(ctr + 8);
@@ -1049,9 +1085,9 @@ static void run_learned(Pipeline<T> &P, const knnjoin_params &p, knnjoin_stats &
       passes++;
       break;
     }
-    // octave classes (4 quarter-octave bins) with <= 2 indexed dims; single bins above, where
-    // a cell up to 2x the radius would cost up to 2^nd in block volume
-    const int cHi = std::min(c + (nd <= 2 ? 3 : 0), 63);
+    // wide classes with <= 2 indexed dims; single bins above, where a cell up to 2x the radius
+    // would cost up to 2^nd in block volume
+    const int cHi = std::min(c + learned_class_bins(nd) - 1, 63);
     uint32_t nq = 0;
     for (int b = c; b <= cHi; b++) nq += h[b];
     double R;
@@ -1073,8 +1109,8 @@ static void run_learned(Pipeline<T> &P, const knnjoin_params &p, knnjoin_stats &
       }
       R = R * (1.0 + 1e-6);
     }
-    // cell width R/2 (blocks of <= 5 cells per dim) with <= 2 indexed dims; above, R (<= 3 cells
-    // per dim), since every extra cell of reach multiplies the rows of a block
+    // cell width R/3 with <= 2 indexed dims; above, R (<= 3 cells per dim), since every extra
+    // cell of reach multiplies the rows of a block
     T w = (T)(R / learned_grid_div(nd));
     if (!(w > (T)0)) w = F.wf;
     for (;;) {  // keep the grid within 2^30 cells

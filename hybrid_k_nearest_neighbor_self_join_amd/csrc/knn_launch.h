@@ -158,6 +158,11 @@ void range_select(const JoinArgs &a, const typename GridOf<T>::type &g, hipStrea
 template <typename T>
 size_t range_select_lds_bytes(uint32_t smax);
 uint32_t range_select_grid(uint32_t ntiles);
+// Collect kernel (wave = query, one pass over its block into an LDS top-(K+1) collector): for
+// queries with many candidates per neighbour.  K+1 <= BRUTE_MAX_K1.
+template <typename T>
+void range_collect(const JoinArgs &a, const typename GridOf<T>::type &g, hipStream_t s);
+size_t range_collect_lds_bytes(uint32_t k1);
 // Heap kernel (per-lane max-heap on (key, id)): handles any input, used for overflow queries.
 template <typename T>
 void range_topk(const JoinArgs &a, const typename GridOf<T>::type &g, hipStream_t s);

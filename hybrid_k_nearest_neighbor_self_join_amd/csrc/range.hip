@@ -34,6 +34,7 @@
 // k_range_heap: per-lane max-heap of (key, id), in LDS when K+1 fits, else in global scratch.
 #include <type_traits>
 
+#include "knn_collect.h"
 #include "knn_launch.h"
 
 namespace knn {
@@ -783,6 +784,177 @@ __global__ __launch_bounds__(64) void k_range_select(JoinArgs a, G g0) {
   }
 }
 
+// ====================================================================== collect kernel
+// Streams the points of the block [lo, hi] 64 per round like for_each_point_flat, but every
+// lane calls f(c, p, valid) in every round (c: the coordinates of sorted position p, valid or
+// not), so f may use wave-wide operations; the coordinates of PR rounds are loaded before the
+// first of them is processed.
+template <typename T, int D, typename G, typename F>
+__device__ __forceinline__ void stream_block(const JoinArgs &a, const G &g, const uint32_t *lo,
+                                             const uint32_t *hi, bool chord, const double *qd,
+                                             double r2, uint32_t lane, uint32_t *rowB, uint32_t *rowX,
+                                             F &&f) {
+  uint64_t R = 1;
+  for (uint32_t j = 1; j < g.nd; j++) R *= (uint64_t)(hi[j] - lo[j] + 1);
+  for (uint64_t rb = 0; rb < R; rb += 64) {
+    const uint64_t rr = rb + lane;
+    uint32_t pB = 0, len = 0;
+    if (rr < R) {
+      uint32_t cur[KNN_MAX_INDEXED];
+      uint64_t t = rr;
+      uint64_t rowBase = g.keyBase;
+      for (uint32_t j = 1; j < g.nd; j++) {
+        const uint32_t span = hi[j] - lo[j] + 1;
+        cur[j] = lo[j] + (uint32_t)(t % span);
+        t /= span;
+        rowBase += (uint64_t)cur[j] * g.mult[j];
+      }
+      uint32_t l0 = lo[0], h0 = hi[0];
+      const bool keep = chord ? row_chord_lane<T, D, G>(g, cur, qd, r2, l0, h0) : true;
+      if (keep) {
+        uint32_t pE;
+        if (a.denseStart) {
+          pB = a.denseStart[rowBase + l0];
+          pE = a.denseStart[rowBase + h0 + 1];
+        } else {
+          const uint32_t cbeg = lower_bound_u64(a.cellKeys, a.ncell, rowBase + l0);
+          const uint32_t cend = upper_bound_u64(a.cellKeys, a.ncell, rowBase + h0);
+          pB = a.cellStart[cbeg];
+          pE = a.cellStart[cend];
+        }
+        len = pE - pB;
+      }
+    }
+    uint32_t inc = len;
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t t = (uint32_t)__shfl_up((int)inc, o, 64);
+      if (lane >= (uint32_t)o) inc += t;
+    }
+    const uint32_t total = (uint32_t)__builtin_amdgcn_readlane((int)inc, 63);
+    rowB[lane] = pB;
+    rowX[lane] = inc - len;
+    wave_lds_sync();
+    constexpr int PR = D <= 8 ? 4 : 2;
+    const T *__restrict__ xs = reinterpret_cast<const T *>(a.xs);
+    for (uint32_t f0 = 0; f0 < total; f0 += 64u * PR) {
+      T cv[PR][D];
+      uint32_t pv[PR];
+      bool val[PR];
+#pragma unroll
+      for (int u = 0; u < PR; u++) {
+        const uint32_t fi = f0 + 64u * u + lane;
+        val[u] = fi < total;
+        uint32_t p = 0;
+        if (val[u]) {
+          uint32_t l = 0, h = 64;
+          while (h - l > 1) {
+            const uint32_t mid = (l + h) >> 1;
+            if (rowX[mid] <= fi) l = mid; else h = mid;
+          }
+          p = rowB[l] + (fi - rowX[l]);
+        }
+        pv[u] = p;
+#pragma unroll
+        for (int j = 0; j < D; j++) cv[u][j] = val[u] ? xs[(size_t)j * a.n + p] : (T)0;
+      }
+#pragma unroll
+      for (int u = 0; u < PR; u++)
+        if (f0 + 64u * u < total) f(cv[u], pv[u], val[u]);
+    }
+    wave_lds_sync();
+  }
+}
+
+// Wave = query, for the queries of a tile in turn: the query's own block (radius r; rows
+// narrowed to the chords of its ball with 3 or more indexed dims) is streamed 64 candidates at
+// a time; the FP32/T sum is compared with the T bound of the collector's threshold (first
+// Tq = r^2 (1 - 2^-20), then the (K+1)-th key held), and only candidates that pass get their
+// FP64 key and id and enter the LDS collector.  The query succeeds when K+1 pairs with key
+// <= Tq were collected: they are then its exact top-(K+1), since every point with key <= Tq
+// lies in the block.  One pass over the candidates, against the select kernel's two (the
+// tile's union, then the query's sub-block): the better choice when a query has many
+// candidates per neighbour, as with 3 or more indexed dims.
+template <typename T, int D, typename G>
+__global__ __launch_bounds__(64) void k_range_collect(JoinArgs a, G g0) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  uint32_t *rowB = reinterpret_cast<uint32_t *>(smem);
+  uint32_t *rowX = rowB + 64;
+  unsigned char *cbase = smem + 128 * sizeof(uint32_t);
+  const uint32_t lane = threadIdx.x;
+  const T *__restrict__ xs = reinterpret_cast<const T *>(a.xs);
+  const T *__restrict__ x = reinterpret_cast<const T *>(a.x);
+  const size_t n = a.n;
+  const uint32_t NT = tile_count(a);
+  for (;;) {
+    const uint32_t t = next_tile(a.tileCounter, lane);
+    if (t >= NT) break;
+    const uint4 tl = a.tiles[t];
+    const TileCtx<G> C = tile_ctx<G>(a, g0, tl);
+    const G &g = C.g;
+    const uint32_t K1 = C.k1;
+    const T eps = (T)g.eps;
+    uint32_t nok = 0;
+    unsigned long long cand = 0;
+    for (uint32_t i = 0; i < tl.z; i++) {
+      const uint32_t q = (uint32_t)__builtin_amdgcn_readfirstlane((int)a.act[tl.y + i]);
+      T qc[D];
+      double qd[D];
+#pragma unroll
+      for (int j = 0; j < D; j++) {
+        qc[j] = x[(size_t)j * n + q];
+        qd[j] = (double)qc[j];
+      }
+      const T r = a.radius ? reinterpret_cast<const T *>(a.radius)[q] : eps;
+      const double Tq = (double)r * (double)r * (1.0 - 0x1p-20);
+      const T F = Num<T>::bound(Tq);
+      uint32_t lo[KNN_MAX_INDEXED], hi[KNN_MAX_INDEXED];
+      for (uint32_t j = 0; j < g.nd; j++) {
+        T xj = qc[0];
+#pragma unroll
+        for (int jj = 1; jj < D; jj++)
+          if ((uint32_t)jj == j) xj = qc[jj];
+        lo[j] = cell_coord<T>(next_down(xj - r), (T)g.minArr[j], eps, g.nCells[j]);
+        hi[j] = cell_coord<T>(next_up(xj + r), (T)g.minArr[j], eps, g.nCells[j]);
+      }
+      Collector col;
+      col.init(cbase, K1);
+      col.limit(Tq);
+      T fb = F;
+      const double rF2 = (double)F * (1.0 + 0x1p-15);
+      stream_block<T, D, G>(a, g, lo, hi, g.nd >= 3, qd, rF2, lane, rowB, rowX, [&](const T *c, uint32_t p, bool valid) {
+        cand += (unsigned long long)__popcll(__ballot(valid));
+        const bool near = valid && Num<T>::template sq<D>(c, qc) <= fb;
+        if (__ballot(near) == 0) return;
+        const uint32_t cid = near ? a.sortedIds[p] : 0u;
+        const double key = near ? fp64_key<D, T>(c, qd) : 0.0;
+        col.offer(near, key, cid, lane);
+        fb = col.nb == K1 ? Num<T>::bound(col.tk) : F;
+      });
+      col.merge(lane);
+      const bool ok = col.nb == K1;
+      if (ok) {
+        T *od = reinterpret_cast<T *>(a.outDists);
+        for (uint32_t e = lane; e < K1; e += 64) {
+          const uint32_t id = col.best_id(e);
+          T c[D];
+#pragma unroll
+          for (int j = 0; j < D; j++) c[j] = x[(size_t)j * n + id];
+          a.outIds[C.row(q) + e] = (int32_t)(id - C.idBase);
+          od[C.row(q) + e] = sqrt(Num<T>::template sq<D>(c, qc));
+        }
+      }
+      if (lane == 0) a.failed[q] = ok ? 0 : 1;
+      nok += ok ? 1u : 0u;
+      br_sync();  // the collector's LDS is reused by the next query
+    }
+    if (lane == 0) {
+      if (tl.z - nok) atomicAdd(a.failCount, tl.z - nok);
+      atomicAdd(a.candCount + 1, (unsigned long long)tl.z * 8ull + (unsigned long long)nok * K1 * (4ull + sizeof(T)));
+      atomicAdd(a.candCount, cand);
+    }
+  }
+}
+
 // ====================================================================== heap kernel
 // Per-lane heap with lane-interleaved slots: key[s*64 + lane], id[s*64 + lane].
 struct Heap {
@@ -961,6 +1133,13 @@ void launch_heap(const JoinArgs &a, const G &g, hipStream_t s) {
 }
 
 template <typename T, int D, typename G>
+void launch_collect(const JoinArgs &a, const G &g, hipStream_t s) {
+  if (a.ntiles == 0) return;
+  const uint32_t grid = a.ntilesDev ? a.ntiles : range_select_grid(a.ntiles);
+  k_range_collect<T, D, G><<<grid, 64, range_collect_lds_bytes(a.k1), s>>>(a, g);
+}
+
+template <typename T, int D, typename G>
 void launch_select(const JoinArgs &a, const G &g, hipStream_t s) {
   if (a.ntiles == 0) return;
   const uint32_t grid = a.ntilesDev ? a.ntiles : range_select_grid(a.ntiles);
@@ -1007,6 +1186,17 @@ void range_select(const JoinArgs &a, const typename GridOf<T>::type &g, hipStrea
 }
 
 template <typename T>
+void range_collect(const JoinArgs &a, const typename GridOf<T>::type &g, hipStream_t s) {
+  using G = typename GridOf<T>::type;
+  DISPATCH_D(launch_collect)
+}
+
+size_t range_collect_lds_bytes(uint32_t k1) {
+  return 128 * sizeof(uint32_t) + (size_t)k1 * 2 * (sizeof(double) + sizeof(uint32_t)) +
+         (size_t)BR_BUF * (sizeof(double) + sizeof(uint32_t));
+}
+
+template <typename T>
 void range_topk(const JoinArgs &a, const typename GridOf<T>::type &g, hipStream_t s) {
   using G = typename GridOf<T>::type;
   DISPATCH_D(launch_heap)
@@ -1024,6 +1214,7 @@ size_t range_heap_scratch_bytes(uint32_t ntiles, uint32_t k1) {
   template void range_select<T>(const JoinArgs &, const GridOf<T>::type &, hipStream_t); \
   template size_t range_select_lds_bytes<T>(uint32_t);                                   \
   template void range_topk<T>(const JoinArgs &, const GridOf<T>::type &, hipStream_t);   \
+  template void range_collect<T>(const JoinArgs &, const GridOf<T>::type &, hipStream_t); \
   template size_t range_heap_scratch_bytes<T>(uint32_t, uint32_t);
 INST(float)
 INST(double)

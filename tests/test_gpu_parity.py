@@ -170,3 +170,23 @@ def test_repeated_calls_reuse_buffers(oracle):
     assert np.array_equal(r1[0], r3[0]) and np.array_equal(bits(r1[1]), bits(r3[1]))
     rc, i2, d2, _ = oracle.knnjoin(b, 8, nd=2, beta=0.05)
     assert np.array_equal(r2[0], i2)
+
+
+@pytest.mark.parametrize("kernel", ["select", "collect"])
+@pytest.mark.parametrize("d,nd,k,gen", [(2, 2, 16, "mixed"), (4, 4, 32, "uniform"), (6, 6, 8, "skewed"),
+                                         (6, 4, 16, "uniform"), (3, 3, 200, "uniform")])
+def test_both_range_kernels_match_oracle(oracle, monkeypatch, kernel, d, nd, k, gen):
+    # the select kernel (two passes, LDS histogram) and the collect kernel (one pass, LDS
+    # collector) are chosen by the number of indexed dims; forced here onto the same inputs
+    monkeypatch.setenv("KNN_RANGE_KERNEL", kernel)
+    if gen == "mixed":
+        pts = synthetic.mixed_box(60000, seed=71)
+    elif gen == "uniform":
+        pts = synthetic.unit_cube(30000, d, seed=72 + d)
+    else:
+        pts = synthetic.skewed_cube(30000, d, seed=73 + d)
+    ids, dist, st = _lib.knnjoin_ex(pts, k, num_indexed_dims=nd, beta=0.025)
+    rc, i2, d2, _ = oracle.knnjoin(pts, k, nd=nd, beta=0.025)
+    assert rc == 0
+    assert np.array_equal(ids, i2)
+    assert np.array_equal(bits(dist), bits(d2))
