@@ -1864,8 +1864,9 @@ static void batch_index(BatchGroup &G, hipStream_t s) {
 }
 
 // One radius pass of a group: every set's lowest pending octave, grids matched to it, tiles of
-// those queries, the select kernel, the heap kernel for staging overflows, and radius growth
-// of the queries that failed; G.ctr[3] = queries still pending.
+// those queries, the select kernel (the collect kernel with 5+ indexed dims), the heap kernel
+// for staging overflows, and radius growth of the queries that failed; G.ctr[3] = queries
+// still pending.
 static void batch_pass(const BatchPlan &B, BatchGroup &G, hipStream_t s) {
   const uint32_t N = G.N;
   batch_class(G.sets, G.nsets, G.setOf, G.failed, G.bin, N, G.setMinBin, G.width, G.actq, s);
@@ -1878,7 +1879,10 @@ static void batch_pass(const BatchPlan &B, BatchGroup &G, hipStream_t s) {
   compact_active(G.ids, G.aflags, G.apos, N, G.act, s);
   batch_tiles(G.cellStart, G.apos, G.cpos + N, N, B.tq, G.ntiles, G.tileOff, G.scratch, G.tiles, s);
   fill_u32(G.ctr, 0u, 4, s);
-  range_select<float>(a, g0, s);
+  if (use_collect(G.hsets.empty() ? 0u : G.hsets[0].nd, G.k1max))
+    range_collect<float>(a, g0, s);
+  else
+    range_select<float>(a, g0, s);
 
   batch_flags_u8(G.ids, G.overflow, N, G.aflags, s);
   scan_exclusive_u32(G.aflags, G.apos, N, G.scratch, s);
