@@ -33,6 +33,7 @@ void minmax_soa(const T *x, uint32_t n, uint32_t nd, T *mn, T *mx, T *scratch, h
 void iota_u32(uint32_t *a, uint32_t n, hipStream_t s);
 void fill_u8(uint8_t *a, uint8_t v, size_t n, hipStream_t s);
 void fill_u32(uint32_t *a, uint32_t v, size_t n, hipStream_t s);
+void fill_u64(uint64_t *a, uint64_t v, size_t n, hipStream_t s);
 
 // --------------------------------------------------------------- estimator (estimator.hip)
 // main.cu:2521-2615: exact sequential variance (T arithmetic) per dim over rows 0,100,...
