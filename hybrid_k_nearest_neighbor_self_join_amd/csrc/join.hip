@@ -239,7 +239,21 @@ __global__ __launch_bounds__(256) void k_row_sums(const T *__restrict__ dists, u
   const uint32_t rows = (uint32_t)((n - r0) < rowsPerBlock ? (n - r0) : rowsPerBlock);
   const T *src = dists + r0 * k1;
   const uint32_t cnt = rows * k1;
-  for (uint32_t i = threadIdx.x; i < cnt; i += blockDim.x) buf[i] = src[i];
+  // eight independent loads in flight per thread before their LDS stores
+  constexpr uint32_t U = 8;
+  for (uint32_t i0 = threadIdx.x; i0 < cnt; i0 += U * 256u) {
+    T v[U];
+#pragma unroll
+    for (uint32_t u = 0; u < U; u++) {
+      const uint32_t i = i0 + u * 256u;
+      v[u] = i < cnt ? src[i] : (T)0;
+    }
+#pragma unroll
+    for (uint32_t u = 0; u < U; u++) {
+      const uint32_t i = i0 + u * 256u;
+      if (i < cnt) buf[i] = v[u];
+    }
+  }
   __syncthreads();
   for (uint32_t r = threadIdx.x; r < rows; r += blockDim.x) {
     const T *row = buf + (size_t)r * k1;
