@@ -1606,8 +1606,13 @@ int knnjoin_radius_features(const float *points, uint32_t n, uint32_t ndim, uint
       Wd = P.buf<float>(S_MODEL, MLP_NW);
       HIP_CHECK(hipMemcpyAsync(Wd, M.w.data(), sizeof(float) * MLP_NW, hipMemcpyHostToDevice, s));
     }
-    radius_kernel<float>(P.x, n, F.g, cnt, F.logk1, F.logdens, Wd, M.ok ? M.margin : 0.0f,
-                         (float)F.wf, rd, fd, nullptr, s);
+    // features in FP32 (training data); the radius with the join's own inference kernel
+    if (fd)
+      radius_kernel<float>(P.x, n, F.g, cnt, F.logk1, F.logdens, nullptr, 0.0f, (float)F.wf, nullptr, fd,
+                           nullptr, s);
+    if (rd)
+      radius_kernel<float>(P.x, n, F.g, cnt, F.logk1, F.logdens, Wd, M.margin, (float)F.wf, rd, nullptr,
+                           nullptr, s);
     if (features)
       HIP_CHECK(hipMemcpyAsync(features, fd, sizeof(float) * (size_t)n * MLP_IN, hipMemcpyDeviceToHost, s));
     if (radius_out)

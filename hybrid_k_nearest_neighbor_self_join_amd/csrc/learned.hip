@@ -21,6 +21,8 @@
 // MLP: h1 = relu(W1 f + b1), h2 = relu(W2 h1 + b2), y = w3 . h2 + b3, FP32 with fmaf in
 // index order (so the CPU oracle, using C's fmaf in the same order, reproduces it bitwise).
 // Weights, flat: W1[64][12], b1[64], W2[64][64], b2[64], w3[64], b3.
+#include <algorithm>
+
 #include "knn_launch.h"
 
 namespace knn {
@@ -174,6 +176,103 @@ __global__ __launch_bounds__(256) void k_radius(const T *__restrict__ x, uint32_
       // r = wf e^(y + margin), the exponent kept within [-8, 14] (about wf/3000 .. 1.2e6 wf)
       // so that any weights give a finite radius the expansion loop can start from
       float e = y + margin;
+      if (!(e == e)) e = 0.0f;
+      e = fminf(fmaxf(e, -8.0f), 14.0f);
+      const float r = wf * expf(e);
+      radius[q] = (T)r;
+      if (rhist) atomicAdd(&lh[radius_bin<float>(r, wf)], 1u);
+    }
+  }
+  __syncthreads();
+  if (rhist && threadIdx.x < 64 && lh[threadIdx.x]) atomicAdd(&rhist[threadIdx.x], lh[threadIdx.x]);
+}
+
+// The join's radius inference on the matrix cores: the MLP in FP16 operands with FP32
+// accumulation (v_mfma_f32_16x16x16f16), in the transposed form H^T = W X^T so that a layer's
+// output fragment (lane: column = query lane%16, rows 4*(lane/16)+i of a 16-row tile) is the
+// next layer's B fragment (rows = k) as it stands.  Per group of 16 queries: 4 MFMAs for
+// layer 1 (12 features padded to 16), 16 for layer 2, then layer 3 as a 16-term dot product
+// per lane and two lane exchanges.  The radius only steers the schedule (rows are exact for
+// any radius), so FP16 operands (y within about 1e-2 of the FP32 forward) are enough; the
+// features are computed in FP32 exactly as k_radius does.  Persistent grid; each wave loads its
+// 20 weight fragments once.
+typedef _Float16 half4_t __attribute__((ext_vector_type(4)));
+typedef float float4_t __attribute__((ext_vector_type(4)));
+
+template <typename T, typename G>
+__global__ __launch_bounds__(256) void k_radius_mfma(const T *__restrict__ x, uint32_t n, G g,
+                                                     const uint32_t *__restrict__ cnt, float logk1,
+                                                     float logdens, const float *__restrict__ Wg,
+                                                     float margin, float wf, T *radius, uint32_t *rhist) {
+  __shared__ _Float16 fs[4][64][16];
+  __shared__ uint32_t lh[64];
+  const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const uint32_t r16 = lane & 15, kq = lane >> 4;
+  const float *W1 = Wg, *b1 = Wg + 768, *W2 = Wg + 832, *b2 = Wg + 4928, *w3 = Wg + 4992;
+  half4_t a1[4], a2[4][4];
+  float b1v[4][4], b2v[4][4], w3v[4][4];
+#pragma unroll
+  for (int mt = 0; mt < 4; mt++) {
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const uint32_t k = 4 * kq + i;
+      a1[mt][i] = (_Float16)(k < (uint32_t)MLP_IN ? W1[(16 * mt + r16) * MLP_IN + k] : 0.0f);
+      b1v[mt][i] = b1[16 * mt + 4 * kq + i];
+      b2v[mt][i] = b2[16 * mt + 4 * kq + i];
+      w3v[mt][i] = w3[16 * mt + 4 * kq + i];
+#pragma unroll
+      for (int t = 0; t < 4; t++) a2[mt][t][i] = (_Float16)W2[(16 * mt + r16) * MLP_H + 16 * t + 4 * kq + i];
+    }
+  }
+  const float b3 = Wg[5056];
+  if (threadIdx.x < 64) lh[threadIdx.x] = 0;
+  __syncthreads();
+  const uint64_t stride = (uint64_t)gridDim.x * 256;
+  for (uint64_t base = (uint64_t)blockIdx.x * 256 + wv * 64; base < n; base += stride) {
+    const uint64_t q = base + lane;
+    const bool valid = q < n;
+    float f[MLP_IN];
+    if (valid) {
+      features_of<T, G>(x, n, (uint32_t)q, g, cnt, logk1, logdens, f);
+    } else {
+#pragma unroll
+      for (int j = 0; j < MLP_IN; j++) f[j] = 0.0f;
+    }
+#pragma unroll
+    for (int j = 0; j < 16; j++) fs[wv][lane][j] = (_Float16)(j < MLP_IN ? f[j] : 0.0f);
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    float yMine = 0.0f;
+#pragma unroll
+    for (uint32_t b = 0; b < 4; b++) {
+      half4_t bx;
+#pragma unroll
+      for (int i = 0; i < 4; i++) bx[i] = fs[wv][16 * b + r16][4 * kq + i];
+      half4_t h1[4];
+#pragma unroll
+      for (int mt = 0; mt < 4; mt++) {
+        float4_t acc = {0.0f, 0.0f, 0.0f, 0.0f};
+        acc = __builtin_amdgcn_mfma_f32_16x16x16f16(a1[mt], bx, acc, 0, 0, 0);
+#pragma unroll
+        for (int i = 0; i < 4; i++) h1[mt][i] = (_Float16)relu(acc[i] + b1v[mt][i]);
+      }
+      float part = 0.0f;
+#pragma unroll
+      for (int mt = 0; mt < 4; mt++) {
+        float4_t acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int t = 0; t < 4; t++) acc = __builtin_amdgcn_mfma_f32_16x16x16f16(a2[mt][t], h1[t], acc, 0, 0, 0);
+#pragma unroll
+        for (int i = 0; i < 4; i++) part = fmaf(w3v[mt][i], relu(acc[i] + b2v[mt][i]), part);
+      }
+      part += __shfl_xor(part, 16, 64);
+      part += __shfl_xor(part, 32, 64);
+      if (kq == b) yMine = part + b3;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();  // fs is rewritten by the next round
+    if (valid) {
+      float e = yMine + margin;
       if (!(e == e)) e = 0.0f;
       e = fminf(fmaxf(e, -8.0f), 14.0f);
       const float r = wf * expf(e);
@@ -373,9 +472,16 @@ template <typename T>
 void radius_kernel(const T *x, uint32_t n, const typename GridOf<T>::type &g, const uint32_t *cnt,
                    float logk1, float logdens, const float *W, float margin, float wf, T *radius,
                    float *featOut, uint32_t *rhist, hipStream_t s) {
-  if (n)
-    k_radius<T, typename GridOf<T>::type><<<(n + 255) / 256, 256, 0, s>>>(
-        x, n, g, cnt, logk1, logdens, W, margin, wf, radius, featOut, rhist);
+  if (!n) return;
+  if (radius && !featOut && W) {
+    // inference only (the join): matrix cores, persistent grid
+    const uint32_t blocks = (uint32_t)std::min<uint64_t>(((uint64_t)n + 255) / 256, 2048);
+    k_radius_mfma<T, typename GridOf<T>::type><<<blocks, 256, 0, s>>>(x, n, g, cnt, logk1, logdens, W,
+                                                                        margin, wf, radius, rhist);
+    return;
+  }
+  k_radius<T, typename GridOf<T>::type><<<(n + 255) / 256, 256, 0, s>>>(
+      x, n, g, cnt, logk1, logdens, W, margin, wf, radius, featOut, rhist);
 }
 
 template <typename T>

@@ -61,18 +61,22 @@ def test_features_match_oracle(oracle, gen, nd, k):
     assert np.array_equal(np.round(np.expm1(f[:, :10])), np.round(np.expm1(f2[:, :10])))
 
 
-def test_radius_inference_matches_oracle_mlp(oracle):
-    # the join's radius kernel (W2 held transposed in LDS) against the oracle's MLP on the same
-    # features: same fmaf order, so y agrees bitwise; exp may differ in the last place
+@pytest.mark.parametrize("scale", [0.5, 2.0])
+def test_radius_inference_matches_oracle_mlp(oracle, scale):
+    # the join's radius inference (MLP on the matrix cores, FP16 operands, FP32 accumulation)
+    # against the oracle's FP32 MLP on the same features: log-radius within 3e-2, 5e-3 at the
+    # median; the radius only chooses which pass a query is first tried in
     pts = synthetic.mixed_box(50000, seed=12)
-    w = learned_radius.init_weights(17) * 0.5
+    w = learned_radius.init_weights(17) * scale
     margin = 0.25
     _lib.set_radius_model(2, w, margin)
     f, wf, r = _lib.radius_features(pts, 16, nd=2, want_radius=True)
     y = oracle.mlp_forward(w, f)
-    e = np.clip(y + np.float32(margin), np.float32(-8.0), np.float32(14.0))  # float32, as on the GPU
-    want = np.float32(wf) * np.exp(e.astype(np.float64)).astype(np.float32)
-    assert np.allclose(r, want, rtol=4e-7, atol=0)
+    e = np.clip(y.astype(np.float64) + margin, -8.0, 14.0)
+    err = np.abs(np.log(r.astype(np.float64) / (wf * np.exp(e))))
+    assert err.max() < 3e-2 and np.median(err) < 5e-3, (err.max(), np.median(err))
+    # spread of the prediction itself, so that the bound above is not vacuous
+    assert np.std(y) > 0.05
 
 
 def _random_model(nd, seed, scale):
