@@ -18,8 +18,10 @@
 // Target: log(r_K1 / wf), r_K1 = distance to the (K+1)-th nearest point (self included).
 // Radius: r = wf * exp(mlp(f) + margin), margin calibrated on the training residuals.
 //
-// MLP: h1 = relu(W1 f + b1), h2 = relu(W2 h1 + b2), y = w3 . h2 + b3, FP32 with fmaf in
-// index order (so the CPU oracle, using C's fmaf in the same order, reproduces it bitwise).
+// MLP: h1 = relu(W1 f + b1), h2 = relu(W2 h1 + b2), y = w3 . h2 + b3.  Training, the
+// gradient and the forward entry point run in FP32 with fmaf in index order (so the CPU
+// oracle, using C's fmaf in the same order, reproduces them bitwise); the join's inference
+// (k_radius_mfma) runs on the matrix cores with FP16 operands and FP32 accumulation.
 // Weights, flat: W1[64][12], b1[64], W2[64][64], b2[64], w3[64], b3.
 #include <algorithm>
 

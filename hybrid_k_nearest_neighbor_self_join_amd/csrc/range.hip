@@ -4,7 +4,8 @@
 // thrust::stable_sort_by_key of GPU.cu:1629-1634, and the host top-k of GPU.cu:3610-3704.
 //
 // Work unit: one wavefront per tile of <= 64 consecutive active queries of one tile block (a
-// window of cells along dim 0 in one row of the grid); lane = query.  Each lane has a search radius r (the grid's eps, or its own radius) and scans the
+// window of cells along dim 0 in one row of the grid); lane = query.  Each lane has a search
+// radius r (the grid's eps, or its own radius, which may span several cells) and scans the
 // cells [coord(x - r, rounded down), coord(x + r, rounded up)] per indexed dim; the tile scans
 // the union of its lanes' blocks.  Within one row of that block (dims 1.. fixed) the cells of
 // dim 0 are adjacent linear ids, i.e. one contiguous range of cell-sorted points, streamed
@@ -28,9 +29,13 @@
 //           sub-block, radius sqrt(edge of b*), coalesced; candidates in buckets <= b* (plus a
 //           2^-16 margin past b*'s edge) are placed in LDS staging in bucket order, ranked
 //           exactly by (key, id) against buckets b-1..b+1 only, and the first K+1 are written
-//           to the row (coalesced) when the (K+1)-th key is <= Tq.
+//           to the row (coalesced) when the (K+1)-th key is <= Tq.  The sub-blocks' row ranges
+//           are looked up for the whole tile at once (lane = query) and a query's candidates
+//           are loaded 4 rounds at a time, so a query waits for one memory round trip.
 //   A query whose staged list exceeds `smax` LDS slots (e.g. thousands of duplicates in one
 //   bucket) is flagged for k_range_heap.
+// k_range_collect (5 or more indexed dims): wave = query, one pass over the query's own block
+//   into the LDS top-(K+1) collector of knn_collect.h, FP32-prefiltered against its threshold.
 // k_range_heap: per-lane max-heap of (key, id), in LDS when K+1 fits, else in global scratch.
 #include <type_traits>
 
