@@ -213,8 +213,8 @@ typedef struct {
   int32_t device_ptrs;       /* 1 = set points and outputs are device pointers */
   int32_t use_graph;         /* 1 = capture the join in a hipGraph, replay for equal shapes */
   uint32_t graph_passes;     /* radius passes inside the graph (0 = 8); later passes run eagerly */
-  uint32_t tile_queries;     /* queries per tile, <= 64 (0 = autotuned value or 64) */
-  uint32_t grid_blocks;      /* persistent range-kernel blocks, <= 8192 (0 = autotuned or 2048) */
+  uint32_t tile_queries;     /* queries per tile, <= 64 (0 = autotuned value or 16) */
+  uint32_t grid_blocks;      /* persistent range-kernel blocks, <= 8192 (0 = autotuned or 4096) */
   int32_t verbose;
 } knnjoin_batch_params;
 
