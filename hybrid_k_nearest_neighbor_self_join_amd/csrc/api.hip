@@ -379,16 +379,13 @@ template <typename T>
 static EpsResult estimate_eps(const T *x, uint32_t n, uint32_t d, uint32_t k, double beta,
                               uint32_t *hist_out, hipStream_t s) {
   EpsResult r{};
-  const size_t m = avgdist_threads(n);
-  double *part = reinterpret_cast<double *>(g_cache.get(S_PART, m * 16 + 64));
-  uint64_t *cnts = reinterpret_cast<uint64_t *>(part + m);
-  double *tot = reinterpret_cast<double *>(cnts + m);
-  uint64_t *cnt = reinterpret_cast<uint64_t *>(tot + 1);
-  avg_dist_soa<T>(x, n, d, part, cnts, tot, cnt, s);
+  const size_t m = avgdist_scratch_doubles(n);
+  double *scr = reinterpret_cast<double *>(g_cache.get(S_PART, m * sizeof(double) + 64));
+  double *tot = scr + m;
+  avg_dist_soa<T>(x, n, d, scr, tot, s);
   double h_tot = 0;
-  uint64_t h_cnt = 0;
+  const uint64_t h_cnt = avgdist_pairs(n);
   HIP_CHECK(hipMemcpyAsync(&h_tot, tot, sizeof(double), hipMemcpyDeviceToHost, s));
-  HIP_CHECK(hipMemcpyAsync(&h_cnt, cnt, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
   HIP_CHECK(hipStreamSynchronize(s));
   r.avg = h_cnt ? h_tot / (h_cnt * 1.0) : 0.0;  // GPU.cu:3334
   r.cnt = h_cnt;

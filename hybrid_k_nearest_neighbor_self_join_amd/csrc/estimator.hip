@@ -10,6 +10,8 @@
 //    into the global one by integer atomics (order-free).  Points past N are skipped where
 //    the reference reads beyond the array (kernel.cu:486-492); bucket == 10000 is dropped
 //    where the reference writes one past the histogram.
+#include <algorithm>
+
 #include "knn_launch.h"
 
 namespace knn {
@@ -133,18 +135,36 @@ __global__ void k_permute_to_soa(const T *__restrict__ pts, uint32_t n, uint32_t
   for (uint32_t j = 0; j < d; j++) out[(size_t)j * n + i] = row[perm.p[j]];
 }
 
+// kernel.cu:343-381: sample point tid = g*1000 (g < G = ceil(0.001N/256)*256) against points
+// i = g%256 + 1000k, per pair sqrt(sum (double)(T)(diff*diff)).  The reference adds with
+// double atomics (no reproducible order); here the sum has a fixed order over a fixed
+// partition, spread over many threads.  With Kmax = ceil(N/1000) pairs for the longest
+// thread, S = min(ceil(Kmax/64), max(1, floor(2^26 / G))) segments per thread of
+// L = ceil(Kmax/S) pairs (L = 64 up to about 50M points; the cap bounds the scratch):
+//   seg(g, s)  = sum of pairs k in [L s, L s + L) in k order
+//   part(g)    = sum over s of seg(g, s) in s order
+//   chunk(c)   = sum over g in [256c, 256c + 256) of part(g) in g order
+//   total      = sum over c of chunk(c) in c order
+struct AvgLayout {
+  uint32_t G, S, L;
+};
+
 template <typename T>
-__global__ __launch_bounds__(256) void k_avg_dist(const T *__restrict__ x, uint32_t n, uint32_t d,
-                                                  double *partials, uint64_t *counts) {
-  const uint32_t g = blockIdx.x * 256 + threadIdx.x;
+__global__ __launch_bounds__(256) void k_avg_seg(const T *__restrict__ x, uint32_t n, uint32_t d,
+                                                 uint32_t G, uint32_t S, uint32_t L, double *seg) {
+  const uint64_t item = (uint64_t)blockIdx.x * 256 + threadIdx.x;  // item = s * G + g
+  if (item >= (uint64_t)G * S) return;
+  const uint32_t g = (uint32_t)(item % G), sg = (uint32_t)(item / G);
   const uint64_t tid = (uint64_t)g * 1000;
-  double part = 0;
-  uint64_t c = 0;
+  double sum = 0;
   if (tid < n) {
     T q[KNN_MAXDIM];
 #pragma unroll
     for (uint32_t j = 0; j < KNN_MAXDIM; j++) q[j] = j < d ? x[(size_t)j * n + tid] : (T)0;
-    for (uint64_t i = threadIdx.x; i < n; i += 1000) {
+    const uint64_t i0 = (uint64_t)(g % 256) + 1000ull * L * sg;
+    for (uint32_t k = 0; k < L; k++) {
+      const uint64_t i = i0 + 1000ull * k;
+      if (i >= n) break;
       double run = 0;
 #pragma unroll
       for (uint32_t j = 0; j < KNN_MAXDIM; j++) {
@@ -154,25 +174,33 @@ __global__ __launch_bounds__(256) void k_avg_dist(const T *__restrict__ x, uint3
           run += (double)sq;
         }
       }
-      part += sqrt(run);
-      c++;
+      sum += sqrt(run);
     }
   }
-  partials[g] = part;
-  counts[g] = c;
+  seg[item] = sum;
 }
 
-__global__ void k_sum_partials(const double *partials, const uint64_t *counts, uint32_t m,
-                               double *total, uint64_t *cnt) {
+__global__ void k_avg_part(const double *seg, uint32_t G, uint32_t S, double *part) {
+  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= G) return;
+  double t = 0;
+  for (uint32_t sg = 0; sg < S; sg++) t += seg[(size_t)sg * G + g];
+  part[g] = t;
+}
+
+__global__ void k_avg_chunks(const double *part, uint32_t G, double *chunk) {
+  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= G / 256) return;
+  double t = 0;
+  for (uint32_t g = 256 * c; g < 256 * c + 256; g++) t += part[g];
+  chunk[c] = t;
+}
+
+__global__ void k_avg_total(const double *chunk, uint32_t C, double *total) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   double t = 0;
-  uint64_t c = 0;
-  for (uint32_t g = 0; g < m; g++) {
-    t += partials[g];
-    c += counts[g];
-  }
+  for (uint32_t c = 0; c < C; c++) t += chunk[c];
   *total = t;
-  *cnt = c;
 }
 
 constexpr uint32_t NBUCKETS = 10000;
@@ -244,12 +272,42 @@ size_t avgdist_threads(uint32_t n) {
   return (size_t)total_blocks * 256;
 }
 
+static AvgLayout avg_layout(uint32_t n) {
+  AvgLayout a;
+  a.G = (uint32_t)avgdist_threads(n);
+  const uint64_t kmax = ((uint64_t)n + 999) / 1000;
+  uint64_t S = (kmax + 63) / 64;
+  const uint64_t cap = std::max<uint64_t>(1, (1ull << 26) / a.G);
+  if (S > cap) S = cap;
+  if (S == 0) S = 1;
+  a.S = (uint32_t)S;
+  a.L = (uint32_t)((kmax + S - 1) / S);
+  if (a.L == 0) a.L = 1;
+  return a;
+}
+
+size_t avgdist_scratch_doubles(uint32_t n) {
+  const AvgLayout a = avg_layout(n);
+  return (size_t)a.G * a.S + a.G + a.G / 256 + 8;
+}
+
 template <typename T>
-void avg_dist_soa(const T *x, uint32_t n, uint32_t d, double *partials, uint64_t *counts,
-                  double *total, uint64_t *cnt, hipStream_t s) {
-  const uint32_t m = (uint32_t)avgdist_threads(n);
-  k_avg_dist<T><<<m / 256, 256, 0, s>>>(x, n, d, partials, counts);
-  k_sum_partials<<<1, 64, 0, s>>>(partials, counts, m, total, cnt);
+void avg_dist_soa(const T *x, uint32_t n, uint32_t d, double *scratch, double *total, hipStream_t s) {
+  const AvgLayout a = avg_layout(n);
+  const uint32_t G = a.G, S = a.S;
+  double *seg = scratch, *part = seg + (size_t)G * S, *chunk = part + G;
+  const uint64_t items = (uint64_t)G * S;
+  k_avg_seg<T><<<(uint32_t)((items + 255) / 256), 256, 0, s>>>(x, n, d, G, S, a.L, seg);
+  k_avg_part<<<(G + 255) / 256, 256, 0, s>>>(seg, G, S, part);
+  k_avg_chunks<<<(G / 256 + 255) / 256, 256, 0, s>>>(part, G, chunk);
+  k_avg_total<<<1, 64, 0, s>>>(chunk, G / 256, total);
+}
+
+uint64_t avgdist_pairs(uint32_t n) {
+  const uint64_t G = avgdist_threads(n);
+  uint64_t c = 0;
+  for (uint64_t g = 0; g < G && g * 1000 < n; g++) c += ((uint64_t)n - g % 256 + 999) / 1000;
+  return c;
 }
 
 template <typename T>
@@ -264,8 +322,7 @@ void kdist_hist_soa(const T *x, uint32_t n, uint32_t d, uint32_t offset, uint32_
   template void variance_moments<T>(const T *, uint32_t, uint32_t, double *, hipStream_t);  \
   template void permute_to_soa<T>(const T *, uint32_t, uint32_t, const uint32_t *, T *,     \
                                   hipStream_t);                                             \
-  template void avg_dist_soa<T>(const T *, uint32_t, uint32_t, double *, uint64_t *,        \
-                                double *, uint64_t *, hipStream_t);                         \
+  template void avg_dist_soa<T>(const T *, uint32_t, uint32_t, double *, double *, hipStream_t); \
   template void kdist_hist_soa<T>(const T *, uint32_t, uint32_t, uint32_t, uint32_t, double, \
                                   double, uint32_t *, hipStream_t);
 INST(float)

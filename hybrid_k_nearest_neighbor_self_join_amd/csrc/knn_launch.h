@@ -48,12 +48,14 @@ void variance_moments(const T *pts, uint32_t n, uint32_t d, double *mom, hipStre
 template <typename T>
 void permute_to_soa(const T *pts, uint32_t n, uint32_t d, const uint32_t *perm_host, T *out,
                     hipStream_t s);
-// kernel.cu:343-381: per-thread double partial sums (partials[g], counts[g]) then a fixed-
-// order total in sums[0] (double) and counts total in cnt[0].
+// kernel.cu:343-381: the sampled pairwise distance sum in a fixed hierarchical order (64-pair
+// segments, per-thread sums, 256-thread chunks, total) into total[0]; scratch holds
+// avgdist_scratch_doubles(n) doubles.  avgdist_pairs(n): the number of pairs summed.
 size_t avgdist_threads(uint32_t n);
+size_t avgdist_scratch_doubles(uint32_t n);
+uint64_t avgdist_pairs(uint32_t n);
 template <typename T>
-void avg_dist_soa(const T *x, uint32_t n, uint32_t d, double *partials, uint64_t *counts,
-                  double *total, uint64_t *cnt, hipStream_t s);
+void avg_dist_soa(const T *x, uint32_t n, uint32_t d, double *scratch, double *total, hipStream_t s);
 // kernel.cu:450-523: k-distance histogram (10000 buckets, +8 per hit) of nsample sample
 // points (stride offset) against every 8th 256-point block.  avg/bw are host values.
 template <typename T>

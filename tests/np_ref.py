@@ -68,27 +68,39 @@ def bruteforce(pts_r: np.ndarray, k: int, queries=None):
 
 
 def avg_dist(pts_r: np.ndarray):
-    """kernel.cu:343-381 restated: sample point g*1000 vs points g%256 + 1000m."""
+    """kernel.cu:343-381 restated: sample point g*1000 vs points g%256 + 1000k, summed in the
+    fixed order of the GPU and the oracle: segments of L pairs, per-thread sums over the
+    segments, sums of 256 consecutive threads, then the total."""
     n, d = pts_r.shape
-    tb = math.ceil((0.001 * n) / 256.0)
+    tb = max(1, math.ceil((0.001 * n) / 256.0))
+    G = tb * 256
+    kmax = (n + 999) // 1000
+    S = max(1, min((kmax + 63) // 64, max(1, (1 << 26) // G)))
+    L = max(1, (kmax + S - 1) // S)
     total = 0.0
     cnt = 0
-    for g in range(tb * 256):
-        tid = g * 1000
-        if tid >= n:
-            continue
-        idx = np.arange(g % 256, n, 1000)
-        df = (pts_r[idx] - pts_r[tid]).astype(np.float32)
-        sq = (df * df).astype(np.float32).astype(np.float64)
-        run = sq[:, 0].copy()
-        for j in range(1, d):
-            run = run + sq[:, j]
-        dist = np.sqrt(run)
-        part = 0.0
-        for v in dist:  # sequential, as the per-thread sum
-            part += float(v)
-        total += part
-        cnt += len(idx)
+    for c0 in range(0, G, 256):
+        chunk = 0.0
+        for g in range(c0, c0 + 256):
+            tid = g * 1000
+            if tid >= n:
+                continue
+            idx = np.arange(g % 256, n, 1000)
+            df = (pts_r[idx] - pts_r[tid]).astype(np.float32)
+            sq = (df * df).astype(np.float32).astype(np.float64)
+            run = sq[:, 0].copy()
+            for j in range(1, d):
+                run = run + sq[:, j]
+            dist = np.sqrt(run)
+            part = 0.0
+            for s0 in range(0, len(dist), L):  # segments of L pairs, each summed in order
+                seg = 0.0
+                for v in dist[s0:s0 + L]:
+                    seg += float(v)
+                part += seg
+            chunk += part
+            cnt += len(idx)
+        total += chunk
     return (total / cnt if cnt else 0.0), cnt
 
 
