@@ -205,6 +205,11 @@ __global__ void k_avg_total(const double *chunk, uint32_t C, double *total) {
 
 constexpr uint32_t NBUCKETS = 10000;
 
+// kernel.cu:450-523: sample s = blockIdx.x / KD_PARTS (point s * offset) against the first
+// 256 points of every 2048, +8 per hit in bucket floor(d / bw) for d < avg (self excluded).
+// The counts are integers, so the partition is free: each sample's 2048-point strides are
+// dealt to KD_PARTS blocks, and a thread loads KD_UNROLL strides before it bins any.
+constexpr uint32_t KD_PARTS = 2, KD_UNROLL = 4;
 template <typename T>
 __global__ __launch_bounds__(256) void k_kdist(const T *__restrict__ x, uint32_t n, uint32_t d,
                                                uint32_t offset, double avg, double bw,
@@ -212,27 +217,41 @@ __global__ __launch_bounds__(256) void k_kdist(const T *__restrict__ x, uint32_t
   __shared__ uint32_t lh[NBUCKETS];
   for (uint32_t b = threadIdx.x; b < NBUCKETS; b += 256) lh[b] = 0;
   __syncthreads();
-  const uint64_t tid = (uint64_t)blockIdx.x * offset;
+  const uint64_t tid = (uint64_t)(blockIdx.x / KD_PARTS) * offset;
+  const uint32_t part = blockIdx.x % KD_PARTS;
   if (tid < n) {
     T q[KNN_MAXDIM];
 #pragma unroll
     for (uint32_t j = 0; j < KNN_MAXDIM; j++) q[j] = j < d ? x[(size_t)j * n + tid] : (T)0;
-    for (uint64_t i = 0; i < n; i += 256 * 8) {
-      const uint64_t p = i + threadIdx.x;
-      if (p >= n) break;
-      double run = 0;
+    const uint64_t step = 2048ull * KD_PARTS;
+    for (uint64_t i0 = 2048ull * part; i0 < n; i0 += step * KD_UNROLL) {
+      T c[KD_UNROLL][KNN_MAXDIM];
+      bool ok[KD_UNROLL];
 #pragma unroll
-      for (uint32_t j = 0; j < KNN_MAXDIM; j++) {
-        if (j < d) {
-          T df = x[(size_t)j * n + p] - q[j];
-          T sq = df * df;
-          run += (double)sq;
-        }
+      for (uint32_t u = 0; u < KD_UNROLL; u++) {
+        const uint64_t p = i0 + step * u + threadIdx.x;
+        ok[u] = p < n;
+#pragma unroll
+        for (uint32_t j = 0; j < KNN_MAXDIM; j++) c[u][j] = (ok[u] && j < d) ? x[(size_t)j * n + p] : (T)0;
       }
-      run = sqrt(run);
-      if (run < avg && p != tid) {
-        uint32_t bucket = (uint32_t)(run / bw);
-        if (bucket < NBUCKETS) atomicAdd(&lh[bucket], 8u);
+#pragma unroll
+      for (uint32_t u = 0; u < KD_UNROLL; u++) {
+        const uint64_t p = i0 + step * u + threadIdx.x;
+        if (!ok[u]) continue;
+        double run = 0;
+#pragma unroll
+        for (uint32_t j = 0; j < KNN_MAXDIM; j++) {
+          if (j < d) {
+            T df = c[u][j] - q[j];
+            T sq = df * df;
+            run += (double)sq;
+          }
+        }
+        run = sqrt(run);
+        if (run < avg && p != tid) {
+          uint32_t bucket = (uint32_t)(run / bw);
+          if (bucket < NBUCKETS) atomicAdd(&lh[bucket], 8u);
+        }
       }
     }
   }
@@ -314,7 +333,7 @@ template <typename T>
 void kdist_hist_soa(const T *x, uint32_t n, uint32_t d, uint32_t offset, uint32_t nsample,
                     double avg, double bw, uint32_t *hist, hipStream_t s) {
   fill_u32(hist, 0u, NBUCKETS, s);
-  if (nsample) k_kdist<T><<<nsample, 256, 0, s>>>(x, n, d, offset, avg, bw, hist);
+  if (nsample) k_kdist<T><<<nsample * KD_PARTS, 256, 0, s>>>(x, n, d, offset, avg, bw, hist);
 }
 
 #define INST(T)                                                                            \
