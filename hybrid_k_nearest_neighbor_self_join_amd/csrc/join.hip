@@ -158,9 +158,22 @@ __global__ __launch_bounds__(256) void k_cell_bytes(const uint32_t *cellMax, uin
     sp += __shfl_xor(sp, o, 64);
     sc += __shfl_xor(sc, o, 64);
   }
-  if ((threadIdx.x & 63) == 0 && (sp || sc)) {
-    atomicAdd(out, sp);
-    atomicAdd(out + 1, sc);
+  // the block's four wave sums, then one pair of atomics per block (every wave adding to the
+  // same two counters serialised them)
+  __shared__ unsigned long long ws[2][4];
+  const uint32_t w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    ws[0][w] = sp;
+    ws[1][w] = sc;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const unsigned long long bp = ws[0][0] + ws[0][1] + ws[0][2] + ws[0][3];
+    const unsigned long long bc = ws[1][0] + ws[1][1] + ws[1][2] + ws[1][3];
+    if (bp || bc) {
+      atomicAdd(out, bp);
+      atomicAdd(out + 1, bc);
+    }
   }
 }
 
@@ -350,7 +363,7 @@ void prefilter(const T *x, uint32_t n, const typename GridOf<T>::type &g, const 
 void cell_bytes(const uint32_t *cellMax, uint32_t ncell, unsigned long long *out, hipStream_t s) {
   if (ncell) {
     const uint32_t blocks = (ncell + 255) / 256;
-    k_cell_bytes<<<blocks < 2048 ? blocks : 2048, 256, 0, s>>>(cellMax, ncell, out);
+    k_cell_bytes<<<blocks < 512 ? blocks : 512, 256, 0, s>>>(cellMax, ncell, out);
   }
 }
 void cell_of(const uint32_t *flags, const uint32_t *pos, uint32_t n, uint32_t *cellOf, hipStream_t s) {

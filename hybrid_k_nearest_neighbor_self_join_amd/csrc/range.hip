@@ -294,21 +294,30 @@ __device__ __forceinline__ uint32_t tile_count(const JoinArgs &a) {
   return a.ntilesDev ? *a.ntilesDev : a.ntiles;
 }
 
+// Per-wave counters of a persistent kernel (wave-uniform), added to the global counters once
+// when the wave runs out of tiles: per-tile atomics on the same three addresses serialise.
+struct WaveTally {
+  uint32_t fails = 0;
+  unsigned long long bytes = 0, cand = 0;
+};
+
 template <typename T, int D>
-__device__ __forceinline__ void tile_accounting(const JoinArgs &a, const uint4 &tl, uint32_t lane,
-                                                bool ok, bool failedLane, uint64_t tileCand,
-                                                uint32_t K1) {
+__device__ __forceinline__ void tile_accounting(WaveTally &w, const uint4 &tl, bool ok, bool failedLane,
+                                                uint64_t tileCand, uint32_t K1) {
   const uint64_t fm = __ballot(failedLane);
   const uint64_t om = __ballot(ok);
-  if (lane == 0) {
-    if (fm) atomicAdd(a.failCount, (uint32_t)__popcll(fm));
-    // compulsory HBM bytes of the tile's queries: ids in / failure flags out and the finished
-    // rows (the query cells' candidate blocks are counted by the pre-filter)
-    unsigned long long b = (unsigned long long)tl.z * 8ull +
-                           (unsigned long long)__popcll(om) * K1 * (4ull + sizeof(T));
-    atomicAdd(a.candCount + 1, b);
-    atomicAdd(a.candCount, (unsigned long long)tileCand * tl.z);
-  }
+  w.fails += (uint32_t)__popcll(fm);
+  // compulsory HBM bytes of the tile's queries: ids in / failure flags out and the finished
+  // rows (the query cells' candidate blocks are counted by the pre-filter)
+  w.bytes += (unsigned long long)tl.z * 8ull + (unsigned long long)__popcll(om) * K1 * (4ull + sizeof(T));
+  w.cand += (unsigned long long)tileCand * tl.z;
+}
+
+__device__ __forceinline__ void flush_tally(const JoinArgs &a, const WaveTally &w, uint32_t lane) {
+  if (lane != 0) return;
+  if (w.fails) atomicAdd(a.failCount, w.fails);
+  if (w.bytes) atomicAdd(a.candCount + 1, w.bytes);
+  if (w.cand) atomicAdd(a.candCount, w.cand);
 }
 
 // row_chord for one lane's own row (no wave reduction): narrows [l0, h0] to the chord of the
@@ -490,6 +499,7 @@ __global__ __launch_bounds__(64) void k_range_select(JoinArgs a, G g0) {
   const size_t n = a.n;
   const uint32_t NT = tile_count(a);
 
+  WaveTally tally;
   for (;;) {
     const uint32_t t = next_tile(a.tileCounter, lane);
     if (t >= NT) break;
@@ -785,8 +795,9 @@ __global__ __launch_bounds__(64) void k_range_select(JoinArgs a, G g0) {
     }
     const uint64_t om = __ballot(L.active && ovfMine);
     if (lane == 0 && om) atomicAdd(a.overflowCount, (uint32_t)__popcll(om));
-    tile_accounting<T, D>(a, tl, lane, okMine, L.active && !okMine && !ovfMine, tileCand, K1);
+    tile_accounting<T, D>(tally, tl, okMine, L.active && !okMine && !ovfMine, tileCand, K1);
   }
+  flush_tally(a, tally, lane);
 }
 
 // ====================================================================== collect kernel
@@ -890,6 +901,7 @@ __global__ __launch_bounds__(64) void k_range_collect(JoinArgs a, G g0) {
   const T *__restrict__ x = reinterpret_cast<const T *>(a.x);
   const size_t n = a.n;
   const uint32_t NT = tile_count(a);
+  WaveTally tally;
   for (;;) {
     const uint32_t t = next_tile(a.tileCounter, lane);
     if (t >= NT) break;
@@ -952,12 +964,11 @@ __global__ __launch_bounds__(64) void k_range_collect(JoinArgs a, G g0) {
       nok += ok ? 1u : 0u;
       br_sync();  // the collector's LDS is reused by the next query
     }
-    if (lane == 0) {
-      if (tl.z - nok) atomicAdd(a.failCount, tl.z - nok);
-      atomicAdd(a.candCount + 1, (unsigned long long)tl.z * 8ull + (unsigned long long)nok * K1 * (4ull + sizeof(T)));
-      atomicAdd(a.candCount, cand);
-    }
+    tally.fails += tl.z - nok;
+    tally.bytes += (unsigned long long)tl.z * 8ull + (unsigned long long)nok * K1 * (4ull + sizeof(T));
+    tally.cand += cand;
   }
+  flush_tally(a, tally, lane);
 }
 
 // ====================================================================== heap kernel
@@ -1044,6 +1055,7 @@ __global__ __launch_bounds__(64) void k_range_heap(JoinArgs a, G g0) {
   const size_t n = a.n;
   const uint32_t NT = tile_count(a);
 
+  WaveTally tally;
   for (;;) {
     const uint32_t t = next_tile(a.tileCounter, lane);
     if (t >= NT) break;
@@ -1118,8 +1130,9 @@ __global__ __launch_bounds__(64) void k_range_heap(JoinArgs a, G g0) {
     } else if (L.active) {
       a.failed[L.q] = 1;
     }
-    tile_accounting<T, D>(a, tl, lane, ok, L.active && !ok, tileCand, K1);
+    tile_accounting<T, D>(tally, tl, ok, L.active && !ok, tileCand, K1);
   }
+  flush_tally(a, tally, lane);
 }
 
 constexpr uint32_t MAX_LDS_BYTES = 65536;
