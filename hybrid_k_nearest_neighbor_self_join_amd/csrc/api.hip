@@ -307,11 +307,13 @@ struct Pipeline {
   // activeFailed[q] != 0 (all when null), grouped by cell into tiles of <= 64.
   PassResult pass(const uint8_t *activeFailed, uint32_t nq, uint8_t *failed, int32_t *dIds,
                   T *dDists, unsigned long long *ctr, Timer &tK);
+  T uniform_radius() const { return radiusAll > (T)0 ? radiusAll : (T)g.eps; }
   uint32_t make_tiles(const uint8_t *activeFailed, uint32_t *&act, uint4 *&tiles, uint8_t *failed,
                       unsigned long long *cellAcc);
   void finish(PassResult &r, uint32_t *counters, unsigned long long *ctr, uint32_t prefailed);
   unsigned long long accBytes = 0;  // accumulated pre-check accounting bytes
-  const T *radius = nullptr;  // per-query radii (learned schedule), null = eps
+  const T *radius = nullptr;  // per-query radii (learned schedule), null = radiusAll
+  T radiusAll = (T)0;         // one radius for every query (plain schedule); 0 = the grid's eps
   uint32_t lastSpan = 1;
   uint32_t lastActive = 0;   // queries left in tiles after the pre-check
   double avgReach = 1.0;     // typical search radius in cells (tile-span cost model)
@@ -476,7 +478,7 @@ uint32_t Pipeline<T>::make_tiles(const uint8_t *activeFailed, uint32_t *&act, ui
     // own-block pre-check (exact early failures) + per-cell candidate accounting
     uint32_t *cellMax = buf<uint32_t>(S_CELLMAX, (size_t)ncell + 1);
     fill_u32(cellMax, 0u, ncell, s);
-    prefilter<T>(x, n, g, radius, denseStart, sortedIds, activeFailed, failed, K1, cellOf, cellMax,
+    prefilter<T>(xs, n, g, radius, uniform_radius(), denseStart, sortedIds, activeFailed, failed, K1, cellOf, cellMax,
                  flags, s);
     if (cellAcc) cell_bytes(cellMax, ncell, cellAcc, s);
     all = false;
@@ -567,6 +569,7 @@ PassResult Pipeline<T>::pass(const uint8_t *activeFailed, uint32_t nq, uint8_t *
   a.d = d;
   a.k1 = K1;
   a.radius = radius;
+  a.radiusAll = (double)uniform_radius();
   a.outIds = dIds;
   a.outDists = dDists;
   a.failed = failed;
@@ -894,19 +897,6 @@ static double plain_max_reach(uint32_t nd) {
   return e > 0.0 ? e : (nd <= 2 ? 2.0 : 1.0);
 }
 
-template <typename T>
-static void fill_value(T *a, T v, size_t n, hipStream_t s) {
-  if (sizeof(T) == 4) {
-    uint32_t b;
-    std::memcpy(&b, &v, 4);
-    fill_u32(reinterpret_cast<uint32_t *>(a), b, n, s);
-  } else {
-    uint64_t b;
-    std::memcpy(&b, &v, 8);
-    fill_u64(reinterpret_cast<uint64_t *>(a), b, n, s);
-  }
-}
-
 // The plain method: BETA eps0 estimate and the failed-query loop of main.cu:305-430.
 template <typename T>
 static void run_plain(Pipeline<T> &P, double beta, knnjoin_stats &stats, uint8_t *failed,
@@ -954,8 +944,7 @@ static void run_plain(Pipeline<T> &P, double beta, knnjoin_stats &stats, uint8_t
     P.build_index(hmn, hmx, w);
     curW = w;
   };
-  T *rad = P.template buf<T>(S_RADIUS, n);
-  P.radius = rad;
+  P.radius = nullptr;  // every query searches the same radius, passed by value
   tPh.start();
   T eps = eps0;
   build(eps);
@@ -993,7 +982,7 @@ static void run_plain(Pipeline<T> &P, double beta, knnjoin_stats &stats, uint8_t
       }
     }
     tPh.start();
-    fill_value<T>(rad, eps, n, s);
+    P.radiusAll = eps;
     P.avgReach = (double)eps / (double)curW;
     PassResult pr = P.pass(first ? nullptr : failed, nq, failed, dIds, dDists, ctr, tK);
     launches++;
@@ -1034,7 +1023,7 @@ static void run_plain(Pipeline<T> &P, double beta, knnjoin_stats &stats, uint8_t
   stats.candidates = hctr[0];
   stats.algo_bytes = hctr[1] + P.accBytes;
   stats.eps_final = (float)eps;
-  P.radius = nullptr;
+  P.radiusAll = (T)0;
   stats.iterations = numIter;
   stats.reindexes = reindexes;
   stats.kernel_launches = launches;

@@ -93,12 +93,12 @@ __global__ void k_compact_blocks(const uint32_t *cellStart, const uint32_t *flag
 }
 
 // Per active query (sorted position p): the number of points in its own cell block (radius
-// r = its own or the grid's eps, cells [coord(x-r) rounded down, coord(x+r) rounded up]) from
+// r = its own, or rAll for every query, cells [coord(x-r) rounded down, coord(x+r) rounded up]) from
 // the dense cell-start table.  Fewer than K+1 points there means the query cannot succeed
 // at this radius: it is marked failed without a distance evaluation.  The per-cell maximum of
 // those counts feeds the algorithmic-bytes accounting (each query cell's block once).
 template <typename T, typename G>
-__global__ void k_prefilter(const T *__restrict__ x, uint32_t n, G g, const T *radius,
+__global__ void k_prefilter(const T *__restrict__ xs, uint32_t n, G g, const T *radius, T rAll,
                             const uint32_t *__restrict__ dense, const uint32_t *__restrict__ sortedIds,
                             const uint8_t *activeSrc, uint8_t *failed, uint32_t k1,
                             const uint32_t *cellOf, uint32_t *cellMax, uint32_t *flags) {
@@ -109,11 +109,11 @@ __global__ void k_prefilter(const T *__restrict__ x, uint32_t n, G g, const T *r
     flags[p] = 0;
     return;
   }
-  const T r = radius ? radius[q] : (T)g.eps;
+  const T r = radius ? radius[q] : rAll;
   const T eps = (T)g.eps;
   uint32_t lo[KNN_MAX_INDEXED], hi[KNN_MAX_INDEXED], cur[KNN_MAX_INDEXED];
   for (uint32_t j = 0; j < g.nd; j++) {
-    const T xj = x[(size_t)j * n + q];
+    const T xj = xs[(size_t)j * n + p];  // cell-sorted copy: the query's own coordinates, coalesced
     lo[j] = cell_coord<T>(next_down(xj - r), (T)g.minArr[j], eps, g.nCells[j]);
     hi[j] = cell_coord<T>(next_up(xj + r), (T)g.minArr[j], eps, g.nCells[j]);
     cur[j] = lo[j];
@@ -352,13 +352,13 @@ void compact_blocks(const uint32_t *cellStart, const uint32_t *flags, const uint
 }
 
 template <typename T>
-void prefilter(const T *x, uint32_t n, const typename GridOf<T>::type &g, const T *radius,
-               const uint32_t *dense, const uint32_t *sortedIds, const uint8_t *activeSrc,
+void prefilter(const T *xs, uint32_t n, const typename GridOf<T>::type &g, const T *radius,
+               T rAll, const uint32_t *dense, const uint32_t *sortedIds, const uint8_t *activeSrc,
                uint8_t *failed, uint32_t k1, const uint32_t *cellOf, uint32_t *cellMax,
                uint32_t *flags, hipStream_t s) {
   if (n)
     k_prefilter<T, typename GridOf<T>::type><<<(n + 255) / 256, 256, 0, s>>>(
-        x, n, g, radius, dense, sortedIds, activeSrc, failed, k1, cellOf, cellMax, flags);
+        xs, n, g, radius, rAll, dense, sortedIds, activeSrc, failed, k1, cellOf, cellMax, flags);
 }
 void cell_bytes(const uint32_t *cellMax, uint32_t ncell, unsigned long long *out, hipStream_t s) {
   if (ncell) {
@@ -414,7 +414,7 @@ void sum_rows(const double *rowSum, uint32_t n, double *partial, double *out, hi
   template void gather_rows<T>(const T *, const uint32_t *, uint32_t, uint32_t, const DimPerm &, T *, \
                                hipStream_t);                                                   \
   template void row_sums<T>(const T *, uint32_t, uint32_t, double *, hipStream_t);          \
-  template void prefilter<T>(const T *, uint32_t, const GridOf<T>::type &, const T *,         \
+  template void prefilter<T>(const T *, uint32_t, const GridOf<T>::type &, const T *, T,      \
                              const uint32_t *, const uint32_t *, const uint8_t *, uint8_t *,  \
                              uint32_t, const uint32_t *, uint32_t *, uint32_t *, hipStream_t);
 INST(float)

@@ -33,7 +33,6 @@ void minmax_soa(const T *x, uint32_t n, uint32_t nd, T *mn, T *mx, T *scratch, h
 void iota_u32(uint32_t *a, uint32_t n, hipStream_t s);
 void fill_u8(uint8_t *a, uint8_t v, size_t n, hipStream_t s);
 void fill_u32(uint32_t *a, uint32_t v, size_t n, hipStream_t s);
-void fill_u64(uint64_t *a, uint64_t v, size_t n, hipStream_t s);
 
 // --------------------------------------------------------------- estimator (estimator.hip)
 // main.cu:2521-2615: exact sequential variance (T arithmetic) per dim over rows 0,100,...
@@ -106,8 +105,8 @@ void compact_blocks(const uint32_t *cellStart, const uint32_t *flags, const uint
 // block holds fewer than k1 points get failed[q] = 1 and flag 0.  activeSrc null = all
 // active.  cellOf/cellMax (optional) collect per-cell candidate counts for accounting.
 template <typename T>
-void prefilter(const T *x, uint32_t n, const typename GridOf<T>::type &g, const T *radius,
-               const uint32_t *dense, const uint32_t *sortedIds, const uint8_t *activeSrc,
+void prefilter(const T *xs, uint32_t n, const typename GridOf<T>::type &g, const T *radius,
+               T rAll, const uint32_t *dense, const uint32_t *sortedIds, const uint8_t *activeSrc,
                uint8_t *failed, uint32_t k1, const uint32_t *cellOf, uint32_t *cellMax,
                uint32_t *flags, hipStream_t s);
 void cell_bytes(const uint32_t *cellMax, uint32_t ncell, unsigned long long *out, hipStream_t s);
@@ -137,7 +136,8 @@ struct JoinArgs {
   const uint4 *tiles;    // {block, first active position, count, first tile of its block}
   uint32_t ntiles;       // tile count; with ntilesDev: the persistent grid size (blocks)
   uint32_t n, d, k1;
-  const void *radius;    // per-query search radius (T); null = the grid's eps for all
+  const void *radius;    // per-query search radius (T); null = radiusAll for all
+  double radiusAll;      // the one search radius when radius is null (a T value)
   int32_t *outIds;
   void *outDists;
   uint8_t *failed;       // per point: 1 = failed this pass

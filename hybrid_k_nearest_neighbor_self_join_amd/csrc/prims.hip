@@ -316,10 +316,6 @@ __global__ void k_fill_u32(uint32_t *a, uint32_t v, size_t n) {
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) a[i] = v;
 }
-__global__ void k_fill_u64(uint64_t *a, uint64_t v, size_t n) {
-  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) a[i] = v;
-}
 }  // namespace
 
 void iota_u32(uint32_t *a, uint32_t n, hipStream_t s) {
@@ -330,9 +326,6 @@ void fill_u8(uint8_t *a, uint8_t v, size_t n, hipStream_t s) {
 }
 void fill_u32(uint32_t *a, uint32_t v, size_t n, hipStream_t s) {
   if (n) k_fill_u32<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(a, v, n);
-}
-void fill_u64(uint64_t *a, uint64_t v, size_t n, hipStream_t s) {
-  if (n) k_fill_u64<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(a, v, n);
 }
 
 }  // namespace knn

@@ -138,7 +138,7 @@ __device__ __forceinline__ void tile_setup(const JoinArgs &a, const G &g, const 
     L.qc[j] = x[(size_t)j * n + L.q];
     L.qd[j] = (double)L.qc[j];
   }
-  const T r = a.radius ? reinterpret_cast<const T *>(a.radius)[L.q] : (T)g.eps;
+  const T r = a.radius ? reinterpret_cast<const T *>(a.radius)[L.q] : (T)a.radiusAll;
   L.Tq = (double)r * (double)r * (1.0 - 0x1p-20);
   L.F = Num<T>::bound(L.Tq);
   const T eps = (T)g.eps;
@@ -921,7 +921,7 @@ __global__ __launch_bounds__(64) void k_range_collect(JoinArgs a, G g0) {
         qc[j] = x[(size_t)j * n + q];
         qd[j] = (double)qc[j];
       }
-      const T r = a.radius ? reinterpret_cast<const T *>(a.radius)[q] : eps;
+      const T r = a.radius ? reinterpret_cast<const T *>(a.radius)[q] : (T)a.radiusAll;
       const double Tq = (double)r * (double)r * (1.0 - 0x1p-20);
       const T F = Num<T>::bound(Tq);
       uint32_t lo[KNN_MAX_INDEXED], hi[KNN_MAX_INDEXED];
