@@ -144,12 +144,22 @@ __global__ void k_prefilter(const T *__restrict__ xs, uint32_t n, G g, const T *
 }
 
 // sum over cells of cellMax (candidate points of each active query cell's block) and the
-// number of active query cells; out[0] += points, out[1] += cells.  Grid-stride per thread,
-// then one wave reduction and one atomic per wave.
+// number of active query cells; out[0] += points, out[1] += cells.  Grid-stride per thread
+// over 16-byte vectors (cellMax comes from an aligned allocation), then a wave reduction and
+// one pair of atomics per block.
 __global__ __launch_bounds__(256) void k_cell_bytes(const uint32_t *cellMax, uint32_t ncell,
                                                     unsigned long long *out) {
   unsigned long long sp = 0, sc = 0;
-  for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < ncell; c += gridDim.x * blockDim.x) {
+  const uint32_t stride = gridDim.x * blockDim.x, gid = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t nv = ncell / 4;
+  const uint4 *v4 = reinterpret_cast<const uint4 *>(cellMax);
+  for (uint32_t i = gid; i < nv; i += stride) {
+    const uint4 v = v4[i];
+    sp += (unsigned long long)v.x + (unsigned long long)v.y + (unsigned long long)v.z +
+          (unsigned long long)v.w;
+    sc += (v.x ? 1ull : 0ull) + (v.y ? 1ull : 0ull) + (v.z ? 1ull : 0ull) + (v.w ? 1ull : 0ull);
+  }
+  for (uint32_t c = 4 * nv + gid; c < ncell; c += stride) {
     const uint32_t v = cellMax[c];
     sp += v;
     sc += v ? 1ull : 0ull;
@@ -362,8 +372,8 @@ void prefilter(const T *xs, uint32_t n, const typename GridOf<T>::type &g, const
 }
 void cell_bytes(const uint32_t *cellMax, uint32_t ncell, unsigned long long *out, hipStream_t s) {
   if (ncell) {
-    const uint32_t blocks = (ncell + 255) / 256;
-    k_cell_bytes<<<blocks < 512 ? blocks : 512, 256, 0, s>>>(cellMax, ncell, out);
+    const uint32_t blocks = (ncell / 4 + 255) / 256 + 1;
+    k_cell_bytes<<<blocks < 2048 ? blocks : 2048, 256, 0, s>>>(cellMax, ncell, out);
   }
 }
 void cell_of(const uint32_t *flags, const uint32_t *pos, uint32_t n, uint32_t *cellOf, hipStream_t s) {
