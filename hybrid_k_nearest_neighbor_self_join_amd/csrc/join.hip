@@ -237,12 +237,22 @@ __global__ __launch_bounds__(256) void k_sum_chunks(const double *rowSum, uint32
   for (uint32_t i = threadIdx.x; i < cnt; i += blockDim.x) buf[i] = rowSum[lo + i];
   __syncthreads();
   if (threadIdx.x == 0) {
+    // rows in order; the next eight values are read while the current eight are added
     double s = 0.0;
     uint32_t r = 0;
-    for (; r + 8 <= cnt; r += 8) {
+    if (cnt >= 8) {
       double v[8];
 #pragma unroll
-      for (int u = 0; u < 8; u++) v[u] = buf[r + u];
+      for (int u = 0; u < 8; u++) v[u] = buf[u];
+      for (r = 8; r + 8 <= cnt; r += 8) {
+        double w[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) w[u] = buf[r + u];
+#pragma unroll
+        for (int u = 0; u < 8; u++) s += v[u];
+#pragma unroll
+        for (int u = 0; u < 8; u++) v[u] = w[u];
+      }
 #pragma unroll
       for (int u = 0; u < 8; u++) s += v[u];
     }
@@ -305,10 +315,19 @@ __global__ void k_row_sums_direct(const T *__restrict__ dists, uint32_t n, uint3
   rowSum[r] = s;
 }
 
+// the chunk sums in order by one thread; sixteen loads in flight ahead of the adds
 __global__ void k_sum_final(const double *partial, uint32_t m, double *out) {
   if (threadIdx.x || blockIdx.x) return;
   double s = 0.0;
-  for (uint32_t c = 0; c < m; c++) s += partial[c];
+  uint32_t c = 0;
+  for (; c + 16 <= m; c += 16) {
+    double v[16];
+#pragma unroll
+    for (int u = 0; u < 16; u++) v[u] = partial[c + u];
+#pragma unroll
+    for (int u = 0; u < 16; u++) s += v[u];
+  }
+  for (; c < m; c++) s += partial[c];
   *out = s;
 }
 
