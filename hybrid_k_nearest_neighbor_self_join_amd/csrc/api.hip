@@ -302,6 +302,7 @@ struct Pipeline {
   // 1 GiB of cell starts at most: filled and scanned once per build (~0.6 ms at 2^28 cells),
   // against a binary search of the cell table for every row of every block without it
   static constexpr uint64_t kDenseMaxCells = 1ull << 28;
+  static constexpr double kMinTiles = 8192.0;
   const uint32_t *denseStart = nullptr;
   uint32_t *cellOf = nullptr;  // compacted cell index of each sorted position
 
@@ -498,7 +499,9 @@ uint32_t Pipeline<T>::make_tiles(const uint8_t *activeFailed, uint32_t *&act, ui
   // of its lanes' blocks, about (span + 2) cells per row, for min(64, span * rho_active)
   // queries (rho_active = active queries per non-empty cell), so the cost per query is about
   // (span + 2) / max(1, min(64, span * rho_active)): wide windows pay off only when active
-  // queries are dense; sparse passes use one cell per window.
+  // queries are dense; sparse passes use one cell per window.  A pass with fewer than
+  // kMinTiles tiles leaves the persistent waves (about 2000) with one or two tiles each and
+  // ends with its slowest tile, so a span that gives fewer tiles pays in proportion.
   uint32_t A = 0;
   HIP_CHECK(hipMemcpyAsync(&A, pos + n, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
   HIP_CHECK(hipStreamSynchronize(s));
@@ -510,7 +513,8 @@ uint32_t Pipeline<T>::make_tiles(const uint8_t *activeFailed, uint32_t *&act, ui
     double best = 1e300;
     for (uint32_t w = 1; w <= spanMax; w *= 2) {
       const double q = std::max(1.0, std::min(64.0, w * rhoA));
-      const double cost = (w + 2.0 * avgReach) / q;
+      const double tilesEst = (double)A / q;
+      const double cost = (w + 2.0 * avgReach) / q * std::max(1.0, kMinTiles / tilesEst);
       if (cost < best * 0.97) {
         best = cost;
         span = w;
