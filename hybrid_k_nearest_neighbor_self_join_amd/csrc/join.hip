@@ -237,22 +237,12 @@ __global__ __launch_bounds__(256) void k_sum_chunks(const double *rowSum, uint32
   for (uint32_t i = threadIdx.x; i < cnt; i += blockDim.x) buf[i] = rowSum[lo + i];
   __syncthreads();
   if (threadIdx.x == 0) {
-    // rows in order; the next eight values are read while the current eight are added
     double s = 0.0;
     uint32_t r = 0;
-    if (cnt >= 8) {
+    for (; r + 8 <= cnt; r += 8) {
       double v[8];
 #pragma unroll
-      for (int u = 0; u < 8; u++) v[u] = buf[u];
-      for (r = 8; r + 8 <= cnt; r += 8) {
-        double w[8];
-#pragma unroll
-        for (int u = 0; u < 8; u++) w[u] = buf[r + u];
-#pragma unroll
-        for (int u = 0; u < 8; u++) s += v[u];
-#pragma unroll
-        for (int u = 0; u < 8; u++) v[u] = w[u];
-      }
+      for (int u = 0; u < 8; u++) v[u] = buf[r + u];
 #pragma unroll
       for (int u = 0; u < 8; u++) s += v[u];
     }
