@@ -1035,29 +1035,32 @@ static void run_plain(Pipeline<T> &P, double beta, knnjoin_stats &stats, uint8_t
   stats.kernel_launches = launches;
 }
 
-// Cells per radius-class upper bound R in the learned schedule's grids: 3 with <= 2 indexed
-// dims, 1 above; KNN_LEARNED_GRID_DIV (1..8) overrides it for tuning.
-static double learned_grid_div(uint32_t nd) {
+// Cells per radius-class upper bound R in the learned schedule's grids: with <= 2 indexed
+// dims 3 up to K+1 = 80 and 4 above (with the wider classes below), 1 with more indexed dims;
+// KNN_LEARNED_GRID_DIV (1..8) overrides it for tuning.
+static double learned_grid_div(uint32_t nd, uint32_t k1) {
   static const double env = []() {
     const char *v = std::getenv("KNN_LEARNED_GRID_DIV");
     const double x = v ? std::atof(v) : 0.0;
     return (x >= 1.0 && x <= 8.0) ? x : 0.0;
   }();
-  return env > 0.0 ? env : (nd <= 2 ? 3.0 : 1.0);
+  return env > 0.0 ? env : (nd <= 2 ? (k1 <= 80 ? 3.0 : 4.0) : 1.0);
 }
 
-// Quarter-octave bins per radius class: 8 (two octaves) with <= 2 indexed dims, 1 above.
-// Measured on MI355X, 2-D mixed/uniform sets of 1M-10M points, K = 8..128: two-octave classes
-// on cells of R/3 beat one-octave classes on cells of R/2 everywhere (10M mixed, K = 64:
-// 41.2 against 47.3 ms; K = 16: 44.0 against 61.5 ms), since each class costs an index
+// Quarter-octave bins per radius class with <= 2 indexed dims: 8 (two octaves) up to
+// K+1 = 80, 10 above; 1 with more indexed dims.  Measured on MI355X, 10M mixed 2-D points
+// (ms per join, classes of 8 bins on cells of R/3 against 10 bins on cells of R/4): K = 16
+// 28.9 / 28.2, K = 32 30.4 / 30.2, K = 64 39.4 / 40.4, K = 96 56.0 / 50.2, K = 128 66.5 /
+// 59.7, K = 256 128.1 / 117.6.  Two-octave classes on cells of R/3 beat one-octave classes
+// on cells of R/2 everywhere (K = 64: 41.2 against 47.3 ms), since each class costs an index
 // build over all N points.  KNN_LEARNED_CLASS_BINS (1..16) overrides it for tuning.
-static int learned_class_bins(uint32_t nd) {
+static int learned_class_bins(uint32_t nd, uint32_t k1) {
   static const int env = []() {
     const char *v = std::getenv("KNN_LEARNED_CLASS_BINS");
     const int x = v ? std::atoi(v) : 0;
     return (x >= 1 && x <= 16) ? x : 0;
   }();
-  return env > 0 ? env : (nd <= 2 ? 8 : 1);
+  return env > 0 ? env : (nd <= 2 ? (k1 <= 80 ? 8 : 10) : 1);
 }
 
 // The learned schedule: per-query initial radius from the MLP, then passes over radius
@@ -1096,8 +1099,8 @@ static void run_learned(Pipeline<T> &P, const knnjoin_params &p, knnjoin_stats &
   P.radius = radius;
 
   // ---- radius classes: the pending queries whose radius lies in the lowest occupied class
-  // of bins [c, c + learned_class_bins) (two octaves [R/4, R) with <= 2 indexed dims, a
-  // quarter octave above) are joined on a grid matched to R (cell width R / learned_grid_div);
+  // of bins [c, c + learned_class_bins) (two or two and a half octaves with <= 2 indexed dims,
+  // a quarter octave above) are joined on a grid matched to R (cell width R / learned_grid_div);
   // queries that fail grow their radius (x2, or x2^(1/4) with 3 or more indexed dims), which
   // moves them to a later class.  One index build per occupied class.
   unsigned long long *ctr = P.template buf<unsigned long long>(S_CTR, 64);
@@ -1139,7 +1142,7 @@ static void run_learned(Pipeline<T> &P, const knnjoin_params &p, knnjoin_stats &
     }
     // wide classes with <= 2 indexed dims; single bins above, where a cell up to 2x the radius
     // would cost up to 2^nd in block volume
-    const int cHi = std::min(c + learned_class_bins(nd) - 1, 63);
+    const int cHi = std::min(c + learned_class_bins(nd, P.K1) - 1, 63);
     uint32_t nq = 0;
     for (int b = c; b <= cHi; b++) nq += h[b];
     double R;
@@ -1163,7 +1166,7 @@ static void run_learned(Pipeline<T> &P, const knnjoin_params &p, knnjoin_stats &
     }
     // cell width R/3 with <= 2 indexed dims; above, R (<= 3 cells per dim), since every extra
     // cell of reach multiplies the rows of a block
-    T w = (T)(R / learned_grid_div(nd));
+    T w = (T)(R / learned_grid_div(nd, P.K1));
     if (!(w > (T)0)) w = F.wf;
     for (;;) {  // keep the grid within 2^30 cells
       typename GridOf<T>::type gt;
