@@ -130,18 +130,33 @@ __global__ __launch_bounds__(RADIX_BLOCK) void k_radix_hist(const K *keys, uint3
   counts[(size_t)t * ntiles + blockIdx.x] = h[t];
 }
 
-// Stable scatter: round r of a tile covers keys base + r*256 + [0,256), thread t takes key
-// t of the round, so tile order = key order.  Inside a round each wave ranks its lanes by
-// the 8-bit match mask (8 ballots); earlier waves' counts come from LDS.
+// Stable scatter in two steps.  Ranking: round r of a tile covers keys base + r*256 + [0,256),
+// thread t takes key t of the round, so tile order = key order; inside a round each wave
+// ranks its lanes by the 8-bit match mask (8 ballots), earlier waves' counts come from LDS,
+// and each key goes to its digit-ordered slot of the tile in LDS.  Writing: the tile's keys
+// leave LDS in slot order, so each digit's run (about 16 keys) is written to consecutive
+// addresses.  The tile's count of digit t is offsets[t][b+1] - offsets[t][b] of the exclusive
+// scan over [digit][tile] (offsets[256 * ntiles] = n).
 template <typename K>
 __global__ __launch_bounds__(RADIX_BLOCK) void k_radix_scatter(
     const K *__restrict__ kin, const uint32_t *__restrict__ vin, K *__restrict__ kout,
     uint32_t *__restrict__ vout, const uint32_t *__restrict__ offsets, uint32_t n,
     uint32_t ntiles, int shift) {
-  __shared__ uint32_t s_off[256];
+  __shared__ uint32_t s_off[256];   // global start of digit t for this tile
+  __shared__ uint32_t s_loc[256];   // tile-local start of digit t
+  __shared__ uint32_t s_run[256];   // next free tile-local slot of digit t
   __shared__ uint32_t s_wc[4][256];
+  __shared__ K s_key[RADIX_TILE];
+  __shared__ uint32_t s_val[RADIX_TILE];
   const int t = threadIdx.x, w = t >> 6;
-  s_off[t] = offsets[(size_t)t * ntiles + blockIdx.x];
+  const size_t oi = (size_t)t * ntiles + blockIdx.x;
+  const uint32_t g0 = offsets[oi];
+  const uint32_t cnt = offsets[oi + 1] - g0;
+  uint32_t total;
+  const uint32_t loc = block_exclusive_scan(cnt, &total);
+  s_off[t] = g0;
+  s_loc[t] = loc;
+  s_run[t] = loc;
   s_wc[0][t] = 0;
   s_wc[1][t] = 0;
   s_wc[2][t] = 0;
@@ -168,18 +183,25 @@ __global__ __launch_bounds__(RADIX_BLOCK) void k_radix_scatter(
     if (valid && rank == 0) s_wc[w][digit] = __popcll(m);
     __syncthreads();
     if (valid) {
-      uint32_t off = s_off[digit] + rank;
+      uint32_t off = s_run[digit] + rank;
       for (int ww = 0; ww < w; ww++) off += s_wc[ww][digit];
-      kout[off] = key;
-      vout[off] = val;
+      s_key[off] = key;
+      s_val[off] = val;
     }
     __syncthreads();
-    s_off[t] += s_wc[0][t] + s_wc[1][t] + s_wc[2][t] + s_wc[3][t];
+    s_run[t] += s_wc[0][t] + s_wc[1][t] + s_wc[2][t] + s_wc[3][t];
     s_wc[0][t] = 0;
     s_wc[1][t] = 0;
     s_wc[2][t] = 0;
     s_wc[3][t] = 0;
     __syncthreads();
+  }
+  for (uint32_t i = t; i < total; i += RADIX_BLOCK) {
+    const K key = s_key[i];
+    const uint32_t digit = (uint32_t)(key >> shift) & 255u;
+    const uint32_t g = s_off[digit] + (i - s_loc[digit]);
+    kout[g] = key;
+    vout[g] = s_val[i];
   }
 }
 
