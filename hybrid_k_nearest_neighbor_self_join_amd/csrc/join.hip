@@ -103,39 +103,56 @@ __global__ void k_prefilter(const T *__restrict__ xs, uint32_t n, G g, const T *
                             const uint8_t *activeSrc, uint8_t *failed, uint32_t k1,
                             const uint32_t *cellOf, uint32_t *cellMax, uint32_t *flags) {
   const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= n) return;
-  const uint32_t q = sortedIds[p];
-  if (activeSrc && !activeSrc[q]) {
-    flags[p] = 0;
-    return;
-  }
-  const T r = radius ? radius[q] : rAll;
-  const T eps = (T)g.eps;
-  uint32_t lo[KNN_MAX_INDEXED], hi[KNN_MAX_INDEXED], cur[KNN_MAX_INDEXED];
-  for (uint32_t j = 0; j < g.nd; j++) {
-    const T xj = xs[(size_t)j * n + p];  // cell-sorted copy: the query's own coordinates, coalesced
-    lo[j] = cell_coord<T>(next_down(xj - r), (T)g.minArr[j], eps, g.nCells[j]);
-    hi[j] = cell_coord<T>(next_up(xj + r), (T)g.minArr[j], eps, g.nCells[j]);
-    cur[j] = lo[j];
-  }
+  const uint32_t lane = threadIdx.x & 63;
+  const bool inRange = p < n;
+  const uint32_t q = inRange ? sortedIds[p] : 0u;
+  const bool active = inRange && !(activeSrc && !activeSrc[q]);
   uint32_t cnt = 0;
-  for (;;) {
-    uint64_t rowBase = g.keyBase;
-    for (uint32_t j = 1; j < g.nd; j++) rowBase += (uint64_t)cur[j] * g.mult[j];
-    cnt += dense[rowBase + hi[0] + 1] - dense[rowBase + lo[0]];
-    uint32_t j = 1;
-    while (j < g.nd) {
-      if (cur[j] < hi[j]) {
-        cur[j]++;
-        break;
-      }
+  if (active) {
+    const T r = radius ? radius[q] : rAll;
+    const T eps = (T)g.eps;
+    uint32_t lo[KNN_MAX_INDEXED], hi[KNN_MAX_INDEXED], cur[KNN_MAX_INDEXED];
+    for (uint32_t j = 0; j < g.nd; j++) {
+      const T xj = xs[(size_t)j * n + p];  // cell-sorted copy: the query's own coordinates, coalesced
+      lo[j] = cell_coord<T>(next_down(xj - r), (T)g.minArr[j], eps, g.nCells[j]);
+      hi[j] = cell_coord<T>(next_up(xj + r), (T)g.minArr[j], eps, g.nCells[j]);
       cur[j] = lo[j];
-      j++;
     }
-    if (j >= g.nd) break;
+    for (;;) {
+      uint64_t rowBase = g.keyBase;
+      for (uint32_t j = 1; j < g.nd; j++) rowBase += (uint64_t)cur[j] * g.mult[j];
+      cnt += dense[rowBase + hi[0] + 1] - dense[rowBase + lo[0]];
+      uint32_t j = 1;
+      while (j < g.nd) {
+        if (cur[j] < hi[j]) {
+          cur[j]++;
+          break;
+        }
+        cur[j] = lo[j];
+        j++;
+      }
+      if (j >= g.nd) break;
+    }
   }
-  if (cellMax) atomicMax(&cellMax[cellOf[p]], cnt);
-  if (cnt < k1) {
+  if (cellMax) {
+    // cellOf is non-decreasing in p, so a cell's lanes are one run of the wave: the run's
+    // maximum is gathered into its first lane (each lane takes the maximum over the next
+    // 1, 2, 4, ... lanes of its own cell), which issues the run's only atomic
+    const uint32_t c = inRange ? cellOf[p] : 0xFFFFFFFFu;
+    uint32_t m = active ? cnt : 0u;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t mo = __shfl_down(m, o, 64);
+      const uint32_t co = __shfl_down(c, o, 64);
+      if (lane + o < 64 && co == c && mo > m) m = mo;
+    }
+    const uint32_t cPrev = __shfl_up(c, 1, 64);
+    if (inRange && m && (lane == 0 || cPrev != c)) atomicMax(&cellMax[c], m);
+  }
+  if (!inRange) return;
+  if (!active) {
+    flags[p] = 0;
+  } else if (cnt < k1) {
     failed[q] = 1;
     flags[p] = 0;
   } else {
