@@ -91,24 +91,22 @@ def run_gpu(pts: np.ndarray, k: int, beta: float, mode: int, steps: int, warmup:
     return out
 
 
-def cpu_baseline(pts: np.ndarray, k: int, beta: float, sample_n: int):
-    from oracle import oracle
+def cpu_baseline(n: int, k: int, beta: float, queries: int):
+    """The oracle's OpenMP grid join on the host cores, in a child process started with
+    OMP_PROC_BIND=close and OMP_NUM_THREADS = the cores used (oracle/cpu_baseline.py: a
+    seeded subset of `queries` queries against all n points of the same set, 1 warm-up +
+    median of 3, projected to the full join)."""
+    import subprocess
 
-    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    sub = np.ascontiguousarray(pts[:sample_n])
-    t0 = time.perf_counter()
-    rc, ids, dists, st = oracle.knnjoin(sub, k, nd=2, beta=beta, nthreads=cores)
-    t = time.perf_counter() - t0
-    assert rc == 0
-    return {
-        "value": sample_n / t,
-        "unit": "queries/s",
-        "cores": cores,
-        "kind": "port",
-        "sample": (f"whole plain-method pipeline (reorder, BETA eps, grid, failed-query loop) on "
-                   f"rows 0..{sample_n - 1} of the same {len(pts)}-point set (a uniform random "
-                   f"subsample, the generator shuffles rows), k={k}, BETA={beta}: {t:.2f} s"),
-    }
+    cores = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
+    env = dict(os.environ, OMP_PROC_BIND="close", OMP_NUM_THREADS=str(cores))
+    cmd = [sys.executable, "-m", "oracle.cpu_baseline", "--n", str(n), "--k", str(k),
+           "--beta", str(beta), "--queries", str(queries), "--threads", str(cores)]
+    r = subprocess.run(cmd, cwd=ROOT, env=env, capture_output=True, text=True, timeout=900)
+    if r.returncode != 0:
+        log(f"[bench] cpu baseline failed: {r.stderr[-2000:]}")
+        return None
+    return json.loads(r.stdout.strip().splitlines()[-1])
 
 
 def load_traffic(workload_key: str):
@@ -250,7 +248,8 @@ def main():
     ap.add_argument("--mode", choices=["auto", "plain", "learned"], default="auto",
                     help="auto: time the plain BETA schedule and, when a trained radius model "
                          "ships with the package, the learned schedule; value = the faster")
-    ap.add_argument("--cpu-sample", type=int, default=1_000_000)
+    ap.add_argument("--cpu-queries", type=int, default=100_000,
+                    help="queries of the CPU baseline's subset join against the full set")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--suite", choices=["highdim", "batched", "brute"], default=None,
                     help="instead of the headline line: the 4-D/6-D cases, the 256-set batch, or "
@@ -337,6 +336,7 @@ def main():
             "num_indexed_dims": 2,
         },
         "modes": {m: {"queries_per_s": r["qps"], "ms_per_step": r["ms_per_step"],
+                      "ms_reference_total": r["stats"]["ms_reference_total"],
                       "iterations": r["stats"]["iterations"], "reindexes": r["stats"]["reindexes"],
                       "candidates": r["stats"]["candidates"],
                       "range_kernel_ms": r["stats"]["ms_range_kernel"],
@@ -358,6 +358,9 @@ def main():
         },
         "pipeline": {
             "ms_device_median": g["ms_device_median"],
+            # the reference's totalTime (main.cu:440-445): the pipeline less the first index build
+            "ms_reference_total": st["ms_reference_total"], "ms_index_first": st["ms_index_first"],
+            "host_syncs": st["host_syncs"],
             "iterations": st["iterations"], "reindexes": st["reindexes"],
             "eps0": st["eps0"], "eps_final": st["eps_final"],
             "ms_reorder": st["ms_reorder"], "ms_eps": st["ms_eps"], "ms_index": st["ms_index"],
@@ -366,8 +369,8 @@ def main():
         },
     }
     if not args.no_cpu:
-        log(f"[bench] cpu baseline on {args.cpu_sample} points ...")
-        line["cpu_baseline"] = cpu_baseline(pts, args.k, args.beta, args.cpu_sample)
+        log(f"[bench] cpu baseline: {args.cpu_queries} queries against {args.n} points ...")
+        line["cpu_baseline"] = cpu_baseline(args.n, args.k, args.beta, args.cpu_queries)
     else:
         line["cpu_baseline"] = None
     print(json.dumps(line), flush=True)

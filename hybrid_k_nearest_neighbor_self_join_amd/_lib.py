@@ -14,6 +14,8 @@ import numpy as np
 
 PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG, "libgpuknnselfjoin.so")
+# the DTYPE=double build: its KNNJoinPy takes double* (params.h:5)
+LIB_DOUBLE_PATH = os.path.join(PKG, "double", "libgpuknnselfjoin.so")
 
 KNN_OK, KNN_ERR_ARG, KNN_ERR_TOO_FEW, KNN_ERR_EPS_ZERO, KNN_ERR_NO_PROGRESS = 0, 1, 2, 3, 4
 KNN_ERR_HIP, KNN_ERR_NOMEM, KNN_ERR_NO_DEVICE, KNN_ERR_MODEL = 5, 6, 7, 8
@@ -28,7 +30,7 @@ STATUS_NAMES = {
 
 # every symbol include/gpuknnselfjoin.h declares
 EXPORTED_SYMBOLS = [
-    "KNNJoinPy", "KNNJoinPyDouble", "knnjoin_ex", "knnjoin_default_params",
+    "KNNJoinPy", "KNNJoinPyDouble", "knnjoin_py_dtype", "knnjoin_ex", "knnjoin_default_params",
     "knnjoin_last_status", "knnjoin_last_error", "knnjoin_release", "knnjoin_stage_reorder",
     "knnjoin_stage_estimate_eps", "knnjoin_stage_index", "knnjoin_stage_join_pass",
     "knnjoin_set_radius_model", "knnjoin_set_radius_model_dims", "knnjoin_radius_features",
@@ -76,6 +78,10 @@ class KnnStats(ctypes.Structure):
         ("overflow_queries", ctypes.c_uint64),
         ("eps_bucket", ctypes.c_uint32),
         ("brute_queries", ctypes.c_uint32),
+        ("ms_index_first", ctypes.c_double),
+        ("ms_reference_total", ctypes.c_double),
+        ("host_syncs", ctypes.c_uint32),
+        ("pad0", ctypes.c_uint32),
     ]
 
     def as_dict(self) -> dict:
@@ -116,6 +122,8 @@ def _setup(L):
     L.KNNJoinPy.restype = None
     L.KNNJoinPyDouble.argtypes = [vp, ctypes.c_uint, ctypes.c_uint, ctypes.c_uint, vp, vp]
     L.KNNJoinPyDouble.restype = None
+    L.knnjoin_py_dtype.argtypes = []
+    L.knnjoin_py_dtype.restype = i32
     L.knnjoin_ex.argtypes = [vp, i32, u32, u32, u32, ctypes.POINTER(KnnParams), vp, vp,
                              ctypes.POINTER(KnnStats)]
     L.knnjoin_ex.restype = i32

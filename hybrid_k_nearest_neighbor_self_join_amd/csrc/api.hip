@@ -41,6 +41,10 @@ struct Error : std::runtime_error {
                          std::string(#expr) + ": " + hipGetErrorString(e_));                \
   } while (0)
 
+// A kernel launch that could not start (launch configuration, LDS size, a prior fault) is
+// reported by the runtime's last-error state, not by the launch itself.
+#define LAUNCH_CHECK() HIP_CHECK(hipGetLastError())
+
 // ------------------------------------------------------------------ buffer cache
 enum Slot {
   S_PTS, S_X, S_XS, S_KEYS, S_KEYS_TMP, S_IDS, S_IDS_TMP, S_CELLKEYS, S_CELLSTART, S_FLAGS, S_POS,
@@ -281,6 +285,7 @@ struct Pipeline {
       gather_rows<T>(rowMajor, sortedIds, n, d, rowPerm, xs, s);
     else
       gather_soa<T>(x, sortedIds, n, d, xs, s);
+    LAUNCH_CHECK();
     sync();
     // dense cell-start table: O(1) row lookups when the grid is small enough
     denseStart = nullptr;
@@ -295,6 +300,7 @@ struct Pipeline {
       fill_u32(dense, 0u, (size_t)tc + 1, s);
       dense_cell_counts(cellKeys, cellStart, ncell, dense, s);
       scan_exclusive_u32(dense, dense, tc, scratch, s);
+      LAUNCH_CHECK();
       denseStart = dense;
     }
   }
@@ -593,6 +599,7 @@ PassResult Pipeline<T>::pass(const uint8_t *activeFailed, uint32_t nq, uint8_t *
   if (use_collect(nd, K1)) {
     tK.start();
     range_collect<T>(a, g, s);
+    LAUNCH_CHECK();
     r.kernel_ms = tK.stop();
     r.noverflow = 0;
     finish(r, counters, ctr, prefailed);
@@ -604,6 +611,7 @@ PassResult Pipeline<T>::pass(const uint8_t *activeFailed, uint32_t nq, uint8_t *
     a.heapScratch = hs ? g_cache.get(S_HEAP, hs) : nullptr;
     tK.start();
     range_topk<T>(a, g, s);
+    LAUNCH_CHECK();
     r.kernel_ms = tK.stop();
     r.noverflow = NT ? lastActive : 0;
     finish(r, counters, ctr, prefailed);
@@ -611,6 +619,7 @@ PassResult Pipeline<T>::pass(const uint8_t *activeFailed, uint32_t nq, uint8_t *
   }
   tK.start();
   range_select<T>(a, g, s);
+  LAUNCH_CHECK();
   r.kernel_ms = tK.stop();
   uint32_t hc[2];
   HIP_CHECK(hipMemcpyAsync(hc, counters, sizeof(hc), hipMemcpyDeviceToHost, s));
@@ -629,6 +638,7 @@ PassResult Pipeline<T>::pass(const uint8_t *activeFailed, uint32_t nq, uint8_t *
     a.heapScratch = hs ? g_cache.get(S_HEAP, hs) : nullptr;
     tK.start();
     range_topk<T>(a, g, s);
+    LAUNCH_CHECK();
     r.kernel_ms += tK.stop();
   }
   finish(r, counters, ctr, prefailed);
@@ -740,6 +750,7 @@ static uint32_t run_brute(Pipeline<T> &P, const uint8_t *sel, uint8_t *failed, i
   a.failed = failed;
   tK.start();
   brute_join<T>(a, s);
+  LAUNCH_CHECK();
   ms += tK.stop();
   if (P.verbose)
     printf("[knn] brute force: queries %u chunks %u x %u points, %u queries per wave\n", nq, pl.nchunks,
@@ -859,10 +870,12 @@ static int run_join(const T *pts_in, uint32_t n, uint32_t d, uint32_t k, const k
   double *csum = part + (n + 4095) / 4096 + 8;
   row_sums<T>(dDists, n, K1, rowSum, s);
   sum_rows(rowSum, n, part, csum, s);
+  LAUNCH_CHECK();
   double hcs = 0;
   HIP_CHECK(hipMemcpyAsync(&hcs, csum, sizeof(double), hipMemcpyDeviceToHost, s));
   HIP_CHECK(hipStreamSynchronize(s));
   stats.ms_total = tAll.stop();
+  stats.ms_reference_total = stats.ms_total - stats.ms_index_first;
   stats.checksum = hcs;
   stats.failed_final = 0;
 
@@ -955,6 +968,7 @@ static void run_plain(Pipeline<T> &P, double beta, knnjoin_stats &stats, uint8_t
   T eps = eps0;
   build(eps);
   stats.ms_index = tPh.stop();
+  stats.ms_index_first = stats.ms_index;
 
   unsigned long long *ctr = P.template buf<unsigned long long>(S_CTR, 64);
   fill_u32(reinterpret_cast<uint32_t *>(ctr), 0u, 16, s);
@@ -1095,6 +1109,7 @@ static void run_learned(Pipeline<T> &P, const knnjoin_params &p, knnjoin_stats &
   T *radius = P.template buf<T>(S_RADIUS, n);
   radius_kernel<T>(P.x, n, F.g, cnt, F.logk1, F.logdens, Wd, M.margin, (float)F.wf, radius, nullptr,
                    nullptr, s);
+  LAUNCH_CHECK();
   stats.ms_eps = tPh.stop();
   P.radius = radius;
 
@@ -1183,7 +1198,9 @@ static void run_learned(Pipeline<T> &P, const knnjoin_params &p, knnjoin_stats &
     if (w != curW) {
       tPh.start();
       P.build_index(hmn, hmx, w);
-      stats.ms_index += tPh.stop();
+      const double bms = tPh.stop();
+      stats.ms_index += bms;
+      if (curW == (T)0) stats.ms_index_first = bms;
       if (curW != (T)0) reindexes++;
       curW = w;
       if (stats.eps0 == 0.0f) stats.eps0 = (float)w;
@@ -1287,14 +1304,16 @@ int knnjoin_ex(const void *points, int dtype, uint32_t n, uint32_t ndim, uint32_
   return KNN_ERR_ARG;
 }
 
-void KNNJoinPy(float *dataset, unsigned int NUMPOINTS, unsigned int kNN, unsigned int NDIM,
-               int *outNearestNeighborTable, float *outNearestNeighborTableDistances) {
+void KNNJoinPy(KNN_PY_DTYPE *dataset, unsigned int NUMPOINTS, unsigned int kNN, unsigned int NDIM,
+               int *outNearestNeighborTable, KNN_PY_DTYPE *outNearestNeighborTableDistances) {
   knnjoin_params p;
   knnjoin_default_params(&p);
   p.num_indexed_dims = NDIM <= 6 ? NDIM : 6;
-  (void)knnjoin_ex(dataset, KNN_FLOAT32, NUMPOINTS, NDIM, kNN, &p, outNearestNeighborTable,
+  (void)knnjoin_ex(dataset, knnjoin_py_dtype(), NUMPOINTS, NDIM, kNN, &p, outNearestNeighborTable,
                    outNearestNeighborTableDistances, nullptr);
 }
+
+int knnjoin_py_dtype(void) { return sizeof(KNN_PY_DTYPE) == 8 ? KNN_FLOAT64 : KNN_FLOAT32; }
 
 void KNNJoinPyDouble(double *dataset, unsigned int NUMPOINTS, unsigned int kNN, unsigned int NDIM,
                      int *outNearestNeighborTable, double *outNearestNeighborTableDistances) {
@@ -1766,18 +1785,45 @@ struct BatchPlan {
     allocs.push_back(p);
     return reinterpret_cast<U *>(p);
   }
+  // The graph's kernel nodes hold pointers into `allocs`: it is destroyed first, and only
+  // once the device has finished every launch of it, then the buffers are freed.
   void drop_graph() {
-    if (exec) (void)hipGraphExecDestroy(exec);
-    if (graph) (void)hipGraphDestroy(graph);
+    if (exec || graph) HIP_CHECK(hipDeviceSynchronize());
+    hipGraphExec_t e = exec;
+    hipGraph_t g = graph;
     exec = nullptr;
     graph = nullptr;
+    if (e) HIP_CHECK(hipGraphExecDestroy(e));
+    if (g) HIP_CHECK(hipGraphDestroy(g));
   }
   void free_all() {
     drop_graph();
-    for (void *p : allocs) (void)hipFree(p);
-    allocs.clear();
+    if (!allocs.empty()) HIP_CHECK(hipDeviceSynchronize());
+    std::vector<void *> a;
+    a.swap(allocs);
     groups.clear();
     sig.clear();
+    for (void *p : a) HIP_CHECK(hipFree(p));
+  }
+};
+
+// Ends a stream capture that an exception interrupts, so that the stream leaves capture mode
+// and later calls can use it; the partial graph is discarded.
+struct CaptureGuard {
+  hipStream_t s;
+  bool open = true;
+  explicit CaptureGuard(hipStream_t st) : s(st) {}
+  hipGraph_t end() {
+    hipGraph_t g = nullptr;
+    open = false;
+    HIP_CHECK(hipStreamEndCapture(s, &g));
+    return g;
+  }
+  ~CaptureGuard() {
+    if (!open) return;
+    hipGraph_t g = nullptr;
+    if (hipStreamEndCapture(s, &g) == hipSuccess && g) (void)hipGraphDestroy(g);
+    (void)hipGetLastError();
   }
 };
 static BatchPlan g_batch;
@@ -2032,11 +2078,14 @@ static int run_batched(uint32_t nsets, const float *const *points, const uint32_
   if (B.useGraph && !B.exec) {
     HIP_CHECK(hipStreamSynchronize(s));
     HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    batch_body(B, s);
-    hipGraph_t gr = nullptr;
-    HIP_CHECK(hipStreamEndCapture(s, &gr));
-    B.graph = gr;
-    HIP_CHECK(hipGraphInstantiate(&B.exec, B.graph, nullptr, nullptr, 0));
+    {
+      CaptureGuard cap(s);
+      batch_body(B, s);
+      B.graph = cap.end();
+    }
+    hipGraphExec_t ex = nullptr;
+    HIP_CHECK(hipGraphInstantiate(&ex, B.graph, nullptr, nullptr, 0));
+    B.exec = ex;
   } else if (B.useGraph) {
     replay = true;
   }
@@ -2047,6 +2096,7 @@ static int run_batched(uint32_t nsets, const float *const *points, const uint32_
     HIP_CHECK(hipGraphLaunch(B.exec, s));
   } else {
     batch_body(B, s);
+    LAUNCH_CHECK();
   }
   // leftover failures: eager passes until none remain
   uint32_t eager = 0;
@@ -2059,6 +2109,7 @@ static int run_batched(uint32_t nsets, const float *const *points, const uint32_
       if (f) {
         any = true;
         batch_pass(B, G, s);
+        LAUNCH_CHECK();
       }
     }
     if (!any) break;
@@ -2171,5 +2222,11 @@ int knnjoin_batch_autotune(uint32_t nsets, const float *const *points, const uin
 }  // extern "C"
 
 namespace knn {
-static void release_batch() { g_batch.free_all(); }
+static void release_batch() {
+  try {
+    g_batch.free_all();
+  } catch (const Error &) {
+    g_batch = BatchPlan{};
+  }
+}
 }  // namespace knn

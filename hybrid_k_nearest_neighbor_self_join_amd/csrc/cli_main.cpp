@@ -7,8 +7,9 @@
 // header "x,y" followed by a newline also swallows the first number of the next line, as
 // in the reference).  N = floor(values / DIM); trailing values are dropped.
 //
-// Output: the reference's console lines ("[Verification] Total distance (without square
-// root): ..."), one line appended to gpu_stats.txt (main.cu:470-489), with --print-table
+// Output: the reference's console lines ("Time to join only", "[Verification] Total distance
+// (without square root): ..."), one line appended to gpu_stats.txt with the reference's
+// fields and tuple (main.cu:481-489), with --print-table
 // the files KNN_out.txt and KNN_out_distances.txt (printNeighborTable, main.cu:911-949), and
 // with --outlier-scores the file KNN_outlier_scores.txt (printOutlierScores,
 // main.cu:952-1067; float32 only).
@@ -152,8 +153,14 @@ int run(const char *fname, unsigned ndim, unsigned k, knnjoin_params &p, bool pr
     std::fprintf(stderr, "\nERROR (%d): %s\n", rc, knnjoin_last_error());
     return 1;
   }
-  std::printf("\nTime to join only: %f", st.ms_join / 1e3);
-  std::printf("\nTotal time (device pipeline): %f", st.ms_total / 1e3);
+  // main.cu:420-445: the join (every pass and re-index) and totalTime, which leaves out the
+  // first index build
+  std::printf("\nTime to join only: %f", (st.ms_join + st.ms_index - st.ms_index_first) / 1e3);
+  std::printf("\nTotal time (reference definition, first index build excluded): %f",
+              st.ms_reference_total / 1e3);
+  std::printf("\nDevice pipeline (first index build included): %f", st.ms_total / 1e3);
+  std::printf("\nSchedule: %s, iterations %u, re-indexes %u",
+              p.mode == KNN_MODE_LEARNED ? "learned" : "beta", st.iterations, st.reindexes);
   std::printf("\n[Verification] Total distance (without square root): %f\n", st.checksum);
   if (printTable) write_table<T>((unsigned)n, k1, ids.data(), d.data());
   if (outlierScores) {
@@ -162,13 +169,18 @@ int run(const char *fname, unsigned ndim, unsigned k, knnjoin_params &p, bool pr
     else
       std::fprintf(stderr, "--outlier-scores needs float32 data\n");
   }
+  // main.cu:481-489 (THREADMULTI -2, params.h:38): totalTime, file, K, eps bucket, total
+  // distance, then the build tuple.  SHORTCIRCUIT, THREADMULTI, MAXTHREADSPERPOINT and
+  // STATICTHREADSPERPOINT configure the reference's CUDA kernel and have no counterpart here;
+  // they are written with the reference's defaults (params.h:32-51) so that the line parses
+  // as the reference's does.
   std::ofstream gs("gpu_stats.txt", std::ios::app);
-  gs << st.ms_total / 1e3 << ", " << fname << ", KNN: " << k << ", Eps bucket: " << st.eps_bucket
+  gs << st.ms_reference_total / 1e3 << ", " << fname << ", KNN: " << k << ", Eps bucket: " << st.eps_bucket
      << ", Total dist: " << std::setprecision(9) << st.checksum << ", "
-     << "GPUNUMDIM/NUMINDEXEDDIM/REORDER/MODE/ITERATIONS/BETA/DTYPE(float/double): " << ndim << ", "
-     << (p.num_indexed_dims ? p.num_indexed_dims : (ndim < 6 ? ndim : 6)) << ", " << p.reorder << ", "
-     << (p.mode == KNN_MODE_LEARNED ? "learned" : "beta") << ", " << st.iterations << ", " << p.beta
-     << ", " << dtypeName << std::endl;
+     << "GPUNUMDIM/NUMINDEXEDDIM/REORDER/SHORTCIRCUIT/THREADMULTI/MAXTHREADSPERPOINT/DYNAMICTHRESHOLD/"
+        "STATICTHREADSPERPOINT/BETA/DTYPE(float/double): "
+     << ndim << ", " << (p.num_indexed_dims ? p.num_indexed_dims : (ndim < 6 ? ndim : 6)) << ", "
+     << p.reorder << ", 1, -2, 1024, N/A, 8, " << p.beta << ", " << dtypeName << std::endl;
   return 0;
 }
 

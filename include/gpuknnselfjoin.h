@@ -83,19 +83,38 @@ typedef struct {
   uint64_t overflow_queries; /* queries re-run by the heap kernel (staging overflow) */
   uint32_t eps_bucket;      /* k-distance histogram bucket of eps0 (GPU.cu:3484-3527) */
   uint32_t brute_queries;   /* queries whose rows the brute-force kernel wrote */
+  double ms_index_first;    /* the first index build (inside ms_index) */
+  double ms_reference_total; /* the reference's totalTime (main.cu:440-445): everything but the
+                               first index build, i.e. ms_total - ms_index_first (reorder,
+                               epsilon / radius, every pass and re-index, checksum) */
+  uint32_t host_syncs;      /* blocking host waits on the device during the join loop */
+  uint32_t pad0;
 } knnjoin_stats;
 
 /*
  * Replaces main.cu:514 `extern "C" void KNNJoinPy(DTYPE*, unsigned, unsigned, unsigned, int*, DTYPE*)`
- * with DTYPE=float, GPUNUMDIM=NUMINDEXEDDIM=NDIM runtime values (the reference fixes them at
- * compile time, params.h:5-9, Makefile:61-71), BETA=0.05 (params.h:58), REORDER=1.
+ * with GPUNUMDIM=NDIM and NUMINDEXEDDIM=min(NDIM, 6) as runtime values (the reference fixes
+ * them at compile time, params.h:5-9, Makefile:61-71), BETA=0.05 (params.h:58), REORDER=1.
+ * DTYPE is fixed when the library is built, as in the reference (params.h:5): float in
+ * libgpuknnselfjoin.so, double in double/libgpuknnselfjoin.so (`build.py --dtype double`,
+ * compiled with KNN_PY_DTYPE_DOUBLE), so the reference wrapper's DTYPE="double" binding
+ * (knnjoingpu.py:213-217) loads the double build under the same symbol.
  * Host pointers.  Void like the reference; errors are reported by knnjoin_last_status().
  * Rows of points never resolved (error exits) are left as -1 (main.cu:260-268).
  */
-void KNNJoinPy(float *dataset, unsigned int NUMPOINTS, unsigned int kNN, unsigned int NDIM,
-               int *outNearestNeighborTable, float *outNearestNeighborTableDistances);
+#ifdef KNN_PY_DTYPE_DOUBLE
+#define KNN_PY_DTYPE double
+#else
+#define KNN_PY_DTYPE float
+#endif
+void KNNJoinPy(KNN_PY_DTYPE *dataset, unsigned int NUMPOINTS, unsigned int kNN, unsigned int NDIM,
+               int *outNearestNeighborTable, KNN_PY_DTYPE *outNearestNeighborTableDistances);
 
-/* Same as KNNJoinPy for DTYPE=double (the wrapper's DTYPE="double", knnjoingpu.py:213-217). */
+/* DTYPE of this build's KNNJoinPy: KNN_FLOAT32 or KNN_FLOAT64. */
+int knnjoin_py_dtype(void);
+
+/* KNNJoinPy with DTYPE=double in either build (the wrapper's DTYPE="double",
+ * knnjoingpu.py:213-217, against a library whose KNNJoinPy is float). */
 void KNNJoinPyDouble(double *dataset, unsigned int NUMPOINTS, unsigned int kNN, unsigned int NDIM,
                      int *outNearestNeighborTable, double *outNearestNeighborTableDistances);
 

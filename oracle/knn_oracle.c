@@ -10,6 +10,7 @@
 #include "knn_oracle.h"
 
 #include <math.h>
+#include <omp.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -77,32 +78,52 @@ static void orc_heap_sort(orc_heap *h) {
 }
 
 /* Stable LSD radix sort of ids 0..n-1 by 64-bit key (the stable sort_by_key of
- * GPU.cu:3959 on a value array initialised to 0..n-1, GPU.cu:3952). */
+ * GPU.cu:3959 on a value array initialised to 0..n-1, GPU.cu:3952).  8-bit digits; each
+ * OpenMP thread counts and then scatters its own contiguous slice, and a slice's
+ * destinations follow the slices before it for every digit, so the order of equal keys is
+ * the input order (stable) with any thread count. */
 static void orc_radix_sort_pairs(const uint64_t *key, uint32_t *ids, uint64_t *sorted_keys,
                                  uint32_t n) {
   uint64_t maxk = 0;
   for (uint32_t i = 0; i < n; i++)
     if (key[i] > maxk) maxk = key[i];
-  uint64_t *ka = (uint64_t *)malloc(sizeof(uint64_t) * (size_t)n);
-  uint64_t *kb = (uint64_t *)malloc(sizeof(uint64_t) * (size_t)n);
+  uint64_t *ka = (uint64_t *)malloc(sizeof(uint64_t) * (size_t)(n ? n : 1));
+  uint64_t *kb = (uint64_t *)malloc(sizeof(uint64_t) * (size_t)(n ? n : 1));
   uint32_t *va = ids;
-  uint32_t *vb = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)n);
+  uint32_t *vb = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)(n ? n : 1));
   for (uint32_t i = 0; i < n; i++) {
     ka[i] = key[i];
     va[i] = i;
   }
   int passes = 0;
   while (passes < 8 && (maxk >> (8 * passes)) != 0) passes++;
+  const int nt = n >= (1u << 16) ? omp_get_max_threads() : 1;
+  size_t *cnt = (size_t *)malloc(sizeof(size_t) * 256 * (size_t)nt);
   for (int p = 0; p < passes; p++) {
-    uint32_t cnt[257];
-    memset(cnt, 0, sizeof(cnt));
-    int sh = 8 * p;
-    for (uint32_t i = 0; i < n; i++) cnt[((ka[i] >> sh) & 255) + 1]++;
-    for (int b = 0; b < 256; b++) cnt[b + 1] += cnt[b];
-    for (uint32_t i = 0; i < n; i++) {
-      uint32_t pos = cnt[(ka[i] >> sh) & 255]++;
-      kb[pos] = ka[i];
-      vb[pos] = va[i];
+    const int sh = 8 * p;
+#pragma omp parallel num_threads(nt)
+    {
+      const int t = omp_get_thread_num();
+      const size_t lo = (size_t)n * t / nt, hi = (size_t)n * (t + 1) / nt;
+      size_t *c = cnt + 256 * (size_t)t;
+      memset(c, 0, sizeof(size_t) * 256);
+      for (size_t i = lo; i < hi; i++) c[(ka[i] >> sh) & 255]++;
+#pragma omp barrier
+#pragma omp single
+      {
+        size_t run = 0;
+        for (int b = 0; b < 256; b++)
+          for (int u = 0; u < nt; u++) {
+            const size_t v = cnt[256 * (size_t)u + b];
+            cnt[256 * (size_t)u + b] = run;
+            run += v;
+          }
+      }
+      for (size_t i = lo; i < hi; i++) {
+        const size_t pos = c[(ka[i] >> sh) & 255]++;
+        kb[pos] = ka[i];
+        vb[pos] = va[i];
+      }
     }
     uint64_t *tk = ka;
     ka = kb;
@@ -111,6 +132,7 @@ static void orc_radix_sort_pairs(const uint64_t *key, uint32_t *ids, uint64_t *s
     va = vb;
     vb = tv;
   }
+  free(cnt);
   if (va != ids) {
     memcpy(ids, va, sizeof(uint32_t) * (size_t)n);
     free(va);

@@ -49,6 +49,9 @@ typedef struct {
   uint32_t reindexes;       /* epsilon growths */
   uint64_t candidates;      /* distance evaluations in the join passes */
   double checksum;          /* sum over rows of sum (float)(d*d), main.cu:452 */
+  double ms_prep;           /* wall ms: dimension reorder, epsilon estimate, min/max */
+  double ms_index;          /* wall ms: every index build (the first and each re-index) */
+  double ms_passes;         /* wall ms: the join passes and the failed-list updates */
 } orc_stats;
 
 /* ---- float (DTYPE float) ---- */

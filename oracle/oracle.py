@@ -42,6 +42,9 @@ class Stats(ctypes.Structure):
         ("reindexes", ctypes.c_uint32),
         ("candidates", ctypes.c_uint64),
         ("checksum", ctypes.c_double),
+        ("ms_prep", ctypes.c_double),
+        ("ms_index", ctypes.c_double),
+        ("ms_passes", ctypes.c_double),
     ]
 
 
@@ -188,21 +191,28 @@ def cell_coord(x: float, mn: float, eps: float, ncell: int) -> int:
 def knnjoin(pts: np.ndarray, k: int, nd: int | None = None, beta: float = 0.05,
             nthreads: int | None = None, subset: np.ndarray | None = None):
     """Plain method (reorder, BETA epsilon, grid, failed-query loop).  Returns
-    (rc, ids[N,K+1] int32, dists[N,K+1], Stats).  Rows of points outside `subset` stay -1."""
+    (rc, ids, dists, Stats): ids/dists are [N, K+1] for the whole join; with `subset` only
+    the subset's queries are joined and ids/dists are [len(subset), K+1], row i = point
+    subset[i]."""
     pts = np.ascontiguousarray(pts)
     n, d = pts.shape
     nd = d if nd is None else nd
+    st = Stats()
+    if subset is not None:
+        if pts.dtype != np.float32:
+            raise ValueError("subset joins take float32 points")
+        sub = np.ascontiguousarray(subset, np.uint32)
+        # rows are written only for the subset: untouched pages of np.empty are never mapped
+        ids = np.empty((n, k + 1), np.int32)
+        dists = np.empty((n, k + 1), np.float32)
+        rc = lib().orc_knnjoin_subset_f32(_p(pts), n, d, k, nd, beta, _threads(nthreads),
+                                          _p(sub), len(sub), _p(ids), _p(dists), ctypes.byref(st))
+        return rc, ids[sub].copy(), dists[sub].copy(), st
     ids = np.full((n, k + 1), -1, np.int32)
     dists = np.zeros((n, k + 1), pts.dtype)
-    st = Stats()
     if pts.dtype == np.float32:
-        if subset is None:
-            rc = lib().orc_knnjoin_f32(_p(pts), n, d, k, nd, beta, _threads(nthreads), _p(ids),
-                                       _p(dists), ctypes.byref(st))
-        else:
-            sub = np.ascontiguousarray(subset, np.uint32)
-            rc = lib().orc_knnjoin_subset_f32(_p(pts), n, d, k, nd, beta, _threads(nthreads),
-                                              _p(sub), len(sub), _p(ids), _p(dists), ctypes.byref(st))
+        rc = lib().orc_knnjoin_f32(_p(pts), n, d, k, nd, beta, _threads(nthreads), _p(ids),
+                                   _p(dists), ctypes.byref(st))
     else:
         rc = lib().orc_knnjoin_f64(_p(pts), n, d, k, nd, beta, _threads(nthreads), _p(ids),
                                    _p(dists), ctypes.byref(st))

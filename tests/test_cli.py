@@ -98,8 +98,17 @@ def test_cli_join_writes_reference_files(tmp_path, oracle):
     dl = (tmp_path / "KNN_out_distances.txt").read_text().splitlines()[1:]
     gd = np.array([[float(t) for t in ln.split(", ") if t] for ln in dl])
     assert np.allclose(gd, d, rtol=1e-5, atol=1e-6)  # written with 6 significant digits
-    stats = (tmp_path / "gpu_stats.txt").read_text().strip().split(", ")
+    # the reference's gpu_stats.txt line (main.cu:481-489): totalTime, file, K, eps bucket,
+    # total distance, then the build tuple under the reference's own label
+    line = (tmp_path / "gpu_stats.txt").read_text().strip()
+    tup = line.rsplit(": ", 1)[1]
+    stats = line.split(", ")
+    assert float(stats[0]) > 0.0
     assert stats[1] == str(path) and stats[2] == "KNN: 5"
+    assert stats[3].startswith("Eps bucket: ") and stats[4].startswith("Total dist: ")
+    assert ("GPUNUMDIM/NUMINDEXEDDIM/REORDER/SHORTCIRCUIT/THREADMULTI/MAXTHREADSPERPOINT/"
+            "DYNAMICTHRESHOLD/STATICTHREADSPERPOINT/BETA/DTYPE(float/double): ") in line
+    assert tup.split(", ") == ["2", "2", "1", "1", "-2", "1024", "N/A", "8", "0.025", "float"]
 
 
 @pytest.mark.gpu

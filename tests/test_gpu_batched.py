@@ -113,3 +113,44 @@ def test_autotune_keeps_rows_and_picks_a_candidate(tmp_path):
     n = [len(p) for p in pts]
     d = [p.shape[1] for p in pts]
     assert batched.tuned(n, d, ks, cache=str(tmp_path / "tune.json")) == (tq, gb)
+
+
+def test_batched_256_sets_full_size_against_oracle(oracle):
+    # the batched workload at its stated size: 256 sets, sizes log-uniform in [1,000, 200,000],
+    # d in {2,3,4,6}, k in {8,32}; captured once, replayed, every set checked against FP64
+    # brute force on 300 seeded queries
+    sets = synthetic.batched_sets(256)
+    pts, ks = [s for s, _ in sets], [k for _, k in sets]
+    print(f"batched 256 sets, {sum(len(p) for p in pts)} points", flush=True)
+    ids0, dists0, st0 = batched.knnjoin_batched(pts, ks)
+    ids, dists, st = batched.knnjoin_batched(pts, ks)
+    assert st.graph_replayed == 1 and st.groups == 4
+    for i, (p_, k) in enumerate(sets):
+        assert np.array_equal(ids[i], ids0[i]) and np.array_equal(bits(dists[i]), bits(dists0[i])), i
+        q = np.random.default_rng(1000 + i).choice(len(p_), min(300, len(p_)), replace=False)
+        perm, _ = oracle.reorder(p_)
+        bi, bd = oracle.bruteforce(np.ascontiguousarray(p_[:, perm]), k, queries=q)
+        assert np.array_equal(ids[i][q], bi), f"set {i}: n={len(p_)} d={p_.shape[1]} k={k}"
+        assert np.array_equal(bits(dists[i][q]), bits(bd)), f"set {i}"
+
+
+def test_batched_new_set_shapes_after_graph_replays():
+    # a captured graph replayed twice, then sets of other shapes (the plan and its graph are
+    # rebuilt), then the first shapes again: every call's rows equal the single-set joins
+    a = _sets(16, 21, lo=1000, hi=30000)
+    b = _sets(9, 22, lo=2000, hi=40000)
+    pa, ka = [s for s, _ in a], [k for _, k in a]
+    pb, kb = [s for s, _ in b], [k for _, k in b]
+    for rep in range(3):
+        ids, dists, st = batched.knnjoin_batched(pa, ka)
+        assert st.graph_replayed == (1 if rep else 0)
+    _check_against_single(a, ids, dists)
+    ids, dists, st = batched.knnjoin_batched(pb, kb)
+    assert st.graph_replayed == 0
+    _check_against_single(b, ids, dists)
+    ids, dists, st = batched.knnjoin_batched(pb, kb)
+    assert st.graph_replayed == 1
+    _check_against_single(b, ids, dists)
+    ids, dists, st = batched.knnjoin_batched(pa, ka)
+    assert st.graph_replayed == 0
+    _check_against_single(a, ids, dists)

@@ -1,12 +1,17 @@
 """The headline workload at full size: 10,000,000 mixed-density 2-D float32 points (70 % uniform
-over [0,360)x[-90,90], 30 % in 64 Gaussian clusters), k = 16, 64, 128, BETA=0.025.  The oracle cannot
-join 10M points in seconds, so the table is checked through properties that do not depend on
-size: the learned and the plain schedule give the same table bit for bit (results do not
-depend on the radius schedule, main.cu:305-430), every row holds the query itself and K+1
-distinct ids in ascending (FP64 squared distance, id) order, the checksum equals the oracle's
-checksum of the table (main.cu:452-455), and 1000 seeded random queries equal the oracle's
-FP64 brute force.  The reported FP32 distances follow that order up to their own rounding: two
-neighbours whose FP64 keys differ by less than a few FP32 ulps may swap their FP32 values."""
+over [0,360)x[-90,90], 30 % in 64 Gaussian clusters), k = 16, 64, 128, BETA=0.025.
+
+Checks per k:
+  * the learned and the plain schedule give the same table bit for bit (results do not depend
+    on the radius schedule, main.cu:305-430);
+  * 100,000 seeded random queries equal the oracle's plain-method grid join restricted to
+    those queries (the same reorder, BETA epsilon, grids over all 10M points and
+    failed-query loop, main.cu:109-510), ids and distance bits;
+  * 1,000 seeded random queries equal the oracle's FP64 brute force over all 10M points;
+  * over the whole table: every row holds the query itself, ids are in range, FP32 distances
+    are non-decreasing up to their rounding (two neighbours whose FP64 keys differ by less
+    than a few FP32 ulps may swap their FP32 values), and the checksum equals the oracle's
+    checksum of the table (main.cu:452-455)."""
 import numpy as np
 import pytest
 
@@ -16,7 +21,7 @@ pytestmark = pytest.mark.gpu
 
 
 @pytest.mark.parametrize("k", [16, 64, 128])
-def test_full_size_tables_agree_and_match_bruteforce(oracle, k):
+def test_full_size_tables_agree_with_oracle(oracle, k):
     if not learned_radius.has_model(learned_radius.register_default_models(), 2, 2):
         pytest.skip("no 2-D radius model")
     n = 10_000_000
@@ -35,21 +40,20 @@ def test_full_size_tables_agree_and_match_bruteforce(oracle, k):
     assert il.min() >= 0 and il.max() < n
     assert oracle.checksum(dl) == sl.checksum
 
-    # 100k random rows: distinct ids, ascending by (FP64 key in indexed-dim order, id)
-    perm = list(sl.perm[:2])
-    pr = np.ascontiguousarray(pts[:, perm]).astype(np.float64)
-    rng = np.random.default_rng(0x5EED0003)
-    rows = rng.choice(n, 100_000, replace=False)
-    ids = il[rows].astype(np.int64)
-    assert np.all(np.diff(np.sort(ids, axis=1), axis=1) != 0)
-    dx = pr[ids, 0] - pr[rows, 0][:, None]
-    dy = pr[ids, 1] - pr[rows, 1][:, None]
-    key = dx * dx + dy * dy
-    step_up = key[:, 1:] > key[:, :-1]
-    tie = (key[:, 1:] == key[:, :-1]) & (ids[:, 1:] > ids[:, :-1])
-    assert np.all(step_up | tie)
+    # 100k seeded queries: the oracle's grid join of exactly those queries
+    rng = np.random.default_rng(0x5EED0003 + k)
+    sub = np.sort(rng.choice(n, 100_000, replace=False)).astype(np.uint32)
+    print(f"oracle subset join: {len(sub)} queries", flush=True)
+    rc, oi, od, ost = oracle.knnjoin(pts, k, nd=2, beta=0.025, subset=sub)
+    assert rc == 0
+    assert list(ost.perm[:2]) == list(sl.perm[:2])
+    bad = np.flatnonzero((il[sub] != oi).any(axis=1))
+    assert bad.size == 0, f"{bad.size} of {len(sub)} rows differ, first query {sub[bad[0]]}"
+    assert np.array_equal(dl[sub].view(np.uint32), od.view(np.uint32))
+    del oi, od
 
-    del pr, dx, dy, key
+    # 1000 seeded queries: FP64 brute force over every point
+    perm = list(sl.perm[:2])
     q = rng.choice(n, 1000, replace=False).astype(np.uint32)
     bi, bd = oracle.bruteforce(np.ascontiguousarray(pts[:, perm]), k, queries=q)
     assert np.array_equal(il[q], bi)

@@ -1,5 +1,7 @@
 """GPU parity tests: the HIP library (through its C ABI) against the CPU oracle and the golden
 fixtures.  Bar: bit-exact ids, FP32 distance bits, index arrays and estimator outputs."""
+import os
+
 import numpy as np
 import pytest
 
@@ -130,6 +132,30 @@ def test_wrapper_flat_interface(oracle):
     assert np.array_equal(ids, i2.reshape(-1)) and np.array_equal(bits(d), bits(d2.reshape(-1)))
     idsd, dd = knnjoingpu.knnjoin(pts.astype(np.float64).reshape(-1), 5, 2, "double", verbose=True)
     assert dd.dtype == np.float64 and idsd.shape == ids.shape
+
+
+def test_reference_double_binding_of_the_double_build(oracle):
+    # knnjoingpu.py:166-217 with DTYPE="double", unmodified: the library found by name in its
+    # directory, KNNJoinPy bound with double ndpointer argtypes, against the DTYPE=double build
+    import numpy.ctypeslib as npct
+    from ctypes import c_double, c_int, c_uint
+
+    libknnjoin = npct.load_library("libgpuknnselfjoin.so", os.path.dirname(_lib.LIB_DOUBLE_PATH))
+    array_1d_double = npct.ndpointer(dtype=c_double, ndim=1, flags="CONTIGUOUS")
+    array_1d_int = npct.ndpointer(dtype=c_int, ndim=1, flags="CONTIGUOUS")
+    libknnjoin.KNNJoinPy.argtypes = [array_1d_double, c_uint, c_uint, c_uint, array_1d_int, array_1d_double]
+    pts = synthetic.mixed_box(30000, seed=6).astype(np.float64)
+    KNN, NDIM = 7, 2
+    numPoints = len(pts)
+    c_DATASET = np.ascontiguousarray(pts.reshape(-1), dtype=c_double)
+    ret_outNeighborTable = np.zeros(numPoints * (KNN + 1), dtype=c_int)
+    ret_outNeighborTableDistances = np.zeros(numPoints * (KNN + 1), dtype=c_double)
+    libknnjoin.KNNJoinPy(c_DATASET, c_uint(numPoints), c_uint(KNN), c_uint(NDIM), ret_outNeighborTable,
+                         ret_outNeighborTableDistances)
+    rc, i2, d2, _ = oracle.knnjoin(pts, KNN, nd=2, beta=0.05)
+    assert rc == 0
+    assert np.array_equal(ret_outNeighborTable, i2.reshape(-1))
+    assert np.array_equal(bits(ret_outNeighborTableDistances), bits(d2.reshape(-1)))
 
 
 def test_errors_and_edges(oracle):

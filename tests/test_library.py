@@ -29,3 +29,14 @@ def test_default_params_match_reference_defaults():
     prm = _lib.default_params()
     assert prm.beta == 0.05 and prm.reorder == 1 and prm.num_indexed_dims == 0
     assert prm.mode == _lib.KNN_MODE_BETA
+
+
+def test_double_build_exports_everything_with_a_double_knnjoinpy():
+    # the DTYPE=double build (params.h:5): same symbols, KNNJoinPy takes double*
+    assert os.path.exists(_lib.LIB_DOUBLE_PATH)
+    L = ctypes.CDLL(_lib.LIB_DOUBLE_PATH)
+    for name in declared_symbols():
+        assert hasattr(L, name), name
+    L.knnjoin_py_dtype.restype = ctypes.c_int
+    assert L.knnjoin_py_dtype() == _lib.KNN_FLOAT64
+    assert _lib.lib().knnjoin_py_dtype() == _lib.KNN_FLOAT32

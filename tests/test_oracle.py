@@ -269,6 +269,40 @@ def test_oracle_mlp_forward_and_gradient_closed_form(oracle):
         assert abs(G[p] - fd) <= 1e-4 * max(1.0, abs(fd)), (p, G[p], fd)
 
 
+def test_oracle_adam_first_two_steps_closed_form(oracle):
+    # Adam (Kingma & Ba 2015) with beta1 0.9, beta2 0.999, eps 1e-8 and bias correction: at
+    # step 1 every weight moves by exactly -lr g1 / (|g1| + eps) (m^ = g1, v^ = g1^2); at step 2
+    # by -lr m^2 / (sqrt(v^2) + eps) with m2 = 0.9 (0.1 g1) + 0.1 g2, v2 = 0.999 (0.001 g1^2) +
+    # 0.001 g2^2, m^2 = m2 / (1 - 0.9^2), v^2 = v2 / (1 - 0.999^2).  A dropped bias correction
+    # scales step 1 by 0.1 / sqrt(0.001); swapped betas change step 2.  One batch of all the
+    # samples per epoch, so the epoch shuffle only reorders the batch's terms (rounding).
+    rng = np.random.default_rng(7)
+    W0 = rng.normal(0, 0.2, 5057).astype(np.float32)
+    X = rng.normal(0, 1, (64, 12)).astype(np.float32)
+    Y = rng.normal(0, 1, 64).astype(np.float32)
+    lr = 1e-3
+    W1 = oracle.mlp_train(W0, X, Y, 1, 64, lr, 11)
+    W2 = oracle.mlp_train(W0, X, Y, 2, 64, lr, 11)
+    g1, _ = oracle.mlp_grad(W0, X, Y)
+    g2, _ = oracle.mlp_grad(W1, X, Y)
+    g1, g2 = g1.astype(np.float64), g2.astype(np.float64)
+    eps = 1e-8
+    step1 = -lr * g1 / (np.abs(g1) + eps)
+    m2 = 0.9 * (0.1 * g1) + 0.1 * g2
+    v2 = 0.999 * (0.001 * g1 * g1) + 0.001 * g2 * g2
+    step2 = -lr * (m2 / (1 - 0.9 ** 2)) / (np.sqrt(v2 / (1 - 0.999 ** 2)) + eps)
+    big1 = np.abs(g1) > 1e-4 * np.abs(g1).max()
+    big2 = np.abs(g2) > 1e-4 * np.abs(g2).max()
+    assert big1.sum() > 3000 and big2.sum() > 3000
+    d1 = W1.astype(np.float64) - W0
+    d2 = W2.astype(np.float64) - W1
+    # float32 weights: a step of lr is resolved to about 2^-24 |W| / lr of itself
+    tol = 2e-3 * lr + 4 * 2.0 ** -24 * np.abs(W0.astype(np.float64))
+    assert np.all(np.abs(d1 - step1)[big1] <= tol[big1])
+    assert np.all(np.abs(d1) <= lr * (1 + 1e-3) + tol)
+    assert np.all(np.abs(d2 - step2)[big1 & big2] <= (tol + 2e-3 * np.abs(step2))[big1 & big2])
+
+
 def test_oracle_features_definition(oracle):
     # lattice with known counts: 3 points per unit cell in a 10x10 box
     g = np.stack(np.meshgrid(np.arange(10), np.arange(10)), -1).reshape(-1, 2).astype(np.float32)
