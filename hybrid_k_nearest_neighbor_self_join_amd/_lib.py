@@ -30,7 +30,7 @@ STATUS_NAMES = {
 
 # every symbol include/gpuknnselfjoin.h declares
 EXPORTED_SYMBOLS = [
-    "KNNJoinPy", "KNNJoinPyDouble", "knnjoin_py_dtype", "knnjoin_ex", "knnjoin_default_params",
+    "KNNJoinPy", "KNNJoinPyDouble", "knnjoin_py_dtype", "knnjoin_ex", "knnjoin_ex_noalloc", "knnjoin_default_params",
     "knnjoin_last_status", "knnjoin_last_error", "knnjoin_release", "knnjoin_stage_reorder",
     "knnjoin_stage_estimate_eps", "knnjoin_stage_index", "knnjoin_stage_join_pass",
     "knnjoin_set_radius_model", "knnjoin_set_radius_model_dims", "knnjoin_radius_features",
@@ -127,6 +127,9 @@ def _setup(L):
     L.knnjoin_ex.argtypes = [vp, i32, u32, u32, u32, ctypes.POINTER(KnnParams), vp, vp,
                              ctypes.POINTER(KnnStats)]
     L.knnjoin_ex.restype = i32
+    L.knnjoin_ex_noalloc.argtypes = [vp, i32, u32, u32, u32, ctypes.POINTER(KnnParams), vp, vp,
+                                     ctypes.POINTER(KnnStats)]
+    L.knnjoin_ex_noalloc.restype = i32
     L.knnjoin_default_params.argtypes = [ctypes.POINTER(KnnParams)]
     L.knnjoin_default_params.restype = None
     L.knnjoin_last_status.argtypes = []

@@ -53,10 +53,39 @@ enum Slot {
   S_BPOS, S_BSTART, S_CELLOF, S_CELLMAX, S_FCNT, S_RADIUS, S_RHIST, S_MODEL, S_TRAIN, S_OCC, S_ACTQ, S_BRUTE_PK, S_BRUTE_PI, S_BLIST, S_NSLOTS
 };
 
+// Slots of one no-allocation call: regions of the static device scratch array handed out in
+// order (a slot asked for more than its region holds gets a new region; the call's regions
+// are all released when it ends).
+struct Arena {
+  unsigned char *base = nullptr;
+  size_t cap = 0, used = 0;
+  void *p[S_NSLOTS] = {};
+  size_t sz[S_NSLOTS] = {};
+  static size_t round(size_t b) { return ((b ? b : 16) + 255) & ~size_t(255); }
+  // bytes of free scratch
+  size_t room() const { return cap - used; }
+  // scratch bytes that slot s would take to hold `bytes` (0 when its region holds them)
+  size_t extra(Slot s, size_t bytes) const { return sz[s] >= bytes && p[s] ? 0 : round(bytes); }
+  void *get(Slot s, size_t bytes) {
+    if (sz[s] >= bytes && p[s]) return p[s];
+    const size_t b = round(bytes);
+    if (b > room())
+      throw Error(KNN_ERR_NOMEM, "no-allocation scratch exhausted: " + std::to_string(used) + " of " +
+                                     std::to_string(cap) + " bytes in use, " + std::to_string(b) + " more needed");
+    p[s] = base + used;
+    sz[s] = b;
+    used += b;
+    return p[s];
+  }
+};
+// set for the duration of a no-allocation call (calls are serialised by g_mu)
+static Arena *g_arena = nullptr;
+
 struct Cache {
   void *p[S_NSLOTS] = {};
   size_t cap[S_NSLOTS] = {};
   void *get(Slot s, size_t bytes) {
+    if (g_arena) return g_arena->get(s, bytes);
     if (bytes == 0) bytes = 16;
     if (cap[s] < bytes) {
       if (p[s]) HIP_CHECK(hipFree(p[s]));
@@ -289,7 +318,16 @@ struct Pipeline {
     sync();
     // dense cell-start table: O(1) row lookups when the grid is small enough
     denseStart = nullptr;
-    if (totalCells + 1 <= kDenseMaxCells) {
+    // with the no-allocation scratch: only when the table, its scan scratch and the pass
+    // buffers still to come (active list, tiles, block tables, flags) fit
+    bool denseFits = totalCells + 1 <= kDenseMaxCells;
+    if (denseFits && g_arena) {
+      const size_t tcb = (size_t)totalCells + 1;
+      const size_t more = g_arena->extra(S_DENSE, tcb * 4) +
+                          g_arena->extra(S_SCRATCH, (scan_scratch_words((uint32_t)tcb) + 1024) * 4);
+      denseFits = more + 24ull * n + 24ull * ((size_t)ncell + 1) + (64u << 10) <= g_arena->room();
+    }
+    if (denseFits) {
       const uint32_t tc = (uint32_t)totalCells;
       const size_t need = scan_scratch_words(tc + 1) + 1024;
       if (need > scratchWords) {
@@ -440,16 +478,23 @@ static EpsResult estimate_eps(const T *x, uint32_t n, uint32_t d, uint32_t k, do
   return r;
 }
 
+// Event pairs created once (the first call that times anything) and reused by every call:
+// pair i serves the i-th timer of a call (calls are serialised by g_mu).
+constexpr int kTimerPairs = 8;
+static hipEvent_t g_events[kTimerPairs][2];
+static bool g_eventsReady = false;
+
 struct Timer {
   hipEvent_t a, b;
   hipStream_t s;
-  explicit Timer(hipStream_t st) : s(st) {
-    HIP_CHECK(hipEventCreate(&a));
-    HIP_CHECK(hipEventCreate(&b));
-  }
-  ~Timer() {
-    (void)hipEventDestroy(a);
-    (void)hipEventDestroy(b);
+  Timer(hipStream_t st, int pair) : s(st) {
+    if (!g_eventsReady) {
+      for (int i = 0; i < kTimerPairs; i++)
+        for (int j = 0; j < 2; j++) HIP_CHECK(hipEventCreate(&g_events[i][j]));
+      g_eventsReady = true;
+    }
+    a = g_events[pair][0];
+    b = g_events[pair][1];
   }
   void start() { HIP_CHECK(hipEventRecord(a, s)); }
   double stop() {
@@ -729,7 +774,10 @@ static uint32_t run_brute(Pipeline<T> &P, const uint8_t *sel, uint8_t *failed, i
     list = l;
   }
   if (nq == 0) return 0;
-  const BrutePlan pl = brute_plan(n, nq, P.K1);
+  // with the no-allocation scratch, partial tables within its free space
+  const uint64_t maxPart = g_arena ? (g_arena->room() > (1u << 20) ? (g_arena->room() - (1u << 20)) / 12 : 0)
+                                   : (1ull << 26);
+  const BrutePlan pl = brute_plan(n, nq, P.K1, std::min<uint64_t>(maxPart, 1ull << 26));
   if (pl.qw == 0) throw Error(KNN_ERR_ARG, "brute force: K+1 must be <= 2048");
   BruteArgs a{};
   a.x = P.x;
@@ -795,7 +843,7 @@ static int run_join(const T *pts_in, uint32_t n, uint32_t d, uint32_t k, const k
   P.scratchWords = std::max(scan_scratch_words(n + 1), radix_scratch_words(n)) + 1024;
   P.scratch = P.template buf<uint32_t>(S_SCRATCH, P.scratchWords);
 
-  Timer tAll(s), tPh(s), tK(s);
+  Timer tAll(s, 0), tPh(s, 1), tK(s, 2);
   auto wall0 = std::chrono::steady_clock::now();
 
   // ---- inputs
@@ -1240,11 +1288,42 @@ static void run_learned(Pipeline<T> &P, const knnjoin_params &p, knnjoin_stats &
   P.radius = nullptr;
 }
 
+// Where a no-allocation call takes its buffers: this device's copy of the static scratch array
+// (set up on the first such call on each device).
+static void *noalloc_scratch(size_t &bytes);
+// Scratch a join of n points of d dims takes at most, besides the optional grid tables (the
+// dense cell-start table and the learned schedule's feature grid, checked where they are
+// laid out): two coordinate copies, 64-bit keys and ids with their ping-pong copies, flags,
+// scans, cell and block tables, the active list and tiles, failure flags, radii, row sums.
+static size_t noalloc_bound(uint32_t n, uint32_t d, size_t tsz) {
+  const size_t perPoint = 2 * d * tsz + tsz + 16 + 8 + 8 + 8 + 4 + 4 + 4 + 16 + 3 + 8 + 24 + 4;
+  return (size_t)n * perPoint + (radix_scratch_words(n) + scan_scratch_words(n + 1) + 2048) * 4 +
+         avgdist_scratch_doubles(n) * 8 + (2u << 20);
+}
+
 template <typename T>
 static int guarded_join(const T *pts, uint32_t n, uint32_t d, uint32_t k, const knnjoin_params &p,
-                        int32_t *ids, T *dists, knnjoin_stats *st) {
+                        int32_t *ids, T *dists, knnjoin_stats *st, bool noalloc = false) {
   std::lock_guard<std::mutex> lk(g_mu);
+  Arena arena;
+  struct ArenaScope {
+    ~ArenaScope() { g_arena = nullptr; }
+  } scope;
   try {
+    if (noalloc) {
+      if (p.device_ptrs == 0) throw Error(KNN_ERR_ARG, "the no-allocation entry point takes device pointers only");
+      if (k + 1 > 85) throw Error(KNN_ERR_ARG, "the no-allocation entry point takes K+1 <= 85");
+      size_t cap = 0;
+      void *base = noalloc_scratch(cap);
+      const size_t need = noalloc_bound(n, d, sizeof(T));
+      if (need > cap)
+        throw Error(KNN_ERR_NOMEM, "no-allocation join of " + std::to_string(n) + " points x " + std::to_string(d) +
+                                       " dims needs up to " + std::to_string(need) + " bytes of scratch; the limit is " +
+                                       std::to_string(cap));
+      arena.base = reinterpret_cast<unsigned char *>(base);
+      arena.cap = cap;
+      g_arena = &arena;
+    }
     int rc = run_join<T>(pts, n, d, k, p, ids, dists, st);
     g_status = rc;
     g_err.clear();
@@ -1322,6 +1401,23 @@ void KNNJoinPyDouble(double *dataset, unsigned int NUMPOINTS, unsigned int kNN, 
   p.num_indexed_dims = NDIM <= 6 ? NDIM : 6;
   (void)knnjoin_ex(dataset, KNN_FLOAT64, NUMPOINTS, NDIM, kNN, &p, outNearestNeighborTable,
                    outNearestNeighborTableDistances, nullptr);
+}
+
+int knnjoin_ex_noalloc(const void *points, int dtype, uint32_t n, uint32_t ndim, uint32_t k,
+                       const knnjoin_params *params, int32_t *out_ids, void *out_dists,
+                       knnjoin_stats *stats) {
+  knnjoin_params p;
+  if (params) p = *params; else knnjoin_default_params(&p);
+  if (dtype == KNN_FLOAT32)
+    return knn::guarded_join<float>(reinterpret_cast<const float *>(points), n, ndim, k, p, out_ids,
+                                    reinterpret_cast<float *>(out_dists), stats, true);
+  if (dtype == KNN_FLOAT64)
+    return knn::guarded_join<double>(reinterpret_cast<const double *>(points), n, ndim, k, p, out_ids,
+                                     reinterpret_cast<double *>(out_dists), stats, true);
+  std::lock_guard<std::mutex> lk(knn::g_mu);
+  knn::g_status = KNN_ERR_ARG;
+  knn::g_err = "dtype must be KNN_FLOAT32 or KNN_FLOAT64";
+  return KNN_ERR_ARG;
 }
 
 int knnjoin_last_status(void) {
@@ -1493,7 +1589,7 @@ int knnjoin_stage_join_pass(const float *points, uint32_t n, uint32_t ndim, uint
     HIP_CHECK(hipMemcpyAsync(dD, out_dists, sizeof(float) * (size_t)n * K1, hipMemcpyHostToDevice, P.s));
     unsigned long long *ctr = P.buf<unsigned long long>(S_CTR, 64);
     fill_u32(reinterpret_cast<uint32_t *>(ctr), 0u, 16, P.s);
-    Timer tK(P.s);
+    Timer tK(P.s, 3);
     PassResult pr = P.pass(dAct, nq, dFailed, dIds, dD, ctr, tK);
     HIP_CHECK(hipMemcpyAsync(out_ids, dIds, sizeof(int32_t) * (size_t)n * K1, hipMemcpyDeviceToHost, P.s));
     HIP_CHECK(hipMemcpyAsync(out_dists, dD, sizeof(float) * (size_t)n * K1, hipMemcpyDeviceToHost, P.s));
@@ -2046,7 +2142,7 @@ static int run_batched(uint32_t nsets, const float *const *points, const uint32_
   if (tq > 64) throw Error(KNN_ERR_ARG, "tile_queries must be <= 64");
   if (grid > kBatchMaxGrid) throw Error(KNN_ERR_ARG, "grid_blocks must be <= 8192");
   const uint32_t passes = p.graph_passes ? p.graph_passes : 8;
-  Timer tAll(s);
+  Timer tAll(s, 4);
   auto w0 = std::chrono::steady_clock::now();
   if (B.sig != sig) {
     batch_build(B, nsets, n, ndim, k, p.num_indexed_dims, s);
@@ -2228,5 +2324,30 @@ static void release_batch() {
   } catch (const Error &) {
     g_batch = BatchPlan{};
   }
+}
+}  // namespace knn
+
+// ------------------------------------------------------------------ no-allocation scratch
+// The buffers of knnjoin_ex_noalloc: a static device array (each device holds its own copy
+// for the life of the process).  Host code uses only the device address that
+// hipGetSymbolAddress returns for the current device, never the array's own name.
+static __device__ __align__(256) unsigned char scratch[KNN_NOALLOC_SCRATCH_BYTES];
+
+namespace knn {
+constexpr int kMaxDevices = 64;
+static void *g_scratchPtr[kMaxDevices] = {};
+
+static void *noalloc_scratch(size_t &bytes) {
+  int dev = 0;
+  HIP_CHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) throw Error(KNN_ERR_NO_DEVICE, "device index out of range");
+  if (!g_scratchPtr[dev]) {
+    void *ptr = nullptr;
+    HIP_CHECK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) throw Error(KNN_ERR_HIP, "no device address for the no-allocation scratch");
+    g_scratchPtr[dev] = ptr;
+  }
+  bytes = KNN_NOALLOC_SCRATCH_BYTES;
+  return g_scratchPtr[dev];
 }
 }  // namespace knn

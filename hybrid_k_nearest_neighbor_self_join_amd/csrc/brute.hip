@@ -200,7 +200,7 @@ void select_list(const uint8_t *sel, uint32_t n, uint32_t *flags, uint32_t *pos,
   k_sel_compact<<<g, 256, 0, s>>>(flags, pos, n, list);
 }
 
-BrutePlan brute_plan(uint32_t n, uint32_t nq, uint32_t k1) {
+BrutePlan brute_plan(uint32_t n, uint32_t nq, uint32_t k1, uint64_t maxPartialEntries) {
   BrutePlan p{};
   if (k1 == 0 || k1 > BRUTE_MAX_K1 || n == 0) return p;
   // queries per wave: as many as the 64 KB LDS budget allows, up to 4
@@ -216,8 +216,8 @@ BrutePlan brute_plan(uint32_t n, uint32_t nq, uint32_t k1) {
   uint64_t maxChunks = (n + minChunk - 1) / minChunk;
   if (nchunks > maxChunks) nchunks = maxChunks;
   if (nchunks == 0) nchunks = 1;
-  // partial tables within 2^26 entries (768 MB)
-  const uint64_t memChunks = ((1ull << 26) / ((uint64_t)nq * k1)) > 0 ? ((1ull << 26) / ((uint64_t)nq * k1)) : 1;
+  // partial tables within maxPartialEntries entries (12 bytes each)
+  const uint64_t memChunks = (maxPartialEntries / ((uint64_t)nq * k1)) > 0 ? (maxPartialEntries / ((uint64_t)nq * k1)) : 1;
   if (nchunks > memChunks) nchunks = memChunks;
   uint64_t chunk = (n + nchunks - 1) / nchunks;
   chunk = (chunk + 63) & ~63ull;

@@ -280,7 +280,8 @@ struct BrutePlan {
   uint32_t qw, chunk, nchunks;
   uint64_t partialEntries;
 };
-BrutePlan brute_plan(uint32_t n, uint32_t nq, uint32_t k1);
+// partial tables of at most maxPartialEntries entries (a single chunk always fits: no table)
+BrutePlan brute_plan(uint32_t n, uint32_t nq, uint32_t k1, uint64_t maxPartialEntries = 1ull << 26);
 // list[0..count) = ids i with sel[i] != 0, ascending; count in pos[n].  flags/pos: n + 1 words,
 // scratch: scan_scratch_words(n)
 void select_list(const uint8_t *sel, uint32_t n, uint32_t *flags, uint32_t *pos, uint32_t *list,

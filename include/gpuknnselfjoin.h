@@ -127,6 +127,26 @@ int knnjoin_ex(const void *points, int dtype, uint32_t n, uint32_t ndim, uint32_
                const knnjoin_params *params, int32_t *out_ids, void *out_dists,
                knnjoin_stats *stats);
 
+/*
+ * knnjoin_ex without any device allocation: no counterpart in the reference.  For callers that
+ * cannot allocate device memory (code under stream capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  Same arguments, rows and status codes as knnjoin_ex, with
+ * params->device_ptrs = 1 required (points and outputs in device memory, the caller's).  Every
+ * working buffer lives in one static device array of KNN_NOALLOC_SCRATCH_BYTES (256 MiB),
+ * which each device the library runs on holds until the process ends.  Limits: K+1 <= 85 and
+ * N * (8 NDIM + 116) bytes (float; 16 NDIM + 120 for double) plus about 3 MiB must fit in the
+ * array (about 2M points of 2 dims); larger inputs return KNN_ERR_ARG / KNN_ERR_NOMEM before
+ * any work.  The dense cell-start table and the learned schedule's feature grid are laid out
+ * in the same array when they fit (without the dense table the join searches the cell table
+ * instead).  The first call on a device looks up the array's address and creates the
+ * library's timing events; later calls allocate nothing.  Calls share the device's array:
+ * the library serialises its own calls, and two processes each hold their own copy.
+ */
+#define KNN_NOALLOC_SCRATCH_BYTES (256u << 20)
+int knnjoin_ex_noalloc(const void *points, int dtype, uint32_t n, uint32_t ndim, uint32_t k,
+                       const knnjoin_params *params, int32_t *out_ids, void *out_dists,
+                       knnjoin_stats *stats);
+
 /* Default parameters (reference defaults: NUMINDEXEDDIM=min(NDIM,6), BETA=0.05, REORDER=1). */
 void knnjoin_default_params(knnjoin_params *p);
 
