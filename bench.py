@@ -237,6 +237,43 @@ def suite_brute(args):
     }), flush=True)
 
 
+def suite_hybrid(args):
+    """Hybrid CPU/GPU split (splitWork, main.cu:2733-2933) against the GPU-only BETA schedule:
+    the far-outlier cluster, 5M skewed 4-D points indexed on 2 dims, and the 10M headline set
+    (host share capped), each with the host thread count of the box.  One JSON line per case
+    with both times, the host share and whether the rows are identical."""
+    import torch  # noqa: F401
+
+    from hybrid_k_nearest_neighbor_self_join_amd import _lib, synthetic
+
+    threads = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
+    rng = np.random.default_rng(0x5EED0300)
+    cases = [
+        ("cluster_60000_plus_20_far_outliers", np.concatenate(
+            [rng.normal(0.0, 1.0, (60_000, 2)), rng.uniform(30.0, 40.0, (20, 2))]).astype(np.float32), 16, 2, 0),
+        ("skewed_4d_nd2_5M", synthetic.skewed_cube(5_000_000, 4, seed=0x5EED0100 + 42), 32, 2, 0),
+        ("mixed_2d_10M_k64_host_share_1e5", synthetic.mixed_box(10_000_000), 64, 2, 100_000),
+        ("mixed_2d_10M_k64_host_share_1e6", synthetic.mixed_box(10_000_000), 64, 2, 1_000_000),
+    ]
+    for tag, pts, k, nd, cap in cases:
+        def best(**kw):
+            _lib.knnjoin_ex(pts, k, num_indexed_dims=nd, beta=0.025, brute_fallback=0, **kw)
+            runs = [_lib.knnjoin_ex(pts, k, num_indexed_dims=nd, beta=0.025, brute_fallback=0, **kw)
+                    for _ in range(max(args.steps, 2))]
+            return min(runs, key=lambda r: r[2].ms_total)
+        ig, dg, sg = best()
+        ih, dh, sh = best(hybrid_threads=threads, hybrid_max_queries=cap)
+        print(json.dumps({
+            "suite": "hybrid", "case": tag, "n": len(pts), "d": pts.shape[1], "nd": nd, "k": k,
+            "host_threads": threads, "host_share_cap": cap,
+            "gpu_only_ms": sg.ms_total, "gpu_only_iterations": sg.iterations, "gpu_only_reindexes": sg.reindexes,
+            "hybrid_ms": sh.ms_total, "hybrid_host_queries": sh.hybrid_queries,
+            "hybrid_host_ms": sh.ms_hybrid_host, "hybrid_gpu_iterations": sh.iterations,
+            "hybrid_vs_gpu_only": sg.ms_total / sh.ms_total,
+            "identical_rows": bool(np.array_equal(ig, ih) and np.array_equal(dg.view(np.uint32), dh.view(np.uint32))),
+        }), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -251,7 +288,7 @@ def main():
     ap.add_argument("--cpu-queries", type=int, default=100_000,
                     help="queries of the CPU baseline's subset join against the full set")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--suite", choices=["highdim", "batched", "brute"], default=None,
+    ap.add_argument("--suite", choices=["highdim", "batched", "brute", "hybrid"], default=None,
                     help="instead of the headline line: the 4-D/6-D cases, the 256-set batch, or "
                          "the brute-force path")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -276,6 +313,8 @@ def main():
         return suite_batched(args)
     if args.suite == "brute":
         return suite_brute(args)
+    if args.suite == "hybrid":
+        return suite_hybrid(args)
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (this build has no multi-GPU path); "
               f"--gpus {args.gpus} is not supported", file=sys.stderr)

@@ -51,6 +51,8 @@ class KnnParams(ctypes.Structure):
         ("device_ptrs", ctypes.c_int32),
         ("stream", ctypes.c_void_p),
         ("brute_fallback", ctypes.c_int32),
+        ("hybrid_threads", ctypes.c_uint32),
+        ("hybrid_max_queries", ctypes.c_uint32),
     ]
 
 
@@ -81,7 +83,8 @@ class KnnStats(ctypes.Structure):
         ("ms_index_first", ctypes.c_double),
         ("ms_reference_total", ctypes.c_double),
         ("host_syncs", ctypes.c_uint32),
-        ("pad0", ctypes.c_uint32),
+        ("hybrid_queries", ctypes.c_uint32),
+        ("ms_hybrid_host", ctypes.c_double),
     ]
 
     def as_dict(self) -> dict:
@@ -187,7 +190,7 @@ def default_params(**kw) -> KnnParams:
 # ------------------------------------------------------------------ convenience wrappers
 def knnjoin_ex(points: np.ndarray, k: int, *, num_indexed_dims: int = 0, beta: float = 0.05,
                mode: int = KNN_MODE_BETA, reorder: int = 1, verbose: int = 0,
-               brute_fallback: int = -1):
+               brute_fallback: int = -1, hybrid_threads: int = 0, hybrid_max_queries: int = 0):
     """Whole join through knnjoin_ex on host arrays.  Returns (ids[N,K+1], dists[N,K+1], stats)."""
     pts = np.ascontiguousarray(points)
     if pts.dtype not in (np.float32, np.float64):
@@ -197,7 +200,8 @@ def knnjoin_ex(points: np.ndarray, k: int, *, num_indexed_dims: int = 0, beta: f
     ids = np.full((n, k + 1), -1, np.int32)
     dists = np.zeros((n, k + 1), pts.dtype)
     prm = default_params(num_indexed_dims=num_indexed_dims, beta=beta, mode=mode,
-                         reorder=reorder, verbose=verbose, brute_fallback=brute_fallback)
+                         reorder=reorder, verbose=verbose, brute_fallback=brute_fallback,
+                         hybrid_threads=hybrid_threads, hybrid_max_queries=hybrid_max_queries)
     st = KnnStats()
     rc = lib().knnjoin_ex(p(pts), dt, n, d, k, ctypes.byref(prm), p(ids), p(dists),
                           ctypes.byref(st))

@@ -29,7 +29,7 @@ LIB_DOUBLE = os.path.join(PKG, "double", "libgpuknnselfjoin.so")
 CLI = os.path.join(PKG, "knnjoin_main")
 
 SOURCES = ["prims.hip", "estimator.hip", "join.hip", "range.hip", "learned.hip", "batched.hip",
-           "outlier.hip", "brute.hip", "api.hip"]
+           "outlier.hip", "brute.hip", "hybrid.hip", "api.hip"]
 ARCH = os.environ.get("KNN_OFFLOAD_ARCH", "gfx950")
 
 HIP_FLAGS = [

@@ -18,9 +18,12 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <exception>
+#include <functional>
 #include <mutex>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "gpuknnselfjoin.h"
@@ -50,7 +53,7 @@ enum Slot {
   S_PTS, S_X, S_XS, S_KEYS, S_KEYS_TMP, S_IDS, S_IDS_TMP, S_CELLKEYS, S_CELLSTART, S_FLAGS, S_POS,
   S_ACT, S_NTILES, S_TILEOFF, S_TILES, S_FAILED, S_ROWSUM, S_OUTIDS, S_OUTDISTS, S_SCRATCH,
   S_HEAP, S_SMALL, S_PART, S_CTR, S_CSUM, S_HOSTFLAGS, S_OVERFLOW, S_STAGE, S_DENSE, S_BFLAGS,
-  S_BPOS, S_BSTART, S_CELLOF, S_CELLMAX, S_FCNT, S_RADIUS, S_RHIST, S_MODEL, S_TRAIN, S_OCC, S_ACTQ, S_BRUTE_PK, S_BRUTE_PI, S_BLIST, S_NSLOTS
+  S_BPOS, S_BSTART, S_CELLOF, S_CELLMAX, S_FCNT, S_RADIUS, S_RHIST, S_MODEL, S_TRAIN, S_OCC, S_ACTQ, S_BRUTE_PK, S_BRUTE_PI, S_BLIST, S_TILEMETA, S_HCPU, S_HGPU, S_HLIST, S_HROWI, S_HROWD, S_NSLOTS
 };
 
 // Slots of one no-allocation call: regions of the static device scratch array handed out in
@@ -240,11 +243,54 @@ static void feature_occupancy(const uint32_t *cnt, uint64_t cells, uint32_t n, f
 // ------------------------------------------------------------------ pipeline
 struct PassResult {
   uint32_t nfail, ntiles, noverflow;
-  double kernel_ms;
-  unsigned long long bytes_total;
+  double kernel_ms, pass_ms;
+  unsigned long long bytes_total, pairs;
 };
 
-struct Timer;
+// Event pairs created once (the first call that times anything) and reused by every call:
+// pair i serves the i-th timer of a call (calls are serialised by g_mu).
+constexpr int kTimerPairs = 8;
+static hipEvent_t g_events[kTimerPairs][2];
+static bool g_eventsReady = false;
+
+struct Timer {
+  hipEvent_t a, b;
+  hipStream_t s;
+  Timer(hipStream_t st, int pair) : s(st) {
+    if (!g_eventsReady) {
+      for (int i = 0; i < kTimerPairs; i++)
+        for (int j = 0; j < 2; j++) HIP_CHECK(hipEventCreate(&g_events[i][j]));
+      g_eventsReady = true;
+    }
+    a = g_events[pair][0];
+    b = g_events[pair][1];
+  }
+  void start() { HIP_CHECK(hipEventRecord(a, s)); }
+  double stop() {
+    HIP_CHECK(hipEventRecord(b, s));
+    HIP_CHECK(hipEventSynchronize(b));
+    float ms = 0;
+    HIP_CHECK(hipEventElapsedTime(&ms, a, b));
+    return ms;
+  }
+  // records the end without waiting; elapsed() once a later wait has covered it
+  void stop_async() { HIP_CHECK(hipEventRecord(b, s)); }
+  double elapsed() const {
+    float ms = 0;
+    HIP_CHECK(hipEventElapsedTime(&ms, a, b));
+    return ms;
+  }
+};
+
+// Pinned host words for the values the host reads back during a join (cell, active, failure
+// and overflow counts, accounting counters, radius histograms): allocated once, so that those
+// copies are asynchronous and one stream wait per pass collects them all.
+enum PinWord { PW_NCELL = 0, PW_KFAIL = 1, PW_OVF = 2, PW_ACTIVE = 3, PW_SPAN = 4, PW_CTR = 8, PW_HIST = 64, PW_MAXR = 128 };
+static uint32_t *g_pin = nullptr;
+static uint32_t *pinned() {
+  if (!g_pin) HIP_CHECK(hipHostMalloc(reinterpret_cast<void **>(&g_pin), 4096, hipHostMallocDefault));
+  return g_pin;
+}
 
 template <typename T>
 struct Pipeline {
@@ -269,8 +315,6 @@ struct Pipeline {
   template <typename V>
   V *buf(Slot sl, size_t count) { return reinterpret_cast<V *>(g_cache.get(sl, sizeof(V) * count)); }
 
-  void sync() { HIP_CHECK(hipStreamSynchronize(s)); }
-
   // populateNDGridIndexAndLookupArrayGPU (GPU.cu:3715-4067), all in HBM
   void build_index(const T *mn, const T *mx, T eps) {
     grid_dims<T>(mn, mx, nd, eps, g, totalCells);
@@ -291,7 +335,7 @@ struct Pipeline {
       compact_cells_u32(keys, flags, pos, n, cellKeys, cellStart, s);
       cellOf = buf<uint32_t>(S_CELLOF, n);
       cell_of(flags, pos, n, cellOf, s);
-      HIP_CHECK(hipMemcpyAsync(&ncell, pos + n, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+      ncellDev = pos + n;
     } else {
       uint64_t *keys = buf<uint64_t>(S_KEYS, n);
       uint64_t *kt = buf<uint64_t>(S_KEYS_TMP, n);
@@ -306,7 +350,7 @@ struct Pipeline {
       compact_cells_u64(keys, flags, pos, n, cellKeys, cellStart, s);
       cellOf = buf<uint32_t>(S_CELLOF, n);
       cell_of(flags, pos, n, cellOf, s);
-      HIP_CHECK(hipMemcpyAsync(&ncell, pos + n, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+      ncellDev = pos + n;
     }
     sortedIds = ids;
     xs = buf<T>(S_XS, (size_t)n * d);
@@ -315,8 +359,8 @@ struct Pipeline {
     else
       gather_soa<T>(x, sortedIds, n, d, xs, s);
     LAUNCH_CHECK();
-    sync();
-    // dense cell-start table: O(1) row lookups when the grid is small enough
+    // dense cell-start table: O(1) row lookups when the grid is small enough (its counts are
+    // scattered for the cells below the device-side cell count, so no host round trip here)
     denseStart = nullptr;
     // with the no-allocation scratch: only when the table, its scan scratch and the pass
     // buffers still to come (active list, tiles, block tables, flags) fit
@@ -325,7 +369,7 @@ struct Pipeline {
       const size_t tcb = (size_t)totalCells + 1;
       const size_t more = g_arena->extra(S_DENSE, tcb * 4) +
                           g_arena->extra(S_SCRATCH, (scan_scratch_words((uint32_t)tcb) + 1024) * 4);
-      denseFits = more + 24ull * n + 24ull * ((size_t)ncell + 1) + (64u << 10) <= g_arena->room();
+      denseFits = more + 48ull * n + (64u << 10) <= g_arena->room();
     }
     if (denseFits) {
       const uint32_t tc = (uint32_t)totalCells;
@@ -336,11 +380,24 @@ struct Pipeline {
       }
       uint32_t *dense = buf<uint32_t>(S_DENSE, (size_t)tc + 1);
       fill_u32(dense, 0u, (size_t)tc + 1, s);
-      dense_cell_counts(cellKeys, cellStart, ncell, dense, s);
+      batch_dense_counts(cellKeys, cellStart, ncellDev, n, dense, s);
       scan_exclusive_u32(dense, dense, tc, scratch, s);
       LAUNCH_CHECK();
       denseStart = dense;
     }
+    // the build's one wait: the cell count, for the launch sizes of the passes
+    uint32_t *pin = pinned();
+    HIP_CHECK(hipMemcpyAsync(pin + PW_NCELL, ncellDev, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    if (tBuild) tBuild->stop_async();
+    sync();
+    ncell = pin[PW_NCELL];
+  }
+  const uint32_t *ncellDev = nullptr;  // device copy of ncell (the scan total of the build)
+  Timer *tBuild = nullptr;              // stopped before the build's wait when set
+  uint32_t syncs = 0;                   // stream waits of the passes and builds
+  void sync() {
+    HIP_CHECK(hipStreamSynchronize(s));
+    syncs++;
   }
 
   // 1 GiB of cell starts at most: filled and scanned once per build (~0.6 ms at 2^28 cells),
@@ -352,18 +409,27 @@ struct Pipeline {
 
   // One range pass (kernel.cu:61-225 + GPU.cu:3610-3704 + main.cu:1192-1223): queries with
   // activeFailed[q] != 0 (all when null), grouped by cell into tiles of <= 64.
+  // `beforeWait` (optional) is queued after the range kernel and before the pass's wait, to
+  // have work whose results the host reads at that same wait; when staging overflows send
+  // queries to the heap kernel (a second wait), `afterHeap` is queued after that kernel.
   PassResult pass(const uint8_t *activeFailed, uint32_t nq, uint8_t *failed, int32_t *dIds,
-                  T *dDists, unsigned long long *ctr, Timer &tK);
+                  T *dDists, unsigned long long *ctr, Timer &tK, Timer &tPass,
+                  const std::function<void()> &beforeWait = {}, const std::function<void()> &afterHeap = {});
   T uniform_radius() const { return radiusAll > (T)0 ? radiusAll : (T)g.eps; }
-  uint32_t make_tiles(const uint8_t *activeFailed, uint32_t *&act, uint4 *&tiles, uint8_t *failed,
-                      unsigned long long *cellAcc);
-  void finish(PassResult &r, uint32_t *counters, unsigned long long *ctr, uint32_t prefailed);
+  // tiles of the active queries, every count kept on the device; the tile count ends in
+  // *ntilesDev (the range kernels read it there)
+  void make_tiles(const uint8_t *activeFailed, uint32_t *&act, uint4 *&tiles, const uint32_t *&ntilesDev,
+                  uint8_t *failed, unsigned long long *cellAcc);
   unsigned long long accBytes = 0;  // accumulated pre-check accounting bytes
   const T *radius = nullptr;  // per-query radii (learned schedule), null = radiusAll
   T radiusAll = (T)0;         // one radius for every query (plain schedule); 0 = the grid's eps
   uint32_t lastSpan = 1;
   uint32_t lastActive = 0;   // queries left in tiles after the pre-check
   double avgReach = 1.0;     // typical search radius in cells (tile-span cost model)
+  static constexpr uint32_t kPassGrid = 2048;  // persistent range-kernel workgroups
+  static constexpr uint32_t kHeapGrid = 512;   // persistent heap-kernel workgroups
+  const uint32_t *activeDev = nullptr;         // active count of the last tile build (device)
+  const uint32_t *spanDev = nullptr;           // its tile-block span (device)
 };
 
 template <typename T>
@@ -478,33 +544,6 @@ static EpsResult estimate_eps(const T *x, uint32_t n, uint32_t d, uint32_t k, do
   return r;
 }
 
-// Event pairs created once (the first call that times anything) and reused by every call:
-// pair i serves the i-th timer of a call (calls are serialised by g_mu).
-constexpr int kTimerPairs = 8;
-static hipEvent_t g_events[kTimerPairs][2];
-static bool g_eventsReady = false;
-
-struct Timer {
-  hipEvent_t a, b;
-  hipStream_t s;
-  Timer(hipStream_t st, int pair) : s(st) {
-    if (!g_eventsReady) {
-      for (int i = 0; i < kTimerPairs; i++)
-        for (int j = 0; j < 2; j++) HIP_CHECK(hipEventCreate(&g_events[i][j]));
-      g_eventsReady = true;
-    }
-    a = g_events[pair][0];
-    b = g_events[pair][1];
-  }
-  void start() { HIP_CHECK(hipEventRecord(a, s)); }
-  double stop() {
-    HIP_CHECK(hipEventRecord(b, s));
-    HIP_CHECK(hipEventSynchronize(b));
-    float ms = 0;
-    HIP_CHECK(hipEventElapsedTime(&ms, a, b));
-    return ms;
-  }
-};
 
 // Range kernel of a pass: the collect kernel (one pass over each query's own block) with 5 or
 // more indexed dims, else the two-pass select kernel, whose first pass reads each candidate
@@ -520,11 +559,14 @@ static bool use_collect(uint32_t nd, uint32_t k1) {
   return env >= 0 ? env == 1 : nd >= 5;
 }
 
-// Builds tiles of <= 64 active queries per cell: activeFailed[q] != 0 selects q (all when
-// null).  Returns the tile count; act/tiles point into the cache.
+// Builds tiles of <= 64 active queries per tile block: activeFailed[q] != 0 selects q (all
+// when null); act/tiles point into the cache, the tile count is left in device memory.
+// Tile blocks are windows of `span` cells along dim 0 within one row; the span is chosen on the
+// device from the active count (k_choose_span: wide windows pay only when active queries are
+// dense, and a pass with few tiles ends with its slowest ones).
 template <typename T>
-uint32_t Pipeline<T>::make_tiles(const uint8_t *activeFailed, uint32_t *&act, uint4 *&tiles,
-                                 uint8_t *failed, unsigned long long *cellAcc) {
+void Pipeline<T>::make_tiles(const uint8_t *activeFailed, uint32_t *&act, uint4 *&tiles,
+                             const uint32_t *&ntilesDev, uint8_t *failed, unsigned long long *cellAcc) {
   uint32_t *flags = buf<uint32_t>(S_FLAGS, n + 1);
   uint32_t *pos = buf<uint32_t>(S_POS, n + 1);
   bool all = !activeFailed;
@@ -546,57 +588,28 @@ uint32_t Pipeline<T>::make_tiles(const uint8_t *activeFailed, uint32_t *&act, ui
     act = buf<uint32_t>(S_ACT, n);
     compact_active(sortedIds, flags, pos, n, act, s);
   }
-  // tile blocks: windows of `span` cells along dim 0 within one row.  A tile scans the union
-  // of its lanes' blocks, about (span + 2) cells per row, for min(64, span * rho_active)
-  // queries (rho_active = active queries per non-empty cell), so the cost per query is about
-  // (span + 2) / max(1, min(64, span * rho_active)): wide windows pay off only when active
-  // queries are dense; sparse passes use one cell per window.  A pass with fewer than
-  // kMinTiles tiles leaves the persistent waves (about 2000) with one or two tiles each and
-  // ends with its slowest tile, so a span that gives fewer tiles pays in proportion.
-  uint32_t A = 0;
-  HIP_CHECK(hipMemcpyAsync(&A, pos + n, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-  HIP_CHECK(hipStreamSynchronize(s));
-  lastActive = A;
+  activeDev = pos + n;
+  uint32_t *meta = buf<uint32_t>(S_TILEMETA, 16);
   const uint32_t spanMax = nd == 1 ? 4096u : (nd == 2 ? 256u : 8u);
-  uint32_t span = 1;
-  if (A && ncell) {
-    const double rhoA = (double)A / (double)ncell;
-    double best = 1e300;
-    for (uint32_t w = 1; w <= spanMax; w *= 2) {
-      const double q = std::max(1.0, std::min(64.0, w * rhoA));
-      const double tilesEst = (double)A / q;
-      const double cost = (w + 2.0 * avgReach) / q * std::max(1.0, kMinTiles / tilesEst);
-      if (cost < best * 0.97) {
-        best = cost;
-        span = w;
-      }
-    }
-  }
-  lastSpan = span;
+  choose_span(activeDev, ncell, avgReach, spanMax, kMinTiles, meta, s);
+  spanDev = meta;
   uint32_t *bflags = buf<uint32_t>(S_BFLAGS, (size_t)ncell + 1);
   uint32_t *bpos = buf<uint32_t>(S_BPOS, (size_t)ncell + 1);
   uint32_t *bstart = buf<uint32_t>(S_BSTART, (size_t)ncell + 1);
-  block_flags(cellKeys, ncell, g.nCells[0], span, bflags, s);
+  block_flags_dev(cellKeys, ncell, g.nCells[0], meta, bflags, s);
   scan_exclusive_u32(bflags, bpos, ncell, scratch, s);
   compact_blocks(cellStart, bflags, bpos, ncell, n, bstart, s);
-  uint32_t B = 0;
-  HIP_CHECK(hipMemcpyAsync(&B, bpos + ncell, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-  HIP_CHECK(hipStreamSynchronize(s));
-  uint32_t *ntiles = buf<uint32_t>(S_NTILES, (size_t)B + 1);
-  uint32_t *tileOff = buf<uint32_t>(S_TILEOFF, (size_t)B + 1);
-  cell_tile_counts(bstart, pos, B, ntiles, s);
-  scan_exclusive_u32(ntiles, tileOff, B, scratch, s);
-  uint32_t NT = 0;
-  HIP_CHECK(hipMemcpyAsync(&NT, tileOff + B, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-  HIP_CHECK(hipStreamSynchronize(s));
-  tiles = buf<uint4>(S_TILES, NT ? NT : 1);
-  write_tiles(bstart, pos, tileOff, B, tiles, s);
-  return NT;
+  uint32_t *ntiles = buf<uint32_t>(S_NTILES, (size_t)ncell + 1);
+  uint32_t *tileOff = buf<uint32_t>(S_TILEOFF, (size_t)ncell + 1);
+  tiles = buf<uint4>(S_TILES, n);
+  block_tiles_dev(bstart, pos, bpos + ncell, ncell, ntiles, tileOff, scratch, tiles, s);
+  ntilesDev = tileOff + ncell;
 }
 
 template <typename T>
-PassResult Pipeline<T>::pass(const uint8_t *activeFailed, uint32_t nq, uint8_t *failed,
-                             int32_t *dIds, T *dDists, unsigned long long *ctr, Timer &tK) {
+PassResult Pipeline<T>::pass(const uint8_t *activeFailed, uint32_t nq, uint8_t *failed, int32_t *dIds,
+                             T *dDists, unsigned long long *ctr, Timer &tK, Timer &tPass,
+                             const std::function<void()> &beforeWait, const std::function<void()> &afterHeap) {
   // ctr layout (u64 words): [0] pairs, [1] algorithmic bytes, [2] fail|overflow counters,
   // [3] tile queue head, [4..5] pre-check accounting (candidate points, query cells)
   uint32_t *counters = reinterpret_cast<uint32_t *>(ctr + 2);
@@ -604,9 +617,10 @@ PassResult Pipeline<T>::pass(const uint8_t *activeFailed, uint32_t nq, uint8_t *
   uint32_t *tileCounter = counters + 2;
   uint32_t *act;
   uint4 *tiles;
+  const uint32_t *ntDev;
+  tPass.start();
   fill_u32(reinterpret_cast<uint32_t *>(ctr + 4), 0u, 4, s);
-  const uint32_t NT = make_tiles(activeFailed, act, tiles, failed, ctr + 4);
-  const uint32_t prefailed = nq > lastActive ? nq - lastActive : 0u;
+  make_tiles(activeFailed, act, tiles, ntDev, failed, ctr + 4);
   uint8_t *overflow = buf<uint8_t>(S_OVERFLOW, n);
   fill_u8(overflow, 0, n, s);
   fill_u32(counters, 0u, 4, s);
@@ -621,7 +635,8 @@ PassResult Pipeline<T>::pass(const uint8_t *activeFailed, uint32_t nq, uint8_t *
   a.denseStart = denseStart;
   a.act = act;
   a.tiles = tiles;
-  a.ntiles = NT;
+  a.ntiles = kPassGrid;  // persistent grid; the tile count is read from ntilesDev
+  a.ntilesDev = ntDev;
   a.n = n;
   a.d = d;
   a.k1 = K1;
@@ -633,78 +648,84 @@ PassResult Pipeline<T>::pass(const uint8_t *activeFailed, uint32_t nq, uint8_t *
   a.failCount = counters;
   a.candCount = ctr;
   a.tileCounter = tileCounter;
-  // LDS staging slots per query: 2(K+1)+32, at most 512 (8 ranked items per lane) and
-  // within the 64 KB a workgroup may allocate
-  a.smax = std::min<uint32_t>(2 * K1 + 32, 512u);
+  // LDS staging slots per query: K+1 and 64 more (the items of the K+1-th neighbour's
+  // bucket beyond K+1), at most 512, within the 64 KB a workgroup may allocate
+  a.smax = std::min<uint32_t>(K1 + 64, 512u);
   while (a.smax > K1 && range_select_lds_bytes<T>(a.smax) > 65536) a.smax -= 16;
   a.overflow = overflow;
   a.overflowCount = overflowCount;
   PassResult r{};
-  r.ntiles = NT;
-  if (use_collect(nd, K1)) {
-    tK.start();
-    range_collect<T>(a, g, s);
-    LAUNCH_CHECK();
-    r.kernel_ms = tK.stop();
-    r.noverflow = 0;
-    finish(r, counters, ctr, prefailed);
-    return r;
-  }
-  if (!(a.smax > K1 && range_select_lds_bytes<T>(a.smax) <= 65536)) {
-    // K+1 too large for LDS staging: the heap kernel takes every query of the pass
-    size_t hs = range_heap_scratch_bytes<T>(NT, K1);
-    a.heapScratch = hs ? g_cache.get(S_HEAP, hs) : nullptr;
-    tK.start();
-    range_topk<T>(a, g, s);
-    LAUNCH_CHECK();
-    r.kernel_ms = tK.stop();
-    r.noverflow = NT ? lastActive : 0;
-    finish(r, counters, ctr, prefailed);
-    return r;
-  }
+  bool selectK = false;
   tK.start();
-  range_select<T>(a, g, s);
+  if (use_collect(nd, K1)) {
+    range_collect<T>(a, g, s);
+  } else if (!(a.smax > K1 && range_select_lds_bytes<T>(a.smax) <= 65536)) {
+    // K+1 too large for LDS staging: the heap kernel takes every query of the pass
+    a.ntiles = kHeapGrid;
+    size_t hs = range_heap_scratch_bytes<T>(kHeapGrid, K1);
+    a.heapScratch = hs ? g_cache.get(S_HEAP, hs) : nullptr;
+    range_topk<T>(a, g, s);
+  } else {
+    range_select<T>(a, g, s);
+    selectK = true;
+  }
   LAUNCH_CHECK();
-  r.kernel_ms = tK.stop();
-  uint32_t hc[2];
-  HIP_CHECK(hipMemcpyAsync(hc, counters, sizeof(hc), hipMemcpyDeviceToHost, s));
-  HIP_CHECK(hipStreamSynchronize(s));
-  r.noverflow = hc[1];
-  if (hc[1]) {
+  tK.stop_async();
+  // the pass's wait: failure / overflow / active counts and the accounting counters
+  uint32_t *pin = pinned();
+  auto readback = [&]() {
+    HIP_CHECK(hipMemcpyAsync(pin + PW_KFAIL, counters, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipMemcpyAsync(pin + PW_ACTIVE, activeDev, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipMemcpyAsync(pin + PW_SPAN, spanDev, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipMemcpyAsync(pin + PW_CTR, ctr, 6 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+  };
+  if (beforeWait) beforeWait();
+  readback();
+  tPass.stop_async();
+  sync();
+  r.kernel_ms = tK.elapsed();
+  lastActive = pin[PW_ACTIVE];
+  lastSpan = pin[PW_SPAN];
+  r.noverflow = selectK ? pin[PW_OVF] : 0u;
+  if (r.noverflow) {
     // queries whose candidates overflow the staging slots: per-lane heap kernel
     uint32_t *act2;
     uint4 *tiles2;
-    const uint32_t NT2 = make_tiles(overflow, act2, tiles2, nullptr, nullptr);
+    const uint32_t *nt2;
+    const uint32_t A1 = lastActive;
+    make_tiles(overflow, act2, tiles2, nt2, nullptr, nullptr);
     fill_u32(tileCounter, 0u, 1, s);
     a.act = act2;
     a.tiles = tiles2;
-    a.ntiles = NT2;
-    size_t hs = range_heap_scratch_bytes<T>(NT2, K1);
+    a.ntilesDev = nt2;
+    a.ntiles = kHeapGrid;
+    size_t hs = range_heap_scratch_bytes<T>(kHeapGrid, K1);
     a.heapScratch = hs ? g_cache.get(S_HEAP, hs) : nullptr;
-    tK.start();
+    Timer tH(s, 5);
+    tH.start();
     range_topk<T>(a, g, s);
     LAUNCH_CHECK();
-    r.kernel_ms += tK.stop();
+    tH.stop_async();
+    if (afterHeap) afterHeap();
+    readback();
+    tPass.stop_async();
+    sync();
+    r.kernel_ms += tH.elapsed();
+    lastActive = A1;
   }
-  finish(r, counters, ctr, prefailed);
-  return r;
-}
-
-// failures = pre-check failures + kernel failures; algorithmic bytes += the pre-check's
-// per-query-cell candidate blocks ((4d+4) bytes per candidate, 8 bytes per scanned cell)
-template <typename T>
-void Pipeline<T>::finish(PassResult &r, uint32_t *counters, unsigned long long *ctr,
-                         uint32_t prefailed) {
+  r.pass_ms = tPass.elapsed();
+  // failures = pre-check failures + kernel failures; algorithmic bytes += the pre-check's
+  // per-query-cell candidate blocks ((4d+4) bytes per candidate, 8 bytes per scanned cell)
+  const uint32_t prefailed = nq > lastActive ? nq - lastActive : 0u;
+  r.nfail = pin[PW_KFAIL] + prefailed;
   unsigned long long hctr[6];
-  uint32_t kf = 0;
-  HIP_CHECK(hipMemcpyAsync(&kf, counters, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-  HIP_CHECK(hipMemcpyAsync(hctr, ctr, sizeof(hctr), hipMemcpyDeviceToHost, s));
-  HIP_CHECK(hipStreamSynchronize(s));
-  r.nfail = kf + prefailed;
+  std::memcpy(hctr, pin + PW_CTR, sizeof(hctr));
   uint64_t cellsScanned = 1;
   for (uint32_t j = 0; j < nd; j++) cellsScanned *= 3;
   accBytes += hctr[4] * (sizeof(T) * d + 4ull) + hctr[5] * cellsScanned * 8ull;
   r.bytes_total = hctr[1] + accBytes;
+  r.pairs = hctr[0];
+  return r;
 }
 
 namespace {
@@ -718,8 +739,8 @@ __global__ void k_fill_failed_rows(const uint8_t *failed, uint32_t n, uint32_t k
 }  // namespace
 
 template <typename T>
-static void run_plain(Pipeline<T> &P, double beta, knnjoin_stats &stats, uint8_t *failed,
-                      int32_t *dIds, T *dDists, Timer &tPh, Timer &tK);
+static void run_plain(Pipeline<T> &P, const knnjoin_params &prm, double beta, knnjoin_stats &stats,
+                      uint8_t *failed, int32_t *dIds, T *dDists, Timer &tPh, Timer &tK);
 template <typename T>
 static void run_learned(Pipeline<T> &P, const knnjoin_params &p, knnjoin_stats &stats,
                         uint8_t *failed, int32_t *dIds, T *dDists, Timer &tPh, Timer &tK);
@@ -836,6 +857,10 @@ static int run_join(const T *pts_in, uint32_t n, uint32_t d, uint32_t k, const k
   P.brute = p.brute_fallback < 0 ? p.mode == KNN_MODE_LEARNED : p.brute_fallback != 0;
   if (p.mode != KNN_MODE_BETA && p.mode != KNN_MODE_LEARNED && p.mode != KNN_MODE_BRUTE)
     throw Error(KNN_ERR_ARG, "unknown radius mode");
+  if (p.hybrid_threads && p.mode != KNN_MODE_BETA)
+    throw Error(KNN_ERR_ARG, "the hybrid CPU/GPU split runs with the BETA schedule (KNN_MODE_BETA)");
+  if (p.hybrid_threads && g_arena)
+    throw Error(KNN_ERR_ARG, "the no-allocation entry point runs without the hybrid split");
   if ((p.mode == KNN_MODE_BRUTE || P.brute) && K1 > BRUTE_MAX_K1) {
     if (p.mode == KNN_MODE_BRUTE) throw Error(KNN_ERR_ARG, "KNN_MODE_BRUTE: K+1 must be <= 2048");
     P.brute = false;
@@ -911,7 +936,7 @@ static int run_join(const T *pts_in, uint32_t n, uint32_t d, uint32_t k, const k
   } else if (p.mode == KNN_MODE_LEARNED) {
     run_learned<T>(P, p, stats, failed, dIds, dDists, tPh, tK);
   } else {
-    run_plain<T>(P, beta, stats, failed, dIds, dDists, tPh, tK);
+    run_plain<T>(P, p, beta, stats, failed, dIds, dDists, tPh, tK);
   }
   // ---- checksum (main.cu:452)
   double *part = P.template buf<double>(S_CSUM, 64 + (n + 4095) / 4096 + 16);
@@ -964,10 +989,167 @@ static double plain_max_reach(uint32_t nd) {
   return e > 0.0 ? e : (nd <= 2 ? 2.0 : 1.0);
 }
 
-// The plain method: BETA eps0 estimate and the failed-query loop of main.cu:305-430.
+// ------------------------------------------------------------------ hybrid split
+// Pinned host buffers of the hybrid split's copies (grow-only, released by knnjoin_release).
+struct HostPinned {
+  void *p[6] = {};
+  size_t cap[6] = {};
+  void *get(int i, size_t bytes) {
+    if (cap[i] < bytes) {
+      if (p[i]) HIP_CHECK(hipHostFree(p[i]));
+      p[i] = nullptr;
+      cap[i] = 0;
+      HIP_CHECK(hipHostMalloc(&p[i], bytes ? bytes : 16, hipHostMallocDefault));
+      cap[i] = bytes;
+    }
+    return p[i];
+  }
+  void release() {
+    for (int i = 0; i < 6; i++) {
+      if (p[i]) (void)hipHostFree(p[i]);
+      p[i] = nullptr;
+      cap[i] = 0;
+    }
+  }
+};
+static HostPinned g_hostPin;
+
+// The host share of a hybrid join (splitWork, main.cu:2733-2933): the sparse-cell queries of
+// the first index, joined by host threads from a host copy of that index while the GPU loop
+// runs over the other queries; the rows are scattered into the device table at the end.
 template <typename T>
-static void run_plain(Pipeline<T> &P, double beta, knnjoin_stats &stats, uint8_t *failed,
-                      int32_t *dIds, T *dDists, Timer &tPh, Timer &tK) {
+struct HybridShare {
+  uint32_t nq = 0;               // host queries
+  const uint8_t *gpuActive = nullptr;  // device: 1 = a query of the GPU loop
+  uint32_t *listIds = nullptr;   // device: the host queries' point ids, row order of the share
+  HybridIndex<T> ix{};
+  typename GridOf<T>::type g{};
+  const uint32_t *qpos = nullptr;  // host: their sorted positions
+  int32_t *rowIds = nullptr;
+  T *rowDists = nullptr;
+  uint32_t threads = 1, k1 = 0;
+  double ms = 0;
+  std::thread th;
+  std::exception_ptr err;
+  ~HybridShare() {
+    if (th.joinable()) th.join();
+  }
+};
+
+namespace {
+__global__ void k_ids_of_positions(const uint32_t *pos, const uint32_t *sortedIds, uint32_t m, uint32_t *ids) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < m) ids[i] = sortedIds[pos[i]];
+}
+// the queries at the listed sorted positions go back to the GPU loop
+__global__ void return_to_gpu(const uint32_t *pos, uint32_t m, const uint32_t *sortedIds, uint8_t *gpu) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < m) gpu[sortedIds[pos[i]]] = 1;
+}
+}  // namespace
+
+template <typename T>
+static void hybrid_start(Pipeline<T> &P, const knnjoin_params &prm, HybridShare<T> &H) {
+  hipStream_t s = P.s;
+  const uint32_t n = P.n;
+  uint32_t thr = hybrid_point_threshold(P.K1, P.nd);
+  // the cell table on the host (also the host index's), and with a cap on the host share the
+  // largest threshold whose sparse cells hold at most that many queries
+  uint32_t *cs = reinterpret_cast<uint32_t *>(g_hostPin.get(0, sizeof(uint32_t) * ((size_t)P.ncell + 1)));
+  uint64_t *ck = reinterpret_cast<uint64_t *>(g_hostPin.get(1, sizeof(uint64_t) * ((size_t)P.ncell + 1)));
+  HIP_CHECK(hipMemcpyAsync(cs, P.cellStart, sizeof(uint32_t) * ((size_t)P.ncell + 1), hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipMemcpyAsync(ck, P.cellKeys, sizeof(uint64_t) * (size_t)P.ncell, hipMemcpyDeviceToHost, s));
+  P.sync();
+  const uint32_t cap = prm.hybrid_max_queries;
+  if (cap) {
+    // the sparsest cells first: every cell of up to t points, where t is the largest
+    // population whose cells hold at most `cap` queries, then cells of t + 1 points in cell
+    // order until the cap (the list below is cut there)
+    std::vector<uint64_t> h((size_t)thr + 1, 0);
+    for (uint32_t c = 0; c < P.ncell; c++) {
+      const uint32_t m = cs[c + 1] - cs[c];
+      if (m <= thr) h[m] += m;
+    }
+    uint64_t acc = 0;
+    uint32_t t = 0;
+    for (uint32_t m = 1; m <= thr; m++) {
+      if (acc + h[m] > cap) break;
+      acc += h[m];
+      t = m;
+    }
+    thr = std::min(thr, t + 1);
+  }
+  uint8_t *cpu = P.template buf<uint8_t>(S_HCPU, n);
+  uint8_t *gpu = P.template buf<uint8_t>(S_HGPU, n);
+  hybrid_flags(P.cellStart, P.cellOf, P.sortedIds, n, thr, cpu, gpu, s);
+  uint32_t *flags = P.template buf<uint32_t>(S_FLAGS, n + 1);
+  uint32_t *pos = P.template buf<uint32_t>(S_POS, n + 1);
+  uint32_t *list = P.template buf<uint32_t>(S_HLIST, n + 1);
+  select_list(cpu, n, flags, pos, list, P.scratch, s);
+  uint32_t *pin = pinned();
+  HIP_CHECK(hipMemcpyAsync(pin + PW_ACTIVE, pos + n, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  P.sync();
+  H.nq = pin[PW_ACTIVE];
+  if (cap && H.nq > cap) {
+    // past the cap the listed queries stay with the GPU
+    return_to_gpu<<<(H.nq - cap + 255) / 256, 256, 0, s>>>(list + cap, H.nq - cap, P.sortedIds, gpu);
+    LAUNCH_CHECK();
+    H.nq = cap;
+  }
+  if (H.nq == 0) return;
+  const uint32_t d = P.d;
+  T *xs = reinterpret_cast<T *>(g_hostPin.get(2, sizeof(T) * (size_t)n * d));
+  uint32_t *sid = reinterpret_cast<uint32_t *>(g_hostPin.get(3, sizeof(uint32_t) * (size_t)n));
+  uint32_t *qp = reinterpret_cast<uint32_t *>(g_hostPin.get(4, sizeof(uint32_t) * (size_t)H.nq));
+  HIP_CHECK(hipMemcpyAsync(xs, P.xs, sizeof(T) * (size_t)n * d, hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipMemcpyAsync(sid, P.sortedIds, sizeof(uint32_t) * (size_t)n, hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipMemcpyAsync(qp, list, sizeof(uint32_t) * (size_t)H.nq, hipMemcpyDeviceToHost, s));
+  // the rows' destinations: the point ids of the listed positions (device, for the scatter)
+  H.listIds = P.template buf<uint32_t>(S_HROWI, H.nq);
+  k_ids_of_positions<<<(H.nq + 255) / 256, 256, 0, s>>>(list, P.sortedIds, H.nq, H.listIds);
+  LAUNCH_CHECK();
+  P.sync();
+  H.ix = HybridIndex<T>{xs, sid, ck, cs, P.ncell, n, d};
+  H.g = P.g;
+  H.qpos = qp;
+  H.k1 = P.K1;
+  H.threads = prm.hybrid_threads;
+  const size_t rowBytes = (size_t)H.nq * P.K1;
+  H.rowIds = reinterpret_cast<int32_t *>(g_hostPin.get(5, rowBytes * (sizeof(int32_t) + sizeof(T))));
+  H.rowDists = reinterpret_cast<T *>(H.rowIds + rowBytes);
+  H.gpuActive = gpu;
+  H.th = std::thread([&H]() {
+    try {
+      auto t0 = std::chrono::steady_clock::now();
+      hybrid_host_join<T>(H.ix, H.g, H.k1, H.qpos, H.nq, H.threads, H.rowIds, H.rowDists);
+      H.ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    } catch (...) {
+      H.err = std::current_exception();
+    }
+  });
+}
+
+// waits for the host share and scatters its rows into the device table
+template <typename T>
+static void hybrid_finish(Pipeline<T> &P, HybridShare<T> &H, int32_t *dIds, T *dDists) {
+  if (!H.nq) return;
+  if (H.th.joinable()) H.th.join();
+  if (H.err) std::rethrow_exception(H.err);
+  const size_t cnt = (size_t)H.nq * P.K1;
+  int32_t *ri = P.template buf<int32_t>(S_HLIST, cnt);
+  T *rd = P.template buf<T>(S_HROWD, cnt);
+  HIP_CHECK(hipMemcpyAsync(ri, H.rowIds, sizeof(int32_t) * cnt, hipMemcpyHostToDevice, P.s));
+  HIP_CHECK(hipMemcpyAsync(rd, H.rowDists, sizeof(T) * cnt, hipMemcpyHostToDevice, P.s));
+  hybrid_scatter<T>(ri, rd, H.listIds, H.nq, P.K1, dIds, dDists, P.s);
+  LAUNCH_CHECK();
+}
+
+// The plain method: BETA eps0 estimate and the failed-query loop of main.cu:305-430.  Each
+// pass waits on the device once (twice when staging overflows send queries to the heap
+// kernel), each index build once.
+template <typename T>
+static void run_plain(Pipeline<T> &P, const knnjoin_params &prm, double beta, knnjoin_stats &stats,
+                      uint8_t *failed, int32_t *dIds, T *dDists, Timer &tPh, Timer &tK) {
   hipStream_t s = P.s;
   const uint32_t n = P.n, d = P.d, nd = P.nd, k = P.k, K1 = P.K1;
   auto fail_all = [&](int code, const std::string &msg) {
@@ -995,7 +1177,7 @@ static void run_plain(Pipeline<T> &P, double beta, knnjoin_stats &stats, uint8_t
   // ---- the join loop (main.cu:305-430)
   const double pdiv = plain_grid_div(nd), preach = plain_max_reach(nd);
   T curW = (T)0;
-  auto build = [&](T e) {
+  auto build = [&](T e) -> double {
     T w = (T)((double)e / pdiv);
     if (pdiv > 1.0) {  // a finer grid only while it keeps the dense start table
       typename GridOf<T>::type gt;
@@ -1008,23 +1190,33 @@ static void run_plain(Pipeline<T> &P, double beta, knnjoin_stats &stats, uint8_t
       }
       if (!ok || tc + 1 > Pipeline<T>::kDenseMaxCells) w = e;
     }
+    P.tBuild = &tPh;
+    tPh.start();
     P.build_index(hmn, hmx, w);
+    P.tBuild = nullptr;
     curW = w;
+    return tPh.elapsed();
   };
   P.radius = nullptr;  // every query searches the same radius, passed by value
-  tPh.start();
   T eps = eps0;
-  build(eps);
-  stats.ms_index = tPh.stop();
+  const uint32_t syncs0 = P.syncs;
+  stats.ms_index = build(eps);
   stats.ms_index_first = stats.ms_index;
 
   unsigned long long *ctr = P.template buf<unsigned long long>(S_CTR, 64);
   fill_u32(reinterpret_cast<uint32_t *>(ctr), 0u, 16, s);
-  uint32_t nq = n, numIter = 0, reindexes = 0, launches = 0;
+  // ---- hybrid split of the first index's queries (splitWork, main.cu:2733-2933)
+  HybridShare<T> H;
+  if (prm.hybrid_threads) {
+    hybrid_start<T>(P, prm, H);
+    if (H.nq) fill_u8(failed, 0, n, s);  // the host queries are never pending on the GPU
+  }
+  stats.hybrid_queries = H.nq;
+  uint32_t nq = n - H.nq, numIter = 0, reindexes = 0, launches = 0;
   double frac = 0;
   bool first = true;
   double joinMs = 0, kMs = 0, kMax = 0;
-  uint64_t bytesAtMax = 0, prevBytes = 0;
+  uint64_t bytesAtMax = 0, prevBytes = 0, pairs = 0;
   while (nq != 0) {
     if (numIter != 0) {
       if (frac > 0.25) {  // main.cu:330-356
@@ -1035,11 +1227,7 @@ static void run_plain(Pipeline<T> &P, double beta, knnjoin_stats &stats, uint8_t
           throw Error(KNN_ERR_NO_PROGRESS, "eps + eps0/2 == eps: epsilon cannot grow");
         }
         eps = grown;
-        if ((double)eps > preach * (double)curW) {
-          tPh.start();
-          build(eps);
-          stats.ms_index += tPh.stop();
-        }
+        if ((double)eps > preach * (double)curW) stats.ms_index += build(eps);
         reindexes++;  // the reference's re-index count: eps growths
       } else {
         // The reference reruns the failed queries at the same eps (main.cu:330 is false);
@@ -1049,10 +1237,9 @@ static void run_plain(Pipeline<T> &P, double beta, knnjoin_stats &stats, uint8_t
         continue;
       }
     }
-    tPh.start();
     P.radiusAll = eps;
     P.avgReach = (double)eps / (double)curW;
-    PassResult pr = P.pass(first ? nullptr : failed, nq, failed, dIds, dDists, ctr, tK);
+    PassResult pr = P.pass(first ? H.gpuActive : failed, nq, failed, dIds, dDists, ctr, tK, tPh);
     launches++;
     stats.overflow_queries += pr.noverflow;
     kMs += pr.kernel_ms;
@@ -1061,11 +1248,12 @@ static void run_plain(Pipeline<T> &P, double beta, knnjoin_stats &stats, uint8_t
       bytesAtMax = pr.bytes_total - prevBytes;
     }
     prevBytes = pr.bytes_total;
+    pairs = pr.pairs;
     const uint32_t nfail = pr.nfail;
-    joinMs += tPh.stop();
+    joinMs += pr.pass_ms;
     if (P.verbose)
-      printf("[knn] iter %u eps %.9g queries %u failed %u tiles %u cells %u span %u overflow %u kernel %.3f ms\n",
-             numIter, (double)eps, nq, nfail, pr.ntiles, P.ncell, P.lastSpan, pr.noverflow, pr.kernel_ms);
+      printf("[knn] iter %u eps %.9g queries %u failed %u cells %u span %u overflow %u kernel %.3f ms\n",
+             numIter, (double)eps, nq, nfail, P.ncell, P.lastSpan, pr.noverflow, pr.kernel_ms);
     frac = (nfail * 1.0) / (nq * 1.0);  // main.cu:402
     nq = nfail;
     numIter++;
@@ -1081,15 +1269,15 @@ static void run_plain(Pipeline<T> &P, double beta, knnjoin_stats &stats, uint8_t
       nq = 0;
     }
   }
+  hybrid_finish<T>(P, H, dIds, dDists);
+  stats.ms_hybrid_host = H.ms;
+  stats.host_syncs = P.syncs - syncs0;
   stats.ms_join = joinMs;
   stats.ms_range_kernel = kMs;
   stats.ms_range_kernel_max = kMax;
   stats.algo_bytes_max = bytesAtMax;
-  unsigned long long hctr[2];
-  HIP_CHECK(hipMemcpyAsync(hctr, ctr, sizeof(hctr), hipMemcpyDeviceToHost, s));
-  HIP_CHECK(hipStreamSynchronize(s));
-  stats.candidates = hctr[0];
-  stats.algo_bytes = hctr[1] + P.accBytes;
+  stats.candidates = pairs;
+  stats.algo_bytes = prevBytes;
   stats.eps_final = (float)eps;
   P.radiusAll = (T)0;
   stats.iterations = numIter;
@@ -1127,7 +1315,8 @@ static int learned_class_bins(uint32_t nd, uint32_t k1) {
 
 // The learned schedule: per-query initial radius from the MLP, then passes over radius
 // octaves (each on a grid matched to its radii) with the expansion loop (radius x2 for the
-// queries that fail) until every query has its K+1 neighbours.
+// queries that fail) until every query has its K+1 neighbours.  A pass waits on the device
+// once: the next pass's radius histogram is queued before that wait.
 template <typename T>
 static void run_learned(Pipeline<T> &P, const knnjoin_params &p, knnjoin_stats &stats,
                         uint8_t *failed, int32_t *dIds, T *dDists, Timer &tPh, Timer &tK) {
@@ -1158,7 +1347,6 @@ static void run_learned(Pipeline<T> &P, const knnjoin_params &p, knnjoin_stats &
   radius_kernel<T>(P.x, n, F.g, cnt, F.logk1, F.logdens, Wd, M.margin, (float)F.wf, radius, nullptr,
                    nullptr, s);
   LAUNCH_CHECK();
-  stats.ms_eps = tPh.stop();
   P.radius = radius;
 
   // ---- radius classes: the pending queries whose radius lies in the lowest occupied class
@@ -1173,17 +1361,26 @@ static void run_learned(Pipeline<T> &P, const knnjoin_params &p, knnjoin_stats &
   uint8_t *act = P.template buf<uint8_t>(S_ACTQ, n);
   fill_u8(failed, 1, n, s);  // pending
   const float wfF = (float)F.wf;
+  uint32_t *pin = pinned();
+  // the pending queries' radius histogram into pin[PW_HIST..] (read at the next wait)
+  auto queue_bins = [&]() {
+    fill_u32(ph, 0u, 64, s);
+    radius_bins(radius, failed, n, wfF, ph, s);
+    HIP_CHECK(hipMemcpyAsync(pin + PW_HIST, ph, 64 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  };
+  queue_bins();
+  tPh.stop_async();
+  const uint32_t syncs0 = P.syncs;
+  P.sync();
+  stats.ms_eps = tPh.elapsed();
   uint32_t passes = 0, reindexes = 0;
   double joinMs = 0, kMs = 0, kMax = 0;
-  uint64_t bytesAtMax = 0, prevBytes = 0;
+  uint64_t bytesAtMax = 0, prevBytes = 0, pairs = 0;
   T curW = (T)0;
   stats.eps0 = 0.0f;
   for (;;) {
-    fill_u32(ph, 0u, 64, s);
-    radius_bins(radius, failed, n, wfF, ph, s);
     uint32_t h[64];
-    HIP_CHECK(hipMemcpyAsync(h, ph, sizeof(h), hipMemcpyDeviceToHost, s));
-    HIP_CHECK(hipStreamSynchronize(s));
+    std::memcpy(h, pin + PW_HIST, sizeof(h));
     uint64_t pending = 0;
     int c = -1;
     for (int b = 0; b < 64; b++) {
@@ -1214,9 +1411,10 @@ static void run_learned(Pipeline<T> &P, const knnjoin_params &p, knnjoin_stats &
     } else {
       // the top bin is open-ended: the class's largest radius bounds it
       max_failed_radius(radius, failed, n, maxr, s);
+      HIP_CHECK(hipMemcpyAsync(pin + PW_MAXR, maxr, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+      P.sync();
       uint64_t mb = 0;
-      HIP_CHECK(hipMemcpyAsync(&mb, maxr, sizeof(mb), hipMemcpyDeviceToHost, s));
-      HIP_CHECK(hipStreamSynchronize(s));
+      std::memcpy(&mb, pin + PW_MAXR, sizeof(mb));
       if (sizeof(T) == 4) {
         uint32_t b32 = (uint32_t)mb;
         float f;
@@ -1244,9 +1442,11 @@ static void run_learned(Pipeline<T> &P, const knnjoin_params &p, knnjoin_stats &
       w = w * (T)2;
     }
     if (w != curW) {
+      P.tBuild = &tPh;
       tPh.start();
       P.build_index(hmn, hmx, w);
-      const double bms = tPh.stop();
+      P.tBuild = nullptr;
+      const double bms = tPh.elapsed();
       stats.ms_index += bms;
       if (curW == (T)0) stats.ms_index_first = bms;
       if (curW != (T)0) reindexes++;
@@ -1255,8 +1455,16 @@ static void run_learned(Pipeline<T> &P, const knnjoin_params &p, knnjoin_stats &
     }
     P.avgReach = 1.5;
     radius_class_flags(radius, failed, n, wfF, (uint32_t)c, (uint32_t)cHi, act, s);
-    tPh.start();
-    PassResult pr = P.pass(act, nq, failed, dIds, dDists, ctr, tK);
+    // growth on failure: one class (an octave; a quarter octave with >= 3 indexed dims), then
+    // the next histogram, both before the pass's wait; after a heap-kernel pass (overflowing
+    // staging) only the histogram again
+    const T factor = nd <= 2 ? (T)2 : (T)1.189207115002721;
+    PassResult pr = P.pass(act, nq, failed, dIds, dDists, ctr, tK, tPh,
+                           [&]() {
+                             grow_class_radius(radius, act, failed, n, factor, s);
+                             queue_bins();
+                           },
+                           [&]() { queue_bins(); });
     stats.overflow_queries += pr.noverflow;
     kMs += pr.kernel_ms;
     if (pr.kernel_ms > kMax) {
@@ -1264,23 +1472,20 @@ static void run_learned(Pipeline<T> &P, const knnjoin_params &p, knnjoin_stats &
       bytesAtMax = pr.bytes_total - prevBytes;
     }
     prevBytes = pr.bytes_total;
-    // growth on failure: one class (an octave; a quarter octave with >= 3 indexed dims)
-    grow_class_radius(radius, act, failed, n, nd <= 2 ? (T)2 : (T)1.189207115002721, s);
-    joinMs += tPh.stop();
+    pairs = pr.pairs;
+    joinMs += pr.pass_ms;
     if (P.verbose)
-      printf("[knn] learned pass %u octave %d grid %.9g queries %u failed %u tiles %u cells %u span %u overflow %u kernel %.3f ms\n",
-             passes, c, (double)P.g.eps, nq, pr.nfail, pr.ntiles, P.ncell, P.lastSpan, pr.noverflow, pr.kernel_ms);
+      printf("[knn] learned pass %u octave %d grid %.9g queries %u failed %u cells %u span %u overflow %u kernel %.3f ms\n",
+             passes, c, (double)P.g.eps, nq, pr.nfail, P.ncell, P.lastSpan, pr.noverflow, pr.kernel_ms);
     passes++;
   }
+  stats.host_syncs = P.syncs - syncs0;
   stats.ms_join = joinMs;
   stats.ms_range_kernel = kMs;
   stats.ms_range_kernel_max = kMax;
   stats.algo_bytes_max = bytesAtMax;
-  unsigned long long hctr[2];
-  HIP_CHECK(hipMemcpyAsync(hctr, ctr, sizeof(hctr), hipMemcpyDeviceToHost, s));
-  HIP_CHECK(hipStreamSynchronize(s));
-  stats.candidates = hctr[0];
-  stats.algo_bytes = hctr[1] + P.accBytes;
+  stats.candidates = pairs;
+  stats.algo_bytes = prevBytes;
   stats.eps_final = (float)P.g.eps;
   stats.iterations = passes;
   stats.reindexes = reindexes;
@@ -1433,6 +1638,7 @@ const char *knnjoin_last_error(void) {
 void knnjoin_release(void) {
   std::lock_guard<std::mutex> lk(knn::g_mu);
   knn::g_cache.release();
+  knn::g_hostPin.release();
   knn::release_batch();
 }
 
@@ -1589,8 +1795,8 @@ int knnjoin_stage_join_pass(const float *points, uint32_t n, uint32_t ndim, uint
     HIP_CHECK(hipMemcpyAsync(dD, out_dists, sizeof(float) * (size_t)n * K1, hipMemcpyHostToDevice, P.s));
     unsigned long long *ctr = P.buf<unsigned long long>(S_CTR, 64);
     fill_u32(reinterpret_cast<uint32_t *>(ctr), 0u, 16, P.s);
-    Timer tK(P.s, 3);
-    PassResult pr = P.pass(dAct, nq, dFailed, dIds, dD, ctr, tK);
+    Timer tK(P.s, 3), tP(P.s, 6);
+    PassResult pr = P.pass(dAct, nq, dFailed, dIds, dD, ctr, tK, tP);
     HIP_CHECK(hipMemcpyAsync(out_ids, dIds, sizeof(int32_t) * (size_t)n * K1, hipMemcpyDeviceToHost, P.s));
     HIP_CHECK(hipMemcpyAsync(out_dists, dD, sizeof(float) * (size_t)n * K1, hipMemcpyDeviceToHost, P.s));
     HIP_CHECK(hipMemcpyAsync(failed, dFailed, n, hipMemcpyDeviceToHost, P.s));

@@ -20,6 +20,8 @@
 //   --double           DTYPE double (default float)
 //   --learned FILE     learned per-query radius: FILE holds 5057 little-endian float32
 //                      weights followed by the float32 margin (written by learned_radius.py)
+//   --hybrid-threads N host threads of the hybrid CPU/GPU split (splitWork, main.cu:2733-2933)
+//   --hybrid-max-queries M  at most M host queries (sparsest cells first)
 //   --print-table      write KNN_out.txt / KNN_out_distances.txt
 //   --outlier-scores   write KNN_outlier_scores.txt
 //   --parse-only       read the file and print N and a digest of the values (no GPU use)
@@ -161,6 +163,9 @@ int run(const char *fname, unsigned ndim, unsigned k, knnjoin_params &p, bool pr
   std::printf("\nDevice pipeline (first index build included): %f", st.ms_total / 1e3);
   std::printf("\nSchedule: %s, iterations %u, re-indexes %u",
               p.mode == KNN_MODE_LEARNED ? "learned" : "beta", st.iterations, st.reindexes);
+  if (p.hybrid_threads)
+    std::printf("\nHybrid split: %u queries on %u host threads (%.3f s, concurrent with the GPU)",
+                st.hybrid_queries, p.hybrid_threads, st.ms_hybrid_host / 1e3);
   std::printf("\n[Verification] Total distance (without square root): %f\n", st.checksum);
   if (printTable) write_table<T>((unsigned)n, k1, ids.data(), d.data());
   if (outlierScores) {
@@ -205,6 +210,8 @@ int main(int argc, char **argv) {
     else if (a == "--beta") p.beta = std::atof(need("--beta"));
     else if (a == "--double") dbl = true;
     else if (a == "--learned") model = need("--learned");
+    else if (a == "--hybrid-threads") p.hybrid_threads = (uint32_t)std::atoi(need("--hybrid-threads"));
+    else if (a == "--hybrid-max-queries") p.hybrid_max_queries = (uint32_t)std::atoi(need("--hybrid-max-queries"));
     else if (a == "--print-table") printTable = true;
     else if (a == "--outlier-scores") outlierScores = true;
     else if (a == "--parse-only") parseOnly = true;

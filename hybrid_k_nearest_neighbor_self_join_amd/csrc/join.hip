@@ -243,6 +243,63 @@ __global__ void k_write_tiles(const uint32_t *cellStart, const uint32_t *pos,
     tiles[t] = make_uint4(c, a, (a1 - a < 64u) ? a1 - a : 64u, a == a0 ? 1u : 0u);
 }
 
+// Tile-block span of a pass from the active count A = *activeCount (read on the device): a
+// tile scans the union of its lanes' blocks, about (span + 2 reach) cells per row, for
+// min(64, span * rho) queries (rho = active queries per non-empty cell), so the cost per
+// query is about (span + 2 reach) / max(1, min(64, span * rho)); a pass with fewer than
+// minTiles tiles leaves the persistent waves idle at its end, which costs in proportion.
+__global__ void k_choose_span(const uint32_t *activeCount, uint32_t ncell, double avgReach,
+                              uint32_t spanMax, double minTiles, uint32_t *span) {
+  if (threadIdx.x || blockIdx.x) return;
+  const uint32_t A = *activeCount;
+  uint32_t best = 1;
+  if (A && ncell) {
+    const double rhoA = (double)A / (double)ncell;
+    double bc = 1e300;
+    for (uint32_t w = 1; w <= spanMax; w *= 2) {
+      const double q = fmax(1.0, fmin(64.0, w * rhoA));
+      const double tilesEst = (double)A / q;
+      const double cost = (w + 2.0 * avgReach) / q * fmax(1.0, minTiles / tilesEst);
+      if (cost < bc * 0.97) {
+        bc = cost;
+        best = w;
+      }
+    }
+  }
+  *span = best;
+}
+
+__global__ void k_block_flags_dev(const uint64_t *cellKeys, uint32_t ncell, uint32_t ncell0,
+                                  const uint32_t *span, uint32_t *flags) {
+  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncell) return;
+  const uint32_t w = *span;
+  flags[c] = (c == 0 || block_key(cellKeys[c], ncell0, w) != block_key(cellKeys[c - 1], ncell0, w)) ? 1u : 0u;
+}
+
+// per tile block b < *nblocks (0 for b up to ncell): tiles of <= 64 active queries
+__global__ void k_block_tile_counts_dev(const uint32_t *blockStart, const uint32_t *pos,
+                                        const uint32_t *nblocks, uint32_t ncell, uint32_t *ntiles) {
+  const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= ncell) return;
+  if (b >= *nblocks) {
+    ntiles[b] = 0;
+    return;
+  }
+  const uint32_t a0 = pos[blockStart[b]], a1 = pos[blockStart[b + 1]];
+  ntiles[b] = (a1 - a0 + 63) / 64;
+}
+
+__global__ void k_write_tiles_dev(const uint32_t *blockStart, const uint32_t *pos, const uint32_t *tileOff,
+                                  const uint32_t *nblocks, uint32_t ncell, uint4 *tiles) {
+  const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= ncell || b >= *nblocks) return;
+  const uint32_t a0 = pos[blockStart[b]], a1 = pos[blockStart[b + 1]];
+  uint32_t t = tileOff[b];
+  for (uint32_t a = a0; a < a1; a += 64, t++)
+    tiles[t] = make_uint4(b, a, (a1 - a < 64u) ? a1 - a : 64u, a == a0 ? 1u : 0u);
+}
+
 // ------------------------------------------------------------------ checksum
 constexpr uint32_t SUM_CHUNK = 4096;
 // one block per chunk of SUM_CHUNK rows: coalesced copy to LDS, then lane 0 adds in row order
@@ -421,6 +478,22 @@ void cell_tile_counts(const uint32_t *cellStart, const uint32_t *pos, uint32_t n
 void write_tiles(const uint32_t *cellStart, const uint32_t *pos, const uint32_t *tileOff,
                  uint32_t ncell, uint4 *tiles, hipStream_t s) {
   if (ncell) k_write_tiles<<<(ncell + 255) / 256, 256, 0, s>>>(cellStart, pos, tileOff, ncell, tiles);
+}
+
+void choose_span(const uint32_t *activeCount, uint32_t ncell, double avgReach, uint32_t spanMax,
+                 double minTiles, uint32_t *span, hipStream_t s) {
+  k_choose_span<<<1, 64, 0, s>>>(activeCount, ncell, avgReach, spanMax, minTiles, span);
+}
+void block_flags_dev(const uint64_t *cellKeys, uint32_t ncell, uint32_t ncell0, const uint32_t *span,
+                     uint32_t *flags, hipStream_t s) {
+  if (ncell) k_block_flags_dev<<<(ncell + 255) / 256, 256, 0, s>>>(cellKeys, ncell, ncell0, span, flags);
+}
+void block_tiles_dev(const uint32_t *blockStart, const uint32_t *pos, const uint32_t *nblocks, uint32_t ncell,
+                     uint32_t *ntiles, uint32_t *tileOff, uint32_t *scratch, uint4 *tiles, hipStream_t s) {
+  if (!ncell) return;
+  k_block_tile_counts_dev<<<(ncell + 255) / 256, 256, 0, s>>>(blockStart, pos, nblocks, ncell, ntiles);
+  scan_exclusive_u32(ntiles, tileOff, ncell, scratch, s);
+  k_write_tiles_dev<<<(ncell + 255) / 256, 256, 0, s>>>(blockStart, pos, tileOff, nblocks, ncell, tiles);
 }
 
 template <typename T>

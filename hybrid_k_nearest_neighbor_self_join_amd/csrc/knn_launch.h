@@ -124,6 +124,17 @@ void cell_tile_counts(const uint32_t *cellStart, const uint32_t *pos, uint32_t n
 void write_tiles(const uint32_t *cellStart, const uint32_t *pos, const uint32_t *tileOff,
                  uint32_t ncell, uint4 *tiles, hipStream_t s);
 
+// The same tile build with every count read on the device (no host round trip): the span from
+// the active count *activeCount (k_choose_span's cost model), block flags at that span, and
+// tiles of the *nblocks blocks (ntiles / tileOff hold ncell + 1 words; tileOff[ncell] = tile
+// count, read by the range kernels through JoinArgs::ntilesDev).
+void choose_span(const uint32_t *activeCount, uint32_t ncell, double avgReach, uint32_t spanMax,
+                 double minTiles, uint32_t *span, hipStream_t s);
+void block_flags_dev(const uint64_t *cellKeys, uint32_t ncell, uint32_t ncell0, const uint32_t *span,
+                     uint32_t *flags, hipStream_t s);
+void block_tiles_dev(const uint32_t *blockStart, const uint32_t *pos, const uint32_t *nblocks, uint32_t ncell,
+                     uint32_t *ntiles, uint32_t *tileOff, uint32_t *scratch, uint4 *tiles, hipStream_t s);
+
 struct JoinArgs {
   const void *x;         // reordered dim-major points x[j*n + i]
   const void *xs;        // cell-sorted copy xs[j*n + p]
@@ -288,6 +299,34 @@ void select_list(const uint8_t *sel, uint32_t n, uint32_t *flags, uint32_t *pos,
                  uint32_t *scratch, hipStream_t s);
 template <typename T>
 void brute_join(const BruteArgs &a, hipStream_t s);
+
+// ------------------------------------------------------------- hybrid split (hybrid.hip)
+// Host copy of a grid index: cell-sorted coordinates xs[j*n + p], the point id of each sorted
+// position, the non-empty cell table.
+template <typename T>
+struct HybridIndex {
+  const T *xs;
+  const uint32_t *sortedIds;
+  const uint64_t *cellKeys;
+  const uint32_t *cellStart;
+  uint32_t ncell, n, d;
+};
+// splitWork's threshold (main.cu:2745-2770): queries of cells with at most this many points
+// go to the host
+uint32_t hybrid_point_threshold(uint32_t k1, uint32_t nd);
+// cpu[p] = 1 for the sorted positions whose own cell holds at most thr points; gpu[q] = 1 for
+// the points of the other cells
+void hybrid_flags(const uint32_t *cellStart, const uint32_t *cellOf, const uint32_t *sortedIds, uint32_t n,
+                  uint32_t thr, uint8_t *cpu, uint8_t *gpu, hipStream_t s);
+// exact rows of the points at sorted positions qpos[0..nq) on `threads` host threads (packed:
+// row i = the point at qpos[i])
+template <typename T>
+void hybrid_host_join(const HybridIndex<T> &ix, const typename GridOf<T>::type &g, uint32_t k1,
+                      const uint32_t *qpos, uint32_t nq, uint32_t threads, int32_t *outIds, T *outDists);
+// packed rows -> table rows of their queries
+template <typename T>
+void hybrid_scatter(const int32_t *rowIds, const T *rowDists, const uint32_t *list, uint32_t nq, uint32_t k1,
+                    int32_t *ids, T *dists, hipStream_t s);
 
 // deterministic sum of rowSum over the rows (chunks of 4096 rows, then chunk sums): out[0]
 void sum_rows(const double *rowSum, uint32_t n, double *partial, double *out, hipStream_t s);

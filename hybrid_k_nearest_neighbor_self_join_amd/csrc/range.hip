@@ -452,7 +452,14 @@ __device__ __forceinline__ void for_each_point_flat(const JoinArgs &a, const G &
 }
 
 // ====================================================================== select kernel
+// Buckets of the select kernel: 63 equal buckets over [0, Tq] of the FP64 key, and bucket 63
+// for every key beyond (or a NaN).  floor(min(key * scale, 63.5)) with scale = 63 / Tq is
+// monotone in the key (a product by a positive constant, a minimum and a truncation, each
+// monotone under round-to-nearest), so every key of bucket b is <= every key of bucket b + 1,
+// and a key of a bucket <= 62 is below Tq (1 + 2^-51) < r^2: such a point lies inside the
+// block of radius r.
 constexpr int NB = 64;
+constexpr int NBIN = NB - 1;
 constexpr uint32_t SELECT_MAX_GRID = 2048;
 // pass 2: sub-blocks of at most RMAX rows have their row ranges looked up for the whole tile
 // at once (lane = query) and their candidates loaded PR rounds of 64 at a time
@@ -462,21 +469,34 @@ template <typename T>
 size_t range_select_lds_bytes(uint32_t smax);
 namespace {
 
-template <typename T>
-__device__ __forceinline__ uint32_t bucket_of(T s, T scale) {
-  // s * scale >= 0; clamped below NB before the conversion (NB - 0.5 truncates to NB - 1)
-  const T t = s * scale;
-  return (uint32_t)(t < (T)NB - (T)0.5 ? t : (T)NB - (T)0.5);
+__device__ __forceinline__ uint32_t kbucket(double key, double scale) {
+  const double t = fmin(key * scale, (double)NBIN + 0.5);
+  return (uint32_t)t;
 }
 
 // LDS: hist[NB/2][64] u32, two 16-bit bucket counters per word (per-lane histogram
 // columns, then slot offsets), the collecting query's cursors qh[NB/2], rowB/rowX[64], the
 // tile's pass-2 row table prB/prX[RMAX][64] (first sorted position and exclusive offset of
 // each row of each query's sub-block), prT/prR[64] (candidates and rows per query), then the
-// per-query staging arrays key[SM] (double), id[SM] (u32), s[SM] (T).  A lane with 65536 or more candidates inside F goes to the heap
-// kernel, so no 16-bit counter or cursor can carry into its neighbour.
+// per-query staging arrays key[SM] (double), id[SM] (u32), s[SM] (T).  A lane with 65536 or
+// more candidates within Tq goes to the heap kernel, so no 16-bit counter of buckets 0..62
+// can carry into its neighbour (bucket 63, the high half of the last word, may wrap: it is
+// never read).
 __device__ __forceinline__ uint32_t hword(uint32_t b, uint32_t col) { return (b >> 1) * 64 + col; }
 __device__ __forceinline__ uint32_t hshift(uint32_t b) { return (b & 1u) << 4; }
+__device__ __forceinline__ uint32_t hget(const uint32_t *w, uint32_t b) { return (w[b >> 1] >> hshift(b)) & 0xffffu; }
+
+// Per tile (lane = query; radius r, Tq = r^2 (1 - 2^-20), FP64 key of each candidate):
+//   pass 1  histogram of the keys of the tile's union of blocks over the 64 buckets;
+//   scan    b* = first bucket whose cumulative count reaches K+1 (b* <= 62: success; the K+1
+//           nearest are exactly the items of buckets < b* plus the smallest of bucket b*, and
+//           every item of buckets <= b* lies in the block); each column becomes that query's
+//           bucket slot offsets (a counting sort);
+//   pass 2  (wave = query) the query's own sub-block of radius sqrt((b* + 1) / scale) is
+//           streamed, lane = candidate; the items of buckets <= b* go to LDS staging slots in
+//           bucket order (the same FP64 keys as pass 1, so exactly cum(b*) of them), each is
+//           ranked against the items of its own bucket only (bucket order is key order), and
+//           the first K+1 are written to the row.
 template <typename T, int D, typename G>
 __global__ __launch_bounds__(64) void k_range_select(JoinArgs a, G g0) {
   extern __shared__ __align__(16) unsigned char smem[];
@@ -512,17 +532,16 @@ __global__ __launch_bounds__(64) void k_range_select(JoinArgs a, G g0) {
     uint32_t lo[KNN_MAX_INDEXED], hi[KNN_MAX_INDEXED], ownLo[KNN_MAX_INDEXED], ownHi[KNN_MAX_INDEXED];
     uint64_t cellId;
     tile_setup<T, D, G>(a, g, tl, lane, L, lo, hi, ownLo, ownHi, cellId);
-    const T scale = (T)NB / L.F;
+    const double scale = L.Tq > 0.0 ? (double)NBIN / L.Tq : __builtin_inf();
     wave_lds_sync();  // the previous tile's readers of hist/staging are done
 #pragma unroll 8
     for (int w = 0; w < NB / 2; w++) hist[w * 64 + lane] = 0u;
     uint64_t tileCand = 0;
     uint32_t inF = 0;
 
-    // ---- pass 1 (lane = query): histogram of s over [0, F] for all queries of the tile
-    // (rows narrowed to the chords of radius sqrt(F (1 + 2^-15)), which holds every point
-    // with s <= F)
-    const double rF2 = (double)L.F * (1.0 + 0x1p-15);
+    // ---- pass 1 (lane = query): histogram of the FP64 keys of the union of the tile's blocks
+    // (rows narrowed to the chords of radius sqrt(Tq) with 3 or more indexed dims)
+    const double rF2 = L.Tq * (1.0 + 0x1p-15);
     auto narrow1 = [&](const uint32_t *cur, uint32_t &l0, uint32_t &h0) {
       return g.nd < 3 ? true : row_chord<T, D, G>(g, cur, L.qd, rF2, L.active, l0, h0);
     };
@@ -535,16 +554,16 @@ __global__ __launch_bounds__(64) void k_range_select(JoinArgs a, G g0) {
         for (int j = 0; j < D; j++) cv[j] = xs[(size_t)j * n + pp];
         const uint32_t m = (uint32_t)__builtin_amdgcn_readfirstlane((int)((pEnd - base) < 64u ? (pEnd - base) : 64u));
         for (uint32_t jj = 0; jj < m; jj++) {
-          T c[D];
+          double key = 0.0;
 #pragma unroll
-          for (int j = 0; j < D; j++) c[j] = readlane<T>(cv[j], (int)jj);
-          const T s = Num<T>::template sq<D>(c, L.qc);
-          // branch-free: lanes out of range add 0 (nearly every candidate is in range for
-          // some lane of the wave, so the LDS add would be issued anyway)
-          const bool in = L.active && s <= L.F;
-          const uint32_t b = bucket_of<T>(s, scale);
-          atomicAdd(&hist[hword(b, lane)], in ? (1u << hshift(b)) : 0u);
-          inF += in ? 1u : 0u;
+          for (int j = 0; j < D; j++) {
+            const double dx = (double)readlane<T>(cv[j], (int)jj) - L.qd[j];
+            key = key + dx * dx;
+          }
+          // branch-free: every lane adds to its own column (an inactive lane's is never read)
+          const uint32_t b = kbucket(key, scale);
+          inF += b < (uint32_t)NBIN ? 1u : 0u;
+          atomicAdd(&hist[hword(b, lane)], 1u << hshift(b));
         }
       }
     });
@@ -562,30 +581,29 @@ __global__ __launch_bounds__(64) void k_range_select(JoinArgs a, G g0) {
       }
       cum += h0;
       const uint32_t o1 = cum < 0xffffu ? cum : 0xffffu;
-      if (bstar == NB && cum + h1 >= K1) {
+      if (bstar == NB && 2 * w + 1 < NBIN && cum + h1 >= K1) {
         bstar = (uint32_t)(2 * w + 1);
         cumB = cum + h1;
       }
       cum += h1;
       hist[w * 64 + lane] = o0 | (o1 << 16);
     }
-    // buckets <= b* alone exceed the staging slots: the heap kernel takes the query, no
-    // collect pass
-    const bool preOvf = L.active && !big && bstar < (uint32_t)NB && cumB > SM;
-    const bool go = L.active && !big && bstar < (uint32_t)NB && !preOvf;
-    const T edge = go ? ((T)(bstar + 1) / scale) * Num<T>::margin : (T)0;
+    // buckets <= b* alone exceed the staging slots: the heap kernel takes the query
+    const bool preOvf = L.active && !big && bstar < (uint32_t)NBIN && cumB > SM;
+    const bool go = L.active && !big && bstar < (uint32_t)NBIN && !preOvf;
+    // every staged key is below (b* + 1) / scale (1 + 2^-51): the sub-block's reach
+    const double edge = go ? (double)(bstar + 1) / scale * (1.0 + 0x1p-40) : 0.0;
     bool okMine = false, ovfMine = preOvf;
     const uint64_t goMask = __ballot(go);
     wave_lds_sync();
 
-    // ---- pass 2 set-up (lane = query): the rows of each query's collect sub-block (radius
-    // sqrt(edge), the same bounds the per-query path computes) and their point ranges, for
-    // the whole tile at once: one memory round trip per tile instead of two per query.
-    // Queries with more than RMAX rows, or grids without the dense start table, take the
-    // per-query path.
+    // ---- pass 2 set-up (lane = query): the rows of each query's collect sub-block and their
+    // point ranges, for the whole tile at once: one memory round trip per tile instead of two
+    // per query.  Queries with more than RMAX rows, or grids without the dense start table,
+    // take the per-query path.
+    const T riL = go ? (T)next_up((T)sqrt(edge)) : (T)0;
     bool fastL = false;
     if (go && a.denseStart) {
-      const T riL = (T)next_up((T)sqrt((double)edge * (1.0 + 0x1p-16)));
       uint32_t sl[KNN_MAX_INDEXED], sh[KNN_MAX_INDEXED];
       uint32_t R = 1;
       for (uint32_t j = 0; j < g.nd; j++) {
@@ -637,9 +655,8 @@ __global__ __launch_bounds__(64) void k_range_select(JoinArgs a, G g0) {
     const uint64_t fastMask = __ballot(fastL);
     wave_lds_sync();
 
-    // ---- pass 2 (wave = query): for each query, scan its own sub-block (radius sqrt(edge)),
-    // stage the candidates of buckets <= b* (plus the margin) in LDS in bucket order, rank
-    // them exactly, write the row coalesced
+    // ---- pass 2 (wave = query): for each query, stream its own sub-block, stage the items of
+    // buckets <= b* in LDS in bucket order, rank each within its bucket, write the row
     for (uint32_t i = 0; i < tl.z; i++) {
       if (!((goMask >> i) & 1ull)) continue;
       T qi[D];
@@ -649,29 +666,24 @@ __global__ __launch_bounds__(64) void k_range_select(JoinArgs a, G g0) {
         qi[j] = readlane<T>(L.qc[j], (int)i);
         qdi[j] = (double)qi[j];
       }
-      const T Fi = readlane<T>(L.F, (int)i);
-      const T sci = readlane<T>(scale, (int)i);
-      const T edi = readlane<T>(edge, (int)i);
-      const double Tqi = readlane<double>(L.Tq, (int)i);
+      const double sci = readlane<double>(scale, (int)i);
       const uint32_t bsi = (uint32_t)__builtin_amdgcn_readlane((int)bstar, (int)i);
+      const uint32_t m = (uint32_t)__builtin_amdgcn_readlane((int)cumB, (int)i);
       const uint32_t qid = (uint32_t)__builtin_amdgcn_readlane((int)L.q, (int)i);
       if (lane < (uint32_t)(NB / 2)) qh[lane] = hist[lane * 64 + i];
       wave_lds_sync();
       bool ovf = false;
       auto stage = [&](const T *c, uint32_t cid) {
-        const T s = Num<T>::template sq<D>(c, qi);
-        if (s <= Fi) {
-          uint32_t b = bucket_of<T>(s, sci);
-          if (b <= bsi || s <= edi) {
-            b = b < bsi ? b : bsi;
-            const uint32_t slot = (atomicAdd(&qh[b >> 1], 1u << hshift(b)) >> hshift(b)) & 0xffffu;
-            if (slot < SM) {
-              skey[slot] = fp64_key<D, T>(c, qdi);
-              sid[slot] = cid;
-              ss[slot] = s;
-            } else {
-              ovf = true;
-            }
+        const double key = fp64_key<D, T>(c, qdi);
+        const uint32_t b = kbucket(key, sci);
+        if (b <= bsi) {
+          const uint32_t slot = (atomicAdd(&qh[b >> 1], 1u << hshift(b)) >> hshift(b)) & 0xffffu;
+          if (slot < SM) {
+            skey[slot] = key;
+            sid[slot] = cid;
+            ss[slot] = Num<T>::template sq<D>(c, qi);
+          } else {
+            ovf = true;
           }
         }
       };
@@ -712,8 +724,7 @@ __global__ __launch_bounds__(64) void k_range_select(JoinArgs a, G g0) {
             if (val[u]) stage(cv[u], cidv[u]);
         }
       } else {
-        // every staged candidate has key <= edge (1 + 2^-19): per-dim reach sqrt of that
-        const T ri = (T)next_up((T)sqrt((double)edi * (1.0 + 0x1p-16)));
+        const T ri = readlane<T>(riL, (int)i);
         uint32_t slo[KNN_MAX_INDEXED], shi[KNN_MAX_INDEXED];
         for (uint32_t j = 0; j < g.nd; j++) {
           T xj = qi[0];
@@ -733,55 +744,33 @@ __global__ __launch_bounds__(64) void k_range_select(JoinArgs a, G g0) {
         });
       }
       wave_lds_sync();
-      const uint32_t m = (qh[bsi >> 1] >> hshift(bsi)) & 0xffffu;
-      if (__ballot(ovf) || m > SM) {
+      // the staged count equals pass 1's cum(b*) (same keys, same buckets); anything else
+      // (an overflowing slot) sends the query to the heap kernel
+      if (__ballot(ovf) || hget(qh, bsi) != m) {
         if (lane == i) ovfMine = true;
         wave_lds_sync();
         continue;
       }
-      // rank: items of buckets <= b-2 all precede an item of bucket b, items of buckets >= b+2
-      // all follow it (a bucket is >= 1/64 of F wide, FP32/FP64 disagree by < 2^-19), so
-      // only buckets b-1..b+1 are compared
-      constexpr int MAXI = 8;
-      uint32_t rk[MAXI];
-      bool okHere = false;
-#pragma unroll
-      for (int u = 0; u < MAXI; u++) {
-        rk[u] = 0xffffffffu;
-        const uint32_t tI = lane + 64u * u;
+      // rank: an item of bucket b follows every item of the buckets below and precedes every
+      // item of the buckets above, so it is compared with its own bucket only
+      T *od = reinterpret_cast<T *>(a.outDists);
+      for (uint32_t t0 = 0; t0 < m; t0 += 64) {
+        const uint32_t tI = t0 + lane;
         if (tI < m) {
           const double kt = skey[tI];
           const uint32_t it = sid[tI];
-          uint32_t b = bucket_of<T>(ss[tI], sci);
-          b = b < bsi ? b : bsi;
-          const uint32_t w0 = b >= 2 ? (qh[(b - 2) >> 1] >> hshift(b - 2)) & 0xffffu : 0u;
-          const uint32_t w1 = b + 1 < bsi ? (qh[(b + 1) >> 1] >> hshift(b + 1)) & 0xffffu : m;
+          const uint32_t b = kbucket(kt, sci);
+          const uint32_t w0 = b ? hget(qh, b - 1) : 0u;
+          const uint32_t w1 = hget(qh, b);
           uint32_t r = w0;
-          uint32_t v = w0;
-          for (; v + 4 <= w1; v += 4) {  // 4 independent LDS reads per wait
-            const double k0 = skey[v], k1 = skey[v + 1], k2 = skey[v + 2], k3 = skey[v + 3];
-            const uint32_t i0 = sid[v], i1 = sid[v + 1], i2 = sid[v + 2], i3 = sid[v + 3];
-            r += (kless(k0, i0, kt, it) ? 1u : 0u) + (kless(k1, i1, kt, it) ? 1u : 0u) +
-                 (kless(k2, i2, kt, it) ? 1u : 0u) + (kless(k3, i3, kt, it) ? 1u : 0u);
-          }
-          for (; v < w1; v++) r += kless(skey[v], sid[v], kt, it) ? 1u : 0u;
-          rk[u] = r;
-          if (r == K1 - 1 && kt <= Tqi) okHere = true;
-        }
-      }
-      const bool okq = __ballot(okHere) != 0;
-      if (okq) {
-        T *od = reinterpret_cast<T *>(a.outDists);
-#pragma unroll
-        for (int u = 0; u < MAXI; u++) {
-          const uint32_t tI = lane + 64u * u;
-          if (tI < m && rk[u] < K1) {
-            a.outIds[C.row(qid) + rk[u]] = (int32_t)(sid[tI] - C.idBase);
-            od[C.row(qid) + rk[u]] = sqrt(ss[tI]);
+          for (uint32_t v = w0; v < w1; v++) r += kless(skey[v], sid[v], kt, it) ? 1u : 0u;
+          if (r < K1) {
+            a.outIds[C.row(qid) + r] = (int32_t)(it - C.idBase);
+            od[C.row(qid) + r] = sqrt(ss[tI]);
           }
         }
       }
-      if (lane == i) okMine = okq;
+      if (lane == i) okMine = true;
       wave_lds_sync();
     }
 

@@ -62,6 +62,13 @@ typedef struct {
                                 when N^2 w <= 2^30, w = max(1, (K+1)/9).  1 on, 0 off (the reference's
                                 loop), -1 (default) = on for KNN_MODE_LEARNED, off for
                                 KNN_MODE_BETA.  Rows are identical either way. */
+  uint32_t hybrid_threads;   /* hybrid CPU/GPU split (splitWork, main.cu:2733-2933), BETA schedule
+                                with host pointers: the queries whose own cell of the first grid
+                                holds at most ceil((K+1) 2^nd / V_nd) points are joined by this
+                                many host threads while the GPU runs the loop over the others;
+                                0 (default) = GPU only.  Rows are identical either way. */
+  uint32_t hybrid_max_queries; /* at most this many host queries, those of the sparsest cells
+                                first (0 = every query the criterion selects) */
 } knnjoin_params;
 
 typedef struct {
@@ -87,8 +94,9 @@ typedef struct {
   double ms_reference_total; /* the reference's totalTime (main.cu:440-445): everything but the
                                first index build, i.e. ms_total - ms_index_first (reorder,
                                epsilon / radius, every pass and re-index, checksum) */
-  uint32_t host_syncs;      /* blocking host waits on the device during the join loop */
-  uint32_t pad0;
+  uint32_t host_syncs;      /* blocking host waits on the device from the first index build on */
+  uint32_t hybrid_queries;  /* queries the host threads joined (hybrid split) */
+  double ms_hybrid_host;    /* wall time of the host share, concurrent with the GPU loop */
 } knnjoin_stats;
 
 /*
