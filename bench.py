@@ -122,6 +122,19 @@ def load_traffic(workload_key: str):
     return e
 
 
+def load_counters(workload_key: str):
+    """The range kernel's issue counters for this workload from a committed rocprofv3 --pmc
+    measurement (SQ_INSTS_VALU, SQ_WAVES, SQ_BUSY_CYCLES, ...): VALU wave instructions per
+    launch, the fraction of the VALU issue peak they use at the traced launch time, and VALU
+    lane instructions per (query, candidate) pair; None when no measurement exists."""
+    path = os.path.join(ROOT, "profiles", "k_range_select_counters.json")
+    try:
+        with open(path) as f:
+            return json.load(f).get(workload_key)
+    except (OSError, ValueError):
+        return None
+
+
 def suite_highdim(args):
     """5,000,000 points in 4-D and 6-D (uniform and skewed cubes), k=32, NUMINDEXEDDIM = d and
     d-2: plain vs learned schedule, one JSON line per case."""
@@ -394,6 +407,9 @@ def main():
             "avg_launch_ms": g["kernel_ms"] / launches,
             "launches_per_step": g["kernel_launches"] / max(args.steps, 1),
             "traffic_source": tr["source"] if tr else None,
+            # what binds instead of HBM: VALU issue (peak 256 CU x 4 SIMD x 2.4 GHz / 4 cycles
+            # per wave64 instruction = 614 G wave instructions/s) and its latency
+            "valu": load_counters(wkey),
         },
         "pipeline": {
             "ms_device_median": g["ms_device_median"],
