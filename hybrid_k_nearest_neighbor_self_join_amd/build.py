@@ -30,6 +30,9 @@ CLI = os.path.join(PKG, "knnjoin_main")
 
 SOURCES = ["prims.hip", "estimator.hip", "join.hip", "range.hip", "learned.hip", "batched.hip",
            "outlier.hip", "brute.hip", "hybrid.hip", "api.hip"]
+# select.hip is compiled once per part (float / double x three ranges of the dimensionality),
+# so its many kernel instantiations build in parallel
+SELECT_PARTS = 6
 ARCH = os.environ.get("KNN_OFFLOAD_ARCH", "gfx950")
 
 HIP_FLAGS = [
@@ -77,11 +80,14 @@ def build(verbose: bool = False, dtype: str = "both") -> str:
     srcs = [s for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
     common = [s for s in srcs if s != "api.hip"]
     jobs = [(s, os.path.join(BUILD, os.path.splitext(s)[0] + ".o"), []) for s in common]
+    jobs += [("select.hip", os.path.join(BUILD, f"select_{p}.o"), [f"-DKNN_SELECT_PART={p}"])
+             for p in range(SELECT_PARTS)]
+    common += [f"select_{p}" for p in range(SELECT_PARTS)]
     if dtype in ("float", "both"):
         jobs.append(("api.hip", os.path.join(BUILD, "api.o"), []))
     if dtype in ("double", "both"):
         jobs.append(("api.hip", os.path.join(BUILD, "api_double.o"), ["-DKNN_PY_DTYPE_DOUBLE"]))
-    with cf.ThreadPoolExecutor(max_workers=min(len(jobs), 6)) as ex:
+    with cf.ThreadPoolExecutor(max_workers=min(len(jobs), max(2, os.cpu_count() or 2), 16)) as ex:
         objs = list(ex.map(_compile, jobs))
     base = objs[:len(common)]
     if dtype in ("float", "both"):

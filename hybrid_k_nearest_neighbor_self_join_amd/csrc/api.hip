@@ -426,7 +426,15 @@ struct Pipeline {
   uint32_t lastSpan = 1;
   uint32_t lastActive = 0;   // queries left in tiles after the pre-check
   double avgReach = 1.0;     // typical search radius in cells (tile-span cost model)
-  static constexpr uint32_t kPassGrid = 2048;  // persistent range-kernel workgroups
+  // persistent range-kernel workgroups: 11 per CU on 256 CUs (the select kernel's LDS,
+  // about 14 KB per one-wave workgroup, admits 11 per CU).  The kernel is latency-bound, so
+  // residency pays: 10M mixed 2-D points, K = 64, learned / plain schedule: 2048 -> 34.7 /
+  // 60.6 ms per join, 2560 -> 31.1 / 54.9 ms, 2816 -> 30.5 / 53.6 ms; 32 buckets with up to
+  // 16 waves per CU: 30.0 / 57.5 ms (coarser buckets stage and rank more in the plain schedule).
+#ifndef KNN_PASS_GRID
+#define KNN_PASS_GRID 2816
+#endif
+  static constexpr uint32_t kPassGrid = KNN_PASS_GRID;
   static constexpr uint32_t kHeapGrid = 512;   // persistent heap-kernel workgroups
   const uint32_t *activeDev = nullptr;         // active count of the last tile build (device)
   const uint32_t *spanDev = nullptr;           // its tile-block span (device)
@@ -1248,12 +1256,13 @@ static void run_plain(Pipeline<T> &P, const knnjoin_params &prm, double beta, kn
       bytesAtMax = pr.bytes_total - prevBytes;
     }
     prevBytes = pr.bytes_total;
-    pairs = pr.pairs;
     const uint32_t nfail = pr.nfail;
     joinMs += pr.pass_ms;
     if (P.verbose)
-      printf("[knn] iter %u eps %.9g queries %u failed %u cells %u span %u overflow %u kernel %.3f ms\n",
-             numIter, (double)eps, nq, nfail, P.ncell, P.lastSpan, pr.noverflow, pr.kernel_ms);
+      printf("[knn] iter %u eps %.9g queries %u failed %u cells %u span %u overflow %u candidates %llu kernel %.3f ms\n",
+             numIter, (double)eps, nq, nfail, P.ncell, P.lastSpan, pr.noverflow,
+             (unsigned long long)(pr.pairs - pairs), pr.kernel_ms);
+    pairs = pr.pairs;
     frac = (nfail * 1.0) / (nq * 1.0);  // main.cu:402
     nq = nfail;
     numIter++;
@@ -1472,11 +1481,12 @@ static void run_learned(Pipeline<T> &P, const knnjoin_params &p, knnjoin_stats &
       bytesAtMax = pr.bytes_total - prevBytes;
     }
     prevBytes = pr.bytes_total;
-    pairs = pr.pairs;
     joinMs += pr.pass_ms;
     if (P.verbose)
-      printf("[knn] learned pass %u octave %d grid %.9g queries %u failed %u cells %u span %u overflow %u kernel %.3f ms\n",
-             passes, c, (double)P.g.eps, nq, pr.nfail, P.ncell, P.lastSpan, pr.noverflow, pr.kernel_ms);
+      printf("[knn] learned pass %u octave %d grid %.9g queries %u failed %u cells %u span %u overflow %u candidates %llu kernel %.3f ms\n",
+             passes, c, (double)P.g.eps, nq, pr.nfail, P.ncell, P.lastSpan, pr.noverflow,
+             (unsigned long long)(pr.pairs - pairs), pr.kernel_ms);
+    pairs = pr.pairs;
     passes++;
   }
   stats.host_syncs = P.syncs - syncs0;
@@ -2281,7 +2291,10 @@ static void batch_pass(const BatchPlan &B, BatchGroup &G, hipStream_t s) {
   batch_index(G, s);
 
   const JoinArgs a = batch_args(B, G);
-  const GridParams g0{};
+  // every tile takes its set's grid; g0 carries only the group's number of indexed dims (one
+  // per group: it depends on the dimensionality alone), which picks the kernel build
+  GridParams g0{};
+  g0.nd = G.hsets.empty() ? 0u : G.hsets[0].nd;
   batch_flags_u8(G.ids, G.actq, N, G.aflags, s);
   scan_exclusive_u32(G.aflags, G.apos, N, G.scratch, s);
   compact_active(G.ids, G.aflags, G.apos, N, G.act, s);

@@ -1,0 +1,507 @@
+// select.hip -- the select range kernel for gfx950: the exact top-(K+1) of each query of a tile
+// from the grid cells around it.
+//
+// Replaces kernel.cu:61-225 (range query over the adjacent cells), the pair buffer and the two
+// thrust::stable_sort_by_key of GPU.cu:1629-1634, and the host top-k of GPU.cu:3610-3704.
+//
+// Work unit: one wavefront per tile of <= 64 consecutive active queries of one tile block (a
+// window of cells along dim 0 in one row of the grid); lane = query.  Each lane has a search
+// radius r (the pass's one radius, or its own) and the tile scans the union of its lanes'
+// blocks (knn_range.h).
+//
+// Ranking: FP64 key = sum_j ((double)c_j - (double)q_j)^2, ties by point id.
+// Success (per query): at least K+1 candidates with key <= Tq = r^2 (1 - 2^-20).  Every point
+// with key <= Tq lies within r of the query in each indexed dim, hence inside the scanned
+// block, so a success is the exact global top-(K+1).  The reference's FP32 test
+// (kernel.cu:130-147) differs from this only for candidates within 2^-20 of eps.
+//
+// Buckets: 63 equal buckets over [0, Tq] of the FP64 key, and bucket 63 for every key beyond
+// (or a NaN).  floor(min(key * scale, 63.5)) with scale = 63 / Tq is monotone in the key (a
+// product by a positive constant, a minimum and a truncation, each monotone under
+// round-to-nearest), so every key of bucket b is <= every key of bucket b + 1, and a key of a
+// bucket <= 62 is below Tq (1 + 2^-51) < r^2: such a point lies inside the block of radius r.
+//
+// Per tile:
+//   pass 1  (lane = query) histogram of the keys of the tile's union of blocks over the 64
+//           buckets: every candidate of a round is broadcast to all lanes (v_readlane), each
+//           lane adds one to its own counter of the candidate's bucket (LDS, no return);
+//   scan    b* = first bucket whose cumulative count reaches K+1 (b* <= 62: success; the K+1
+//           nearest are exactly the items of buckets < b* plus the smallest of bucket b*, and
+//           every item of buckets <= b* lies in the block); each lane's counts become its slot
+//           cursors (a counting sort), stored as the query's own LDS row;
+//   pass 2  (wave = query) the query's own sub-block of radius sqrt((b* + 1) / scale) is
+//           streamed, lane = candidate; the items of buckets <= b* go to LDS staging slots in
+//           bucket order (the same FP64 keys as pass 1, so exactly cum(b*) of them), each is
+//           ranked against the items of its own bucket only (bucket order is key order), and
+//           the first K+1 are written to the row.  The sub-blocks' row ranges are looked up for
+//           the whole tile at once (lane = query).
+//   A query whose staged list exceeds `smax` LDS slots (e.g. thousands of duplicates in one
+//   bucket), or with 65536 or more candidates within Tq (16-bit counters), is flagged for the
+//   heap kernel (range.hip).
+//
+// This file is compiled once per KNN_SELECT_PART (0..5): float / double x three ranges of D,
+// each instantiating the kernel for every number of indexed dims 1..4 (all per-dim arrays in
+// registers) and for the grid's runtime count (5..8 indexed dims).
+#include "knn_range.h"
+
+#ifndef KNN_SELECT_PART
+#define KNN_SELECT_PART 0
+#endif
+
+namespace knn {
+namespace {
+
+// LDS words of the select kernel before the staging arrays (in order): H (pass-1 histogram,
+// then the query rows of slot cursors), rowB/rowX[64], prB/prX[RMAX][64], prT/prR[64].
+constexpr uint32_t SEL_FIXED_WORDS = SEL_HWORDS + 128 + SEL_RMAX * 128 + 128;
+
+__device__ __forceinline__ uint32_t kbucket(double key, double scale) {
+  const double t = fmin(key * scale, (double)SEL_NBIN + 0.5);
+  return (uint32_t)t;
+}
+// 16-bit slot cursor of bucket b in a query row (two per word)
+__device__ __forceinline__ uint32_t hget(const uint32_t *w, uint32_t b) {
+  return (w[b >> 1] >> ((b & 1u) << 4)) & 0xffffu;
+}
+
+// Streams `total` candidates in rounds of 64, lane = candidate (fi = round base + lane; lanes
+// past the end load the last candidate again): f(cv, m) gets the coordinates of a round and
+// its candidate count m (wave-uniform).  The next round's loads are issued before f runs on
+// the current one.  pos_of(fi) = sorted position of the fi-th candidate.
+template <typename T, int D, typename P, typename F>
+__device__ __forceinline__ void stream_rounds(const T *__restrict__ xs, size_t n, uint32_t total, uint32_t lane,
+                                              P &&pos_of, F &&f) {
+  if (total == 0) return;
+  auto load = [&](T (&v)[D], uint32_t f0) {
+    uint32_t fi = f0 + lane;
+    fi = fi < total ? fi : total - 1;
+    const uint32_t p = pos_of(fi);
+#pragma unroll
+    for (int j = 0; j < D; j++) v[j] = xs[(size_t)j * n + p];
+  };
+  T cv[D];
+  load(cv, 0);
+  for (uint32_t f0 = 0; f0 < total; f0 += 64) {
+    T nx[D];
+    const bool more = f0 + 64 < total;
+    if (more) load(nx, f0 + 64);
+    f(cv, total - f0 < 64u ? total - f0 : 64u);
+    if (more) {
+#pragma unroll
+      for (int j = 0; j < D; j++) cv[j] = nx[j];
+    }
+  }
+}
+
+template <typename T, int D, int ND, typename G>
+__global__ __launch_bounds__(64) void k_select(JoinArgs a, G g0) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  uint32_t *H = reinterpret_cast<uint32_t *>(smem);
+  uint32_t *rowB = H + SEL_HWORDS;
+  uint32_t *rowX = rowB + 64;
+  uint32_t *prB = rowX + 64;
+  uint32_t *prX = prB + SEL_RMAX * 64;
+  uint32_t *prT = prX + SEL_RMAX * 64;
+  uint32_t *prR = prT + 64;
+  const uint32_t SM = a.smax;
+  double *skey = reinterpret_cast<double *>(smem + SEL_FIXED_WORDS * sizeof(uint32_t));
+  uint32_t *sid = reinterpret_cast<uint32_t *>(skey + SM);
+  T *ss = reinterpret_cast<T *>(sid + SM + (SM & 1));
+  const uint32_t lane = threadIdx.x;
+  const T *__restrict__ xs = reinterpret_cast<const T *>(a.xs);
+  const size_t n = a.n;
+  const uint32_t NT = tile_count(a);
+  // pass-1 histogram, lane-pair layout: the counter of bucket b of lane L is the (L & 1) half
+  // of word b * 32 + (L >> 1), so a lane's update address is one shift-add of its bucket
+  uint32_t *hcol = H + (lane >> 1);
+  const uint32_t hval = (lane & 1u) ? 0x10000u : 1u;
+  const uint32_t hsh = (lane & 1u) << 4;
+
+  WaveTally tally;
+  for (;;) {
+    const uint32_t t = next_tile(a.tileCounter, lane);
+    if (t >= NT) break;
+    const uint4 tl = a.tiles[t];
+    const TileCtx<G> C = tile_ctx<G>(a, g0, tl);
+    const G &g = C.g;
+    const uint32_t nd = nd_of<ND>(g);
+    const uint32_t K1 = C.k1;
+    const T eps = (T)g.eps;
+    Lane<T, D> L;
+    uint32_t lo[nd_cap<ND>()], hi[nd_cap<ND>()];
+    tile_setup<T, D, ND, G>(a, g, tl, lane, L, lo, hi);
+    const double scale = L.Tq > 0.0 ? (double)SEL_NBIN / L.Tq : __builtin_inf();
+    wave_lds_sync();  // the previous tile's readers of H and the staging arrays are done
+#pragma unroll 8
+    for (uint32_t w = 0; w < SEL_NB / 2; w++) H[w * 64 + lane] = 0u;
+
+    // ---- pass 1 (lane = query): histogram of the FP64 keys of the union of the tile's blocks.
+    // While the tile has seen fewer than 65536 candidates no 16-bit counter can carry; past
+    // that each lane counts its candidates within Tq (from its counters at that point on).
+    uint64_t seen = 0;
+    bool counting = false;
+    uint32_t inF = 0;
+    auto add1 = [&](const T (&cv)[D], uint32_t jj) -> uint32_t {
+      double key = 0.0;
+#pragma unroll
+      for (int j = 0; j < D; j++) {
+        const double dx = (double)readlane<T>(cv[j], (int)jj) - L.qd[j];
+        key = key + dx * dx;
+      }
+      const uint32_t b = kbucket(key, scale);
+      atomicAdd(hcol + b * 32u, hval);
+      return b;
+    };
+    auto hist_round = [&](const T (&cv)[D], uint32_t m) {
+      if (!counting && seen + m >= 65536u) {
+        counting = true;
+        inF = 0;
+#pragma unroll 8
+        for (uint32_t b = 0; b < (uint32_t)SEL_NBIN; b++) inF += (hcol[b * 32u] >> hsh) & 0xffffu;
+      }
+      seen += m;
+      if (!counting) {
+        if (m == 64u) {
+#pragma unroll 16
+          for (uint32_t jj = 0; jj < 64u; jj++) add1(cv, jj);
+        } else {
+          for (uint32_t jj = 0; jj < m; jj++) add1(cv, jj);
+        }
+      } else {
+        for (uint32_t jj = 0; jj < m; jj++) inF += add1(cv, jj) < (uint32_t)SEL_NBIN ? 1u : 0u;
+      }
+    };
+    if constexpr (ND == 1 || ND == 2) {
+      // the union's rows (no chords below 3 indexed dims) concatenated: rounds mix rows
+      const uint64_t R = block_rows<ND>(g, lo, hi);
+      for (uint64_t rb = 0; rb < R; rb += 64) {
+        uint32_t pB, len;
+        row_batch<T, D, ND, G>(a, g, lo, hi, false, L.qd, 0.0, R, rb + lane, pB, len);
+        const uint32_t inc = wave_incl_sum(len, lane);
+        const uint32_t total = (uint32_t)__builtin_amdgcn_readlane((int)inc, 63);
+        const uint32_t excl = inc - len;
+        const uint64_t left = R - rb;
+        if (left <= 8) {
+          uint32_t X[8], Bs[8];
+#pragma unroll
+          for (int k = 0; k < 8; k++) {
+            X[k] = (uint32_t)__builtin_amdgcn_readlane((int)excl, k);
+            Bs[k] = (uint32_t)__builtin_amdgcn_readlane((int)pB, k);
+          }
+          const uint32_t nr = (uint32_t)left;
+          stream_rounds<T, D>(xs, n, total, lane, [&](uint32_t fi) {
+            uint32_t b = Bs[0], x = X[0];
+#pragma unroll
+            for (int k = 1; k < 8; k++)
+              if ((uint32_t)k < nr && fi >= X[k]) {
+                b = Bs[k];
+                x = X[k];
+              }
+            return b + (fi - x);
+          }, hist_round);
+        } else {
+          rowB[lane] = pB;
+          rowX[lane] = excl;
+          wave_lds_sync();
+          stream_rounds<T, D>(xs, n, total, lane, [&](uint32_t fi) {
+            uint32_t l = 0, h = 64;
+            while (h - l > 1) {
+              const uint32_t mid = (l + h) >> 1;
+              if (rowX[mid] <= fi) l = mid; else h = mid;
+            }
+            return rowB[l] + (fi - rowX[l]);
+          }, hist_round);
+          wave_lds_sync();
+        }
+      }
+    } else {
+      // rows one at a time, narrowed to the wave's union of chords of radius sqrt(Tq)
+      const double rF2 = L.Tq * (1.0 + 0x1p-15);
+      auto narrow1 = [&](const uint32_t *cur, uint32_t &l0, uint32_t &h0) {
+        return nd < 3 ? true : row_chord<T, D, ND, G>(g, cur, L.qd, rF2, L.active, l0, h0);
+      };
+      for_each_row<ND>(a, g, lo, hi, narrow1, [&](uint32_t pBeg, uint32_t pEnd) {
+        stream_rounds<T, D>(xs, n, pEnd - pBeg, lane, [&](uint32_t fi) { return pBeg + fi; }, hist_round);
+      });
+    }
+
+    // ---- scan: b* per query; each lane's counts become exclusive slot offsets (saturated past
+    // b*), written as the query's row of cursors: lane L's row at word L * SEL_RS
+    // a lane with 65536 or more candidates within Tq may have carried into its pair partner's
+    // counters (the high half of the same words): both go to the heap kernel
+    const bool bigOwn = counting && inF >= 65536u;
+    const bool big = bigOwn || __shfl_xor((int)bigOwn, 1, 64) != 0;
+    uint32_t cum = 0, bstar = SEL_NB, cumB = 0;
+    uint32_t col[SEL_NB / 2];
+#pragma unroll
+    for (uint32_t b = 0; b < (uint32_t)SEL_NB; b++) {
+      const uint32_t h = (hcol[b * 32u] >> hsh) & 0xffffu;
+      const uint32_t o = cum < 0xffffu ? cum : 0xffffu;
+      if (b & 1u) col[b >> 1] |= o << 16; else col[b >> 1] = o;
+      if (b < (uint32_t)SEL_NBIN && bstar == (uint32_t)SEL_NB && cum + h >= K1) {
+        bstar = b;
+        cumB = cum + h;
+      }
+      cum += h;
+    }
+    wave_lds_sync();  // every lane has read its counters
+    {
+      uint32_t *myRow = H + lane * SEL_RS;
+#pragma unroll
+      for (uint32_t w = 0; w < (uint32_t)SEL_NB / 2; w++) myRow[w] = col[w];
+    }
+    // buckets <= b* alone exceed the staging slots: the heap kernel takes the query
+    const bool preOvf = L.active && !big && bstar < (uint32_t)SEL_NBIN && cumB > SM;
+    const bool go = L.active && !big && bstar < (uint32_t)SEL_NBIN && !preOvf;
+    // every staged key is below (b* + 1) / scale (1 + 2^-51): the sub-block's reach
+    const double edge = go ? (double)(bstar + 1) / scale * (1.0 + 0x1p-40) : 0.0;
+    bool okMine = false, ovfMine = preOvf;
+    const uint64_t goMask = __ballot(go);
+
+    // ---- pass 2 set-up (lane = query): the rows of each query's sub-block and their point
+    // ranges, for the whole tile at once (one memory round trip per tile instead of two per
+    // query).  Queries with more than RMAX rows, or grids without the dense start table, take
+    // the per-query path.
+    const T riL = go ? (T)next_up((T)sqrt(edge)) : (T)0;
+    bool fastL = false;
+    if (go && a.denseStart) {
+      uint32_t sl[nd_cap<ND>()], sh[nd_cap<ND>()];
+      uint32_t R = 1;
+#pragma unroll
+      for (uint32_t j = 0; j < nd_cap<ND>(); j++) {
+        if (j >= nd) break;
+        const T xj = pick_t<T, D>(L.qc, j);
+        sl[j] = cell_coord<T>(next_down(xj - riL), (T)g.minArr[j], eps, g.nCells[j]);
+        sh[j] = cell_coord<T>(next_up(xj + riL), (T)g.minArr[j], eps, g.nCells[j]);
+        if (j >= 1) {
+          const uint32_t sp = sh[j] - sl[j] + 1;
+          R = (R <= SEL_RMAX && sp <= SEL_RMAX) ? R * sp : SEL_RMAX + 1;
+        }
+      }
+      if (R <= SEL_RMAX) {
+        fastL = true;
+        uint32_t cur[nd_cap<ND>()];
+#pragma unroll
+        for (uint32_t j = 0; j < nd_cap<ND>(); j++) cur[j] = sl[j < nd ? j : 0];
+        const double ri2L = (double)riL * (double)riL * (1.0 + 0x1p-15);
+        uint32_t acc = 0;
+#pragma unroll
+        for (uint32_t r = 0; r < SEL_RMAX; r++) {
+          if (r < R) {
+            uint32_t l0 = sl[0], h0 = sh[0];
+            const bool keep = nd >= 3 ? row_chord_lane<T, D, ND, G>(g, cur, L.qd, ri2L, l0, h0) : true;
+            uint32_t pB = 0, len = 0;
+            if (keep) {
+              const uint64_t rowBase = row_base<ND>(g, cur);
+              pB = a.denseStart[rowBase + l0];
+              len = a.denseStart[rowBase + h0 + 1] - pB;
+            }
+            prB[r * 64 + lane] = pB;
+            prX[r * 64 + lane] = acc;
+            acc += len;
+            next_row<ND>(cur, sl, sh, nd);
+          }
+        }
+        prT[lane] = acc;
+        prR[lane] = R;
+      }
+    }
+    const uint64_t fastMask = __ballot(fastL);
+    wave_lds_sync();
+
+    // ---- pass 2 (wave = query): for each query, stream its own sub-block, stage the items of
+    // buckets <= b* in LDS in bucket order, rank each within its bucket, write the row
+    for (uint32_t i = 0; i < tl.z; i++) {
+      if (!((goMask >> i) & 1ull)) continue;
+      T qi[D];
+      double qdi[D];
+#pragma unroll
+      for (int j = 0; j < D; j++) {
+        qi[j] = readlane<T>(L.qc[j], (int)i);
+        qdi[j] = (double)qi[j];
+      }
+      const double sci = readlane<double>(scale, (int)i);
+      const uint32_t bsi = (uint32_t)__builtin_amdgcn_readlane((int)bstar, (int)i);
+      const uint32_t m = (uint32_t)__builtin_amdgcn_readlane((int)cumB, (int)i);
+      const uint32_t qid = (uint32_t)__builtin_amdgcn_readlane((int)L.q, (int)i);
+      uint32_t *qh = H + i * SEL_RS;  // the query's slot cursors
+      bool ovf = false;
+      auto stage = [&](const T *c, uint32_t cid) {
+        const double key = fp64_key<D, T>(c, qdi);
+        const uint32_t b = kbucket(key, sci);
+        if (b <= bsi) {
+          const uint32_t sh = (b & 1u) << 4;
+          const uint32_t slot = (atomicAdd(&qh[b >> 1], 1u << sh) >> sh) & 0xffffu;
+          if (slot < SM) {
+            skey[slot] = key;
+            sid[slot] = cid;
+            ss[slot] = Num<T>::template sq<D>(c, qi);
+          } else {
+            ovf = true;
+          }
+        }
+      };
+      if ((fastMask >> i) & 1ull) {
+        // rows from the tile's table; PR rounds of candidates loaded before any is staged
+        constexpr int PR = D <= 4 ? 4 : (D <= 8 ? 2 : 1);
+        const uint32_t Ri = prR[i], tot = prT[i];
+        uint32_t X[SEL_RMAX], Bs[SEL_RMAX];
+#pragma unroll
+        for (uint32_t k = 0; k < SEL_RMAX; k++) {
+          X[k] = prX[k * 64 + i];
+          Bs[k] = prB[k * 64 + i];
+        }
+        for (uint32_t f0 = 0; f0 < tot; f0 += 64u * PR) {
+          T cv[PR][D];
+          uint32_t cidv[PR];
+          bool val[PR];
+#pragma unroll
+          for (int u = 0; u < PR; u++) {
+            const uint32_t fi = f0 + 64u * u + lane;
+            val[u] = fi < tot;
+            if (val[u]) {
+              uint32_t b = Bs[0], x = X[0];
+#pragma unroll
+              for (uint32_t k = 1; k < SEL_RMAX; k++)
+                if (k < Ri && fi >= X[k]) {
+                  b = Bs[k];
+                  x = X[k];
+                }
+              const uint32_t p = b + (fi - x);
+#pragma unroll
+              for (int j = 0; j < D; j++) cv[u][j] = xs[(size_t)j * n + p];
+              cidv[u] = a.sortedIds[p];
+            }
+          }
+#pragma unroll
+          for (int u = 0; u < PR; u++)
+            if (val[u]) stage(cv[u], cidv[u]);
+        }
+      } else {
+        const T ri = readlane<T>(riL, (int)i);
+        uint32_t slo[nd_cap<ND>()], shi[nd_cap<ND>()];
+#pragma unroll
+        for (uint32_t j = 0; j < nd_cap<ND>(); j++) {
+          if (j >= nd) break;
+          const T xj = pick_t<T, D>(qi, j);
+          slo[j] = cell_coord<T>(next_down(xj - ri), (T)g.minArr[j], eps, g.nCells[j]);
+          shi[j] = cell_coord<T>(next_up(xj + ri), (T)g.minArr[j], eps, g.nCells[j]);
+        }
+        const double ri2 = (double)ri * (double)ri * (1.0 + 0x1p-15);
+        for_each_point_flat<T, D, ND, G>(a, g, slo, shi, nd >= 3, qdi, ri2, lane, rowB, rowX, [&](uint32_t p) {
+          T c[D];
+#pragma unroll
+          for (int j = 0; j < D; j++) c[j] = xs[(size_t)j * n + p];
+          const uint32_t cid = a.sortedIds[p];  // with the coordinates: one memory round trip
+          stage(c, cid);
+        });
+      }
+      wave_lds_sync();
+      // the staged count equals pass 1's cum(b*) (same keys, same buckets); anything else
+      // (an overflowing slot) sends the query to the heap kernel
+      if (__ballot(ovf) || hget(qh, bsi) != m) {
+        if (lane == i) ovfMine = true;
+        wave_lds_sync();
+        continue;
+      }
+      // rank: an item of bucket b follows every item of the buckets below and precedes every
+      // item of the buckets above, so it is compared with its own bucket only (whose slots are
+      // [end of bucket b-1, end of bucket b), the cursors after staging)
+      T *od = reinterpret_cast<T *>(a.outDists);
+      for (uint32_t t0 = 0; t0 < m; t0 += 64) {
+        const uint32_t tI = t0 + lane;
+        if (tI < m) {
+          const double kt = skey[tI];
+          const uint32_t it = sid[tI];
+          const uint32_t b = kbucket(kt, sci);
+          const uint32_t w0 = b ? hget(qh, b - 1) : 0u;
+          const uint32_t w1 = hget(qh, b);
+          uint32_t r = w0, v = w0;
+          for (; v + 1 < w1; v += 2) {
+            const double k0 = skey[v], k1 = skey[v + 1];
+            const uint32_t i0 = sid[v], i1 = sid[v + 1];
+            r += kless(k0, i0, kt, it) ? 1u : 0u;
+            r += kless(k1, i1, kt, it) ? 1u : 0u;
+          }
+          if (v < w1) r += kless(skey[v], sid[v], kt, it) ? 1u : 0u;
+          if (r < K1) {
+            a.outIds[C.row(qid) + r] = (int32_t)(it - C.idBase);
+            od[C.row(qid) + r] = sqrt(ss[tI]);
+          }
+        }
+      }
+      if (lane == i) okMine = true;
+      wave_lds_sync();
+    }
+
+    ovfMine = ovfMine || (L.active && big);
+    if (L.active) {
+      if (ovfMine) {
+        a.overflow[L.q] = 1;
+      } else {
+        a.failed[L.q] = okMine ? 0 : 1;
+      }
+    }
+    const uint64_t om = __ballot(L.active && ovfMine);
+    if (lane == 0 && om) atomicAdd(a.overflowCount, (uint32_t)__popcll(om));
+    tile_accounting<T>(tally, tl, okMine, L.active && !okMine && !ovfMine, seen, K1);
+  }
+  flush_tally(a, tally, lane);
+}
+
+template <typename T, int D, int ND, typename G>
+void launch_nd(const JoinArgs &a, const G &g, hipStream_t s) {
+  const uint32_t grid = a.ntilesDev ? a.ntiles : range_select_grid(a.ntiles);
+  k_select<T, D, ND, G><<<grid, 64, range_select_lds_bytes<T>(a.smax), s>>>(a, g);
+}
+
+}  // namespace
+
+// Launches the select kernel for D data dims and nd indexed dims (nd <= 4 on a kernel built
+// for that count, else the runtime-count kernel).
+template <typename T, int D>
+void select_launch(const JoinArgs &a, const typename GridOf<T>::type &g, uint32_t nd, hipStream_t s) {
+  using G = typename GridOf<T>::type;
+  if (a.ntiles == 0) return;
+  if (nd == 1) {
+    launch_nd<T, D, 1, G>(a, g, s);
+  } else if (nd == 2) {
+    if constexpr (D >= 2) launch_nd<T, D, 2, G>(a, g, s);
+  } else if (nd == 3) {
+    if constexpr (D >= 3) launch_nd<T, D, 3, G>(a, g, s);
+  } else if (nd == 4) {
+    if constexpr (D >= 4) launch_nd<T, D, 4, G>(a, g, s);
+  } else {
+    launch_nd<T, D, 0, G>(a, g, s);
+  }
+}
+
+#define SEL_INST(T, D) \
+  template void select_launch<T, D>(const JoinArgs &, const GridOf<T>::type &, uint32_t, hipStream_t);
+#if KNN_SELECT_PART < 3
+#define KNN_SEL_T float
+#else
+#define KNN_SEL_T double
+#endif
+#if KNN_SELECT_PART % 3 == 0
+SEL_INST(KNN_SEL_T, 1)
+SEL_INST(KNN_SEL_T, 2)
+SEL_INST(KNN_SEL_T, 3)
+SEL_INST(KNN_SEL_T, 4)
+SEL_INST(KNN_SEL_T, 5)
+#elif KNN_SELECT_PART % 3 == 1
+SEL_INST(KNN_SEL_T, 6)
+SEL_INST(KNN_SEL_T, 7)
+SEL_INST(KNN_SEL_T, 8)
+SEL_INST(KNN_SEL_T, 9)
+SEL_INST(KNN_SEL_T, 10)
+#else
+SEL_INST(KNN_SEL_T, 11)
+SEL_INST(KNN_SEL_T, 12)
+SEL_INST(KNN_SEL_T, 13)
+SEL_INST(KNN_SEL_T, 14)
+SEL_INST(KNN_SEL_T, 15)
+SEL_INST(KNN_SEL_T, 16)
+#endif
+
+}  // namespace knn
