@@ -35,6 +35,9 @@ constexpr int SEL_NBIN = SEL_NB - 1;
 constexpr uint32_t SEL_RMAX = KNN_SELECT_RMAX;
 constexpr uint32_t SEL_RS = SEL_NB / 2 + 1;
 constexpr uint32_t SEL_HWORDS = (uint32_t)SEL_NB * 32 > 64 * SEL_RS ? (uint32_t)SEL_NB * 32 : 64 * SEL_RS;
+// LDS words of the select kernel before its staging arrays: H, rowB/rowX[64],
+// prB/prX[SEL_RMAX][64], prT/prR[64], the collecting query's 32-bit cursors qc[64]
+constexpr uint32_t SEL_FIXED_WORDS = SEL_HWORDS + 128 + SEL_RMAX * 128 + 128 + 64;
 // persistent grid of the select kernel when the tile count is known on the host: 11 one-wave
 // workgroups per CU on 256 CUs
 constexpr uint32_t SELECT_MAX_GRID = 2816;
@@ -440,11 +443,13 @@ __device__ __forceinline__ uint32_t wave_incl_sum(uint32_t v, uint32_t lane) {
   return v;
 }
 
-// Calls f(p) for every sorted position p of the block [lo, hi] (rows with nothing within
-// sqrt(r2) of qd dropped when chord is set), 64 positions per round, lane = position.  The
-// rows' ranges are looked up in parallel (lane = row, 64 rows per batch) and concatenated
-// through a wave prefix sum, so a round mixes rows and every lane has a candidate.
-// rowB / rowX: 64-entry LDS arrays of the wave.  Wave-uniform arguments.
+// Calls f(p, valid) for every sorted position p of the block [lo, hi] (rows with nothing
+// within sqrt(r2) of qd dropped when chord is set), 64 positions per round, lane = position;
+// every lane of the wave makes each round's call (lanes past the block's end with valid =
+// false and p = the block's last position), so f may use wave-wide operations.  The rows'
+// ranges are looked up in parallel (lane = row, 64 rows per batch) and concatenated through a
+// wave prefix sum, so a round mixes rows.  rowB / rowX: 64-entry LDS arrays of the wave.
+// Wave-uniform arguments.
 template <typename T, int D, int ND, typename G, typename F>
 __device__ __forceinline__ void for_each_point_flat(const JoinArgs &a, const G &g, const uint32_t *lo,
                                                     const uint32_t *hi, bool chord, const double *qd,
@@ -469,17 +474,16 @@ __device__ __forceinline__ void for_each_point_flat(const JoinArgs &a, const G &
       }
       const uint32_t nr = (uint32_t)left;
       for (uint32_t f0 = 0; f0 < total; f0 += 64) {
-        const uint32_t fi = f0 + lane;
-        if (fi < total) {
-          uint32_t b = Bs[0], x = X[0];
+        const bool valid = f0 + lane < total;
+        const uint32_t fi = valid ? f0 + lane : total - 1;
+        uint32_t b = Bs[0], x = X[0];
 #pragma unroll
-          for (int k = 1; k < 8; k++)
-            if ((uint32_t)k < nr && fi >= X[k]) {
-              b = Bs[k];
-              x = X[k];
-            }
-          f(b + (fi - x));
-        }
+        for (int k = 1; k < 8; k++)
+          if ((uint32_t)k < nr && fi >= X[k]) {
+            b = Bs[k];
+            x = X[k];
+          }
+        f(b + (fi - x), valid);
       }
       continue;
     }
@@ -487,16 +491,15 @@ __device__ __forceinline__ void for_each_point_flat(const JoinArgs &a, const G &
     rowX[lane] = excl;
     wave_lds_sync();
     for (uint32_t f0 = 0; f0 < total; f0 += 64) {
-      const uint32_t fi = f0 + lane;
-      if (fi < total) {
-        // last row whose exclusive offset is <= fi (rows of length 0 share the next offset)
-        uint32_t l = 0, h = 64;
-        while (h - l > 1) {
-          const uint32_t mid = (l + h) >> 1;
-          if (rowX[mid] <= fi) l = mid; else h = mid;
-        }
-        f(rowB[l] + (fi - rowX[l]));
+      const bool valid = f0 + lane < total;
+      const uint32_t fi = valid ? f0 + lane : total - 1;
+      // last row whose exclusive offset is <= fi (rows of length 0 share the next offset)
+      uint32_t l = 0, h = 64;
+      while (h - l > 1) {
+        const uint32_t mid = (l + h) >> 1;
+        if (rowX[mid] <= fi) l = mid; else h = mid;
       }
+      f(rowB[l] + (fi - rowX[l]), valid);
     }
     wave_lds_sync();
   }

@@ -378,7 +378,7 @@ uint32_t range_select_grid(uint32_t ntiles) {
 
 template <typename T>
 size_t range_select_lds_bytes(uint32_t smax) {
-  return ((size_t)SEL_HWORDS + 128 + SEL_RMAX * 128 + 128) * sizeof(uint32_t) + (size_t)smax * 8 +
+  return (size_t)SEL_FIXED_WORDS * sizeof(uint32_t) + (size_t)smax * 8 +
          (size_t)(smax + (smax & 1)) * 4 + (size_t)smax * sizeof(T);
 }
 

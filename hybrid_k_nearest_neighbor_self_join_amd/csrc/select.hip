@@ -51,17 +51,9 @@
 namespace knn {
 namespace {
 
-// LDS words of the select kernel before the staging arrays (in order): H (pass-1 histogram,
-// then the query rows of slot cursors), rowB/rowX[64], prB/prX[RMAX][64], prT/prR[64].
-constexpr uint32_t SEL_FIXED_WORDS = SEL_HWORDS + 128 + SEL_RMAX * 128 + 128;
-
 __device__ __forceinline__ uint32_t kbucket(double key, double scale) {
   const double t = fmin(key * scale, (double)SEL_NBIN + 0.5);
   return (uint32_t)t;
-}
-// 16-bit slot cursor of bucket b in a query row (two per word)
-__device__ __forceinline__ uint32_t hget(const uint32_t *w, uint32_t b) {
-  return (w[b >> 1] >> ((b & 1u) << 4)) & 0xffffu;
 }
 
 // Streams `total` candidates in rounds of 64, lane = candidate (fi = round base + lane; lanes
@@ -103,6 +95,7 @@ __global__ __launch_bounds__(64) void k_select(JoinArgs a, G g0) {
   uint32_t *prX = prB + SEL_RMAX * 64;
   uint32_t *prT = prX + SEL_RMAX * 64;
   uint32_t *prR = prT + 64;
+  uint32_t *qc = prR + 64;  // the collecting query's slot cursors, one 32-bit word per bucket
   const uint32_t SM = a.smax;
   double *skey = reinterpret_cast<double *>(smem + SEL_FIXED_WORDS * sizeof(uint32_t));
   uint32_t *sid = reinterpret_cast<uint32_t *>(skey + SM);
@@ -111,6 +104,13 @@ __global__ __launch_bounds__(64) void k_select(JoinArgs a, G g0) {
   const T *__restrict__ xs = reinterpret_cast<const T *>(a.xs);
   const size_t n = a.n;
   const uint32_t NT = tile_count(a);
+  // pass-2 loads: with ND > 0 a scalar base per array plus a 32-bit byte offset per lane (one
+  // shift per candidate instead of 64-bit address arithmetic per array); the host launches
+  // those builds only when every array fits in 4 GiB, and the ND = 0 build otherwise
+  const char *xsb[D];
+#pragma unroll
+  for (int j = 0; j < D; j++) xsb[j] = reinterpret_cast<const char *>(xs + (size_t)j * n);
+  const char *idb = reinterpret_cast<const char *>(a.sortedIds);
   // pass-1 histogram, lane-pair layout: the counter of bucket b of lane L is the (L & 1) half
   // of word b * 32 + (L >> 1), so a lane's update address is one shift-add of its bucket
   uint32_t *hcol = H + (lane >> 1);
@@ -289,7 +289,10 @@ __global__ __launch_bounds__(64) void k_select(JoinArgs a, G g0) {
         for (uint32_t r = 0; r < SEL_RMAX; r++) {
           if (r < R) {
             uint32_t l0 = sl[0], h0 = sh[0];
-            const bool keep = nd >= 3 ? row_chord_lane<T, D, ND, G>(g, cur, L.qd, ri2L, l0, h0) : true;
+            // each row narrowed to the chord of the query's ball (one query per lane here, so
+            // the chords cost a few operations per row of a tile, and drop a quarter or more of
+            // the sub-block's cells with 2 indexed dims)
+            const bool keep = nd >= 2 ? row_chord_lane<T, D, ND, G>(g, cur, L.qd, ri2L, l0, h0) : true;
             uint32_t pB = 0, len = 0;
             if (keep) {
               const uint64_t rowBase = row_base<ND>(g, cur);
@@ -310,7 +313,78 @@ __global__ __launch_bounds__(64) void k_select(JoinArgs a, G g0) {
     wave_lds_sync();
 
     // ---- pass 2 (wave = query): for each query, stream its own sub-block, stage the items of
-    // buckets <= b* in LDS in bucket order, rank each within its bucket, write the row
+    // buckets <= b* in LDS in bucket order, rank each within its bucket, write the row.
+    // (Loading the next query's first rounds before the current query's rank was measured
+    // slower: the kernel is bound by instruction issue, not by that memory round trip.)
+#ifndef KNN_SELECT_PR
+#define KNN_SELECT_PR 3
+#endif
+    constexpr int PR = D <= 4 ? KNN_SELECT_PR : (D <= 8 ? 2 : 1);
+    // a fast-path query's sub-block rows: candidate fi of the concatenated rows sits at
+    // fi + dlt0 + the steps stp[k] of the rows k it has passed (X[k] <= fi; rows past the
+    // query's count never start)
+    struct Rows {
+      uint32_t tot, dlt0, X[SEL_RMAX], stp[SEL_RMAX];
+    };
+    auto rows_of = [&](uint32_t q) {
+      Rows w;
+      const uint32_t Rq = prR[q];
+      w.tot = prT[q];
+      w.dlt0 = prB[q];
+      w.X[0] = 0;
+      w.stp[0] = 0;
+      uint32_t prev = w.dlt0;
+#pragma unroll
+      for (uint32_t k = 1; k < SEL_RMAX; k++) {
+        w.X[k] = k < Rq ? prX[k * 64 + q] : 0xffffffffu;
+        const uint32_t dk = prB[k * 64 + q] - w.X[k];
+        w.stp[k] = dk - prev;
+        prev = dk;
+      }
+      return w;
+    };
+    // positions and loads of rounds f0 .. f0 + 64 PR - 1 (lanes past the end load the last
+    // candidate again; rounds past the end are loaded too: guarding them costs more than the
+    // redundant loads)
+    auto load_group = [&](const Rows &w, uint32_t f0, T (&cv)[PR][D], uint32_t (&cid)[PR], bool (&val)[PR]) {
+      uint32_t pv[PR];
+#pragma unroll
+      for (int u = 0; u < PR; u++) {
+        uint32_t fi = f0 + 64u * u + lane;
+        val[u] = fi < w.tot;
+        fi = val[u] ? fi : w.tot - 1;
+        uint32_t p = fi + w.dlt0;
+#pragma unroll
+        for (uint32_t k = 1; k < SEL_RMAX; k++) p += fi >= w.X[k] ? w.stp[k] : 0u;
+        pv[u] = p;
+      }
+      if constexpr (ND > 0) {
+#pragma unroll
+        for (int u = 0; u < PR; u++) {
+          const uint32_t off = pv[u] * (uint32_t)sizeof(T);
+#pragma unroll
+          for (int j = 0; j < D; j++) cv[u][j] = *reinterpret_cast<const T *>(xsb[j] + off);
+          cid[u] = *reinterpret_cast<const uint32_t *>(idb + pv[u] * 4u);
+        }
+      } else {
+#pragma unroll
+        for (int u = 0; u < PR; u++) {
+#pragma unroll
+          for (int j = 0; j < D; j++) cv[u][j] = xs[(size_t)j * n + pv[u]];
+          cid[u] = a.sortedIds[pv[u]];
+        }
+      }
+    };
+    // every loaded value is consumed here, unconditionally: the compiler may not sink a load
+    // into the staging branch that uses it (a memory round trip per round)
+    auto consume = [&](T (&cv)[PR][D], uint32_t (&cid)[PR]) {
+#pragma unroll
+      for (int u = 0; u < PR; u++) {
+#pragma unroll
+        for (int j = 0; j < D; j++) asm volatile("" ::"v"(cv[u][j]));
+        asm volatile("" ::"v"(cid[u]));
+      }
+    };
     for (uint32_t i = 0; i < tl.z; i++) {
       if (!((goMask >> i) & 1ull)) continue;
       T qi[D];
@@ -324,58 +398,55 @@ __global__ __launch_bounds__(64) void k_select(JoinArgs a, G g0) {
       const uint32_t bsi = (uint32_t)__builtin_amdgcn_readlane((int)bstar, (int)i);
       const uint32_t m = (uint32_t)__builtin_amdgcn_readlane((int)cumB, (int)i);
       const uint32_t qid = (uint32_t)__builtin_amdgcn_readlane((int)L.q, (int)i);
-      uint32_t *qh = H + i * SEL_RS;  // the query's slot cursors
+      {
+        // the query's row of 16-bit cursors (lane i's exclusive offsets) widened to 32 bits,
+        // lane = bucket: a slot claim is then one atomic add of 1 on its bucket's word
+        const uint32_t *qh = H + i * SEL_RS;
+        qc[lane] = (qh[lane >> 1] >> ((lane & 1u) << 4)) & 0xffffu;
+        wave_lds_sync();
+      }
       bool ovf = false;
-      auto stage = [&](const T *c, uint32_t cid) {
-        const double key = fp64_key<D, T>(c, qdi);
-        const uint32_t b = kbucket(key, sci);
-        if (b <= bsi) {
-          const uint32_t sh = (b & 1u) << 4;
-          const uint32_t slot = (atomicAdd(&qh[b >> 1], 1u << sh) >> sh) & 0xffffu;
-          if (slot < SM) {
-            skey[slot] = key;
-            sid[slot] = cid;
-            ss[slot] = Num<T>::template sq<D>(c, qi);
-          } else {
-            ovf = true;
-          }
-        }
-      };
+      uint32_t nStaged = 0;  // wave-uniform: staging runs on every lane, valid or not
       if ((fastMask >> i) & 1ull) {
-        // rows from the tile's table; PR rounds of candidates loaded before any is staged
-        constexpr int PR = D <= 4 ? 4 : (D <= 8 ? 2 : 1);
-        const uint32_t Ri = prR[i], tot = prT[i];
-        uint32_t X[SEL_RMAX], Bs[SEL_RMAX];
-#pragma unroll
-        for (uint32_t k = 0; k < SEL_RMAX; k++) {
-          X[k] = prX[k * 64 + i];
-          Bs[k] = prB[k * 64 + i];
-        }
-        for (uint32_t f0 = 0; f0 < tot; f0 += 64u * PR) {
+        const Rows w = rows_of(i);
+        for (uint32_t f0 = 0; f0 < w.tot; f0 += 64u * PR) {
           T cv[PR][D];
           uint32_t cidv[PR];
           bool val[PR];
+          load_group(w, f0, cv, cidv, val);
+          consume(cv, cidv);
+          // the rounds' keys and buckets, then every slot claim, then the staging writes: the
+          // PR rounds' LDS atomics are in flight together
+          double keyv[PR];
+          uint32_t bv[PR], slot[PR];
+          bool take[PR];
 #pragma unroll
           for (int u = 0; u < PR; u++) {
-            const uint32_t fi = f0 + 64u * u + lane;
-            val[u] = fi < tot;
-            if (val[u]) {
-              uint32_t b = Bs[0], x = X[0];
-#pragma unroll
-              for (uint32_t k = 1; k < SEL_RMAX; k++)
-                if (k < Ri && fi >= X[k]) {
-                  b = Bs[k];
-                  x = X[k];
-                }
-              const uint32_t p = b + (fi - x);
-#pragma unroll
-              for (int j = 0; j < D; j++) cv[u][j] = xs[(size_t)j * n + p];
-              cidv[u] = a.sortedIds[p];
+            take[u] = false;
+            if (f0 + 64u * u < w.tot) {  // wave-uniform
+              keyv[u] = fp64_key<D, T>(cv[u], qdi);
+              bv[u] = kbucket(keyv[u], sci);
+              take[u] = val[u] && bv[u] <= bsi;
+              nStaged += (uint32_t)__popcll(__ballot(take[u]));
             }
           }
 #pragma unroll
-          for (int u = 0; u < PR; u++)
-            if (val[u]) stage(cv[u], cidv[u]);
+          for (int u = 0; u < PR; u++) {
+            slot[u] = 0;
+            if (take[u]) slot[u] = atomicAdd(&qc[bv[u]], 1u);
+          }
+#pragma unroll
+          for (int u = 0; u < PR; u++) {
+            if (take[u]) {
+              if (slot[u] < SM) {
+                skey[slot[u]] = keyv[u];
+                sid[slot[u]] = cidv[u];
+                ss[slot[u]] = Num<T>::template sq<D>(cv[u], qi);
+              } else {
+                ovf = true;
+              }
+            }
+          }
         }
       } else {
         const T ri = readlane<T>(riL, (int)i);
@@ -388,45 +459,67 @@ __global__ __launch_bounds__(64) void k_select(JoinArgs a, G g0) {
           shi[j] = cell_coord<T>(next_up(xj + ri), (T)g.minArr[j], eps, g.nCells[j]);
         }
         const double ri2 = (double)ri * (double)ri * (1.0 + 0x1p-15);
-        for_each_point_flat<T, D, ND, G>(a, g, slo, shi, nd >= 3, qdi, ri2, lane, rowB, rowX, [&](uint32_t p) {
+        for_each_point_flat<T, D, ND, G>(a, g, slo, shi, nd >= 2, qdi, ri2, lane, rowB, rowX,
+                                         [&](uint32_t p, bool valid) {
           T c[D];
 #pragma unroll
           for (int j = 0; j < D; j++) c[j] = xs[(size_t)j * n + p];
           const uint32_t cid = a.sortedIds[p];  // with the coordinates: one memory round trip
-          stage(c, cid);
+          const double key = fp64_key<D, T>(c, qdi);
+          const uint32_t b = kbucket(key, sci);
+          const bool take = valid && b <= bsi;
+          nStaged += (uint32_t)__popcll(__ballot(take));
+          if (take) {
+            const uint32_t slot = atomicAdd(&qc[b], 1u);
+            if (slot < SM) {
+              skey[slot] = key;
+              sid[slot] = cid;
+              ss[slot] = Num<T>::template sq<D>(c, qi);
+            } else {
+              ovf = true;
+            }
+          }
         });
       }
       wave_lds_sync();
       // the staged count equals pass 1's cum(b*) (same keys, same buckets); anything else
       // (an overflowing slot) sends the query to the heap kernel
-      if (__ballot(ovf) || hget(qh, bsi) != m) {
+      if (__ballot(ovf) || nStaged != m) {
         if (lane == i) ovfMine = true;
         wave_lds_sync();
         continue;
       }
       // rank: an item of bucket b follows every item of the buckets below and precedes every
       // item of the buckets above, so it is compared with its own bucket only (whose slots are
-      // [end of bucket b-1, end of bucket b), the cursors after staging)
-      T *od = reinterpret_cast<T *>(a.outDists);
+      // [end of bucket b-1, end of bucket b), the cursors after staging).  The row is written
+      // through a scalar base and the rank's byte offset (r < K+1 <= 2^31).
+      char *rowI = reinterpret_cast<char *>(a.outIds + C.row(qid));
+      char *rowD = reinterpret_cast<char *>(reinterpret_cast<T *>(a.outDists) + C.row(qid));
       for (uint32_t t0 = 0; t0 < m; t0 += 64) {
         const uint32_t tI = t0 + lane;
         if (tI < m) {
           const double kt = skey[tI];
           const uint32_t it = sid[tI];
+          const T st = ss[tI];
           const uint32_t b = kbucket(kt, sci);
-          const uint32_t w0 = b ? hget(qh, b - 1) : 0u;
-          const uint32_t w1 = hget(qh, b);
-          uint32_t r = w0, v = w0;
-          for (; v + 1 < w1; v += 2) {
+          // both cursors read at once (the empty asm consumes both loads where they are issued,
+          // so neither is deferred into a branch of its own)
+          const uint32_t wa = qc[b ? b - 1 : 0u], w1 = qc[b];
+          asm volatile("" ::"v"(wa), "v"(w1));
+          const uint32_t w0 = b ? wa : 0u;
+          uint32_t r = w0;
+          for (uint32_t v = w0; v < w1; v += 2) {
+            // two slots per step, loaded together; slot v + 1 <= w1 <= smax stays inside the
+            // staging arrays and counts only when it belongs to the bucket
             const double k0 = skey[v], k1 = skey[v + 1];
             const uint32_t i0 = sid[v], i1 = sid[v + 1];
-            r += kless(k0, i0, kt, it) ? 1u : 0u;
-            r += kless(k1, i1, kt, it) ? 1u : 0u;
+            const bool l0 = (k0 < kt) | ((k0 == kt) & (i0 < it));
+            const bool l1 = (k1 < kt) | ((k1 == kt) & (i1 < it));
+            r += (l0 ? 1u : 0u) + ((l1 & (v + 1 < w1)) ? 1u : 0u);
           }
-          if (v < w1) r += kless(skey[v], sid[v], kt, it) ? 1u : 0u;
           if (r < K1) {
-            a.outIds[C.row(qid) + r] = (int32_t)(it - C.idBase);
-            od[C.row(qid) + r] = sqrt(ss[tI]);
+            *reinterpret_cast<int32_t *>(rowI + r * 4u) = (int32_t)(it - C.idBase);
+            *reinterpret_cast<T *>(rowD + r * (uint32_t)sizeof(T)) = sqrt(st);
           }
         }
       }
@@ -463,7 +556,9 @@ template <typename T, int D>
 void select_launch(const JoinArgs &a, const typename GridOf<T>::type &g, uint32_t nd, hipStream_t s) {
   using G = typename GridOf<T>::type;
   if (a.ntiles == 0) return;
-  if (nd == 1) {
+  if ((uint64_t)a.n * sizeof(T) > 0xffffffffull) {
+    launch_nd<T, D, 0, G>(a, g, s);  // 64-bit addressing throughout
+  } else if (nd == 1) {
     launch_nd<T, D, 1, G>(a, g, s);
   } else if (nd == 2) {
     if constexpr (D >= 2) launch_nd<T, D, 2, G>(a, g, s);
