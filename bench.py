@@ -113,7 +113,7 @@ def load_traffic(workload_key: str):
     """HBM bytes per launch of the range kernel measured with rocprofv3 PMC counters
     (FETCH_SIZE / WRITE_SIZE, scaled by the same counters' reading on a copy of known size),
     committed beside the profile summaries; None when no measurement of this workload exists."""
-    path = os.path.join(ROOT, "profiles", "k_range_select_traffic.json")
+    path = os.path.join(ROOT, "profiles", "k_select_traffic.json")
     try:
         with open(path) as f:
             e = json.load(f).get(workload_key)
@@ -127,7 +127,7 @@ def load_counters(workload_key: str):
     measurement (SQ_INSTS_VALU, SQ_WAVES, SQ_BUSY_CYCLES, ...): VALU wave instructions per
     launch, the fraction of the VALU issue peak they use at the traced launch time, and VALU
     lane instructions per (query, candidate) pair; None when no measurement exists."""
-    path = os.path.join(ROOT, "profiles", "k_range_select_counters.json")
+    path = os.path.join(ROOT, "profiles", "k_select_counters.json")
     try:
         with open(path) as f:
             return json.load(f).get(workload_key)
@@ -154,8 +154,11 @@ def suite_highdim(args):
             m = _lib.KNN_MODE_LEARNED if mode == "learned" else _lib.KNN_MODE_BETA
             g = run_gpu(pts, 32, 0.025, m, args.steps, args.warmup, nd=nd, keep_rows=True)
             res[mode] = g
+            kms = g["stats"]["ms_range_kernel"]
             out[mode] = {"queries_per_s": n / (g["ms_per_step"] / 1e3), "ms_per_step": g["ms_per_step"],
                          "iterations": g["stats"]["iterations"], "candidates": g["stats"]["candidates"],
+                         "range_kernel_ms": kms,
+                         "candidates_per_s": g["stats"]["candidates"] / (kms / 1e3) if kms > 0 else None,
                          "checksum": g["stats"]["checksum"]}
         if "learned" in res:
             out["speedup_learned_vs_plain"] = res["plain"]["ms_per_step"] / res["learned"]["ms_per_step"]
@@ -183,6 +186,13 @@ def suite_batched(args):
     for _ in range(max(args.steps, 3)):
         ids, dists, st = batched.knnjoin_batched(pts, ks)
         times.append(st.ms_device)
+    if args.batched_only:
+        print(json.dumps({"suite": "batched", "sets": len(sets), "points": N,
+                          "batched": {"ms_device_median": float(np.median(times)),
+                                      "queries_per_s": N / (np.median(times) / 1e3),
+                                      "tile_queries": tq, "grid_blocks": gb,
+                                      "graph_replayed": st.graph_replayed}}), flush=True)
+        return
     lat = []
     mism = 0
     for i, (p_, k) in enumerate(sets):
@@ -304,6 +314,9 @@ def main():
     ap.add_argument("--suite", choices=["highdim", "batched", "brute", "hybrid"], default=None,
                     help="instead of the headline line: the 4-D/6-D cases, the 256-set batch, or "
                          "the brute-force path")
+    ap.add_argument("--batched-only", action="store_true",
+                    help="--suite batched: time the batched join only (no one-by-one comparison), "
+                         "for profiling")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 5 device copies of 1 GiB (the reference kernel for scaling "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE readings to bytes)")
@@ -402,7 +415,7 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": tr["bytes_per_launch"] if tr else None,
-            "kernel": "k_range_select",
+            "kernel": "k_select",
             "algo_bytes_per_launch": g["algo_bytes"] / launches,
             "avg_launch_ms": g["kernel_ms"] / launches,
             "launches_per_step": g["kernel_launches"] / max(args.steps, 1),

@@ -53,7 +53,7 @@ enum Slot {
   S_PTS, S_X, S_XS, S_KEYS, S_KEYS_TMP, S_IDS, S_IDS_TMP, S_CELLKEYS, S_CELLSTART, S_FLAGS, S_POS,
   S_ACT, S_NTILES, S_TILEOFF, S_TILES, S_FAILED, S_ROWSUM, S_OUTIDS, S_OUTDISTS, S_SCRATCH,
   S_HEAP, S_SMALL, S_PART, S_CTR, S_CSUM, S_HOSTFLAGS, S_OVERFLOW, S_STAGE, S_DENSE, S_BFLAGS,
-  S_BPOS, S_BSTART, S_CELLOF, S_CELLMAX, S_FCNT, S_RADIUS, S_RHIST, S_MODEL, S_TRAIN, S_OCC, S_ACTQ, S_BRUTE_PK, S_BRUTE_PI, S_BLIST, S_TILEMETA, S_HCPU, S_HGPU, S_HLIST, S_HROWI, S_HROWD, S_NSLOTS
+  S_BPOS, S_BSTART, S_CELLOF, S_CELLMAX, S_FCNT, S_RADIUS, S_RHIST, S_MODEL, S_TRAIN, S_OCC, S_ACTQ, S_BRUTE_PK, S_BRUTE_PI, S_BLIST, S_TILEMETA, S_HCPU, S_HGPU, S_HLIST, S_HROWI, S_HROWD, S_XS2, S_SIDS, S_SIDS2, S_NSLOTS
 };
 
 // Slots of one no-allocation call: regions of the static device scratch array handed out in
@@ -315,16 +315,25 @@ struct Pipeline {
   template <typename V>
   V *buf(Slot sl, size_t count) { return reinterpret_cast<V *>(g_cache.get(sl, sizeof(V) * count)); }
 
-  // populateNDGridIndexAndLookupArrayGPU (GPU.cu:3715-4067), all in HBM
+  // populateNDGridIndexAndLookupArrayGPU (GPU.cu:3715-4067), all in HBM.
+  // The first build of a join sorts the points (original ids) by cell; a later build (a
+  // re-index at another cell width) computes its keys from the previous build's cell-sorted
+  // copy and sorts those positions, so its gathers of coordinates and ids read a spatially
+  // ordered array (nearby positions, cached lines) instead of the input's random order.
+  // Within a cell the points then follow the previous order; every row and count is the same
+  // (rows are ranked by (key, id)).  The no-allocation path keeps one copy (original order).
   void build_index(const T *mn, const T *mx, T eps) {
     grid_dims<T>(mn, mx, nd, eps, g, totalCells);
     const int bits = bits_for(totalCells);
-    uint32_t *ids = buf<uint32_t>(S_IDS, n);
+    const bool fromPrev = xs != nullptr && sortedIds != nullptr && !g_arena;
+    const T *keySrc = fromPrev ? xs : x;
+    // the sort's payload: original ids (first build) or previous positions (later builds)
+    uint32_t *ids = buf<uint32_t>(fromPrev ? S_IDS : S_SIDS, n);
     uint32_t *ids_tmp = buf<uint32_t>(S_IDS_TMP, n);
     if (bits <= 32) {
       uint32_t *keys = buf<uint32_t>(S_KEYS, n);
       uint32_t *kt = buf<uint32_t>(S_KEYS_TMP, n);
-      cell_keys_u32<T>(x, n, g, keys, ids, s);
+      cell_keys_u32<T>(keySrc, n, g, keys, ids, s);
       radix_sort_pairs_u32(keys, ids, kt, ids_tmp, n, bits, scratch, s);
       uint32_t *flags = buf<uint32_t>(S_FLAGS, n + 1);
       uint32_t *pos = buf<uint32_t>(S_POS, n + 1);
@@ -339,7 +348,7 @@ struct Pipeline {
     } else {
       uint64_t *keys = buf<uint64_t>(S_KEYS, n);
       uint64_t *kt = buf<uint64_t>(S_KEYS_TMP, n);
-      cell_keys_u64<T>(x, n, g, keys, ids, s);
+      cell_keys_u64<T>(keySrc, n, g, keys, ids, s);
       radix_sort_pairs_u64(keys, ids, kt, ids_tmp, n, bits, scratch, s);
       uint32_t *flags = buf<uint32_t>(S_FLAGS, n + 1);
       uint32_t *pos = buf<uint32_t>(S_POS, n + 1);
@@ -352,12 +361,24 @@ struct Pipeline {
       cell_of(flags, pos, n, cellOf, s);
       ncellDev = pos + n;
     }
-    sortedIds = ids;
-    xs = buf<T>(S_XS, (size_t)n * d);
-    if (rowMajor)
-      gather_rows<T>(rowMajor, sortedIds, n, d, rowPerm, xs, s);
-    else
-      gather_soa<T>(x, sortedIds, n, d, xs, s);
+    if (fromPrev) {
+      const bool second = xsSlot == S_XS;
+      T *xsNew = buf<T>(second ? S_XS2 : S_XS, (size_t)n * d);
+      uint32_t *sidNew = buf<uint32_t>(second ? S_SIDS2 : S_SIDS, n);
+      gather_soa<T>(xs, ids, n, d, xsNew, s);
+      gather_u32(sortedIds, ids, n, sidNew, s);
+      xs = xsNew;
+      sortedIds = sidNew;
+      xsSlot = second ? S_XS2 : S_XS;
+    } else {
+      sortedIds = ids;
+      xs = buf<T>(S_XS, (size_t)n * d);
+      xsSlot = S_XS;
+      if (rowMajor)
+        gather_rows<T>(rowMajor, sortedIds, n, d, rowPerm, xs, s);
+      else
+        gather_soa<T>(x, sortedIds, n, d, xs, s);
+    }
     LAUNCH_CHECK();
     // dense cell-start table: O(1) row lookups when the grid is small enough (its counts are
     // scattered for the cells below the device-side cell count, so no host round trip here)
@@ -393,6 +414,7 @@ struct Pipeline {
     ncell = pin[PW_NCELL];
   }
   const uint32_t *ncellDev = nullptr;  // device copy of ncell (the scan total of the build)
+  Slot xsSlot = S_XS;                   // the slot holding the current cell-sorted copy
   Timer *tBuild = nullptr;              // stopped before the build's wait when set
   uint32_t syncs = 0;                   // stream waits of the passes and builds
   void sync() {

@@ -5,7 +5,7 @@
 // cell table {cellKeys[c], cellStart[c]} in HBM, plus a cell-sorted dim-major copy of the
 // coordinates (xs) so that every candidate range is one contiguous, coalesced stream.
 //
-#include "knn_launch.h"
+#include "knn_range.h"
 
 namespace knn {
 
@@ -53,6 +53,12 @@ __global__ void k_gather_soa(const T *__restrict__ x, const uint32_t *__restrict
   for (uint32_t j = 0; j < d; j++) xs[(size_t)j * n + p] = x[(size_t)j * n + i];
 }
 
+__global__ void k_gather_u32(const uint32_t *__restrict__ src, const uint32_t *__restrict__ idx, uint32_t n,
+                             uint32_t *__restrict__ dst) {
+  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < n) dst[p] = src[idx[p]];
+}
+
 // xs[j*n + p] = in[ids[p]*d + perm[j]]: the same values as k_gather_soa from the permuted
 // SoA copy, read from the row-major input, whose d coordinates of a point share a cache line
 template <typename T>
@@ -97,7 +103,7 @@ __global__ void k_compact_blocks(const uint32_t *cellStart, const uint32_t *flag
 // the dense cell-start table.  Fewer than K+1 points there means the query cannot succeed
 // at this radius: it is marked failed without a distance evaluation.  The per-cell maximum of
 // those counts feeds the algorithmic-bytes accounting (each query cell's block once).
-template <typename T, typename G>
+template <typename T, typename G, int ND>
 __global__ void k_prefilter(const T *__restrict__ xs, uint32_t n, G g, const T *radius, T rAll,
                             const uint32_t *__restrict__ dense, const uint32_t *__restrict__ sortedIds,
                             const uint8_t *activeSrc, uint8_t *failed, uint32_t k1,
@@ -111,28 +117,21 @@ __global__ void k_prefilter(const T *__restrict__ xs, uint32_t n, G g, const T *
   if (active) {
     const T r = radius ? radius[q] : rAll;
     const T eps = (T)g.eps;
-    uint32_t lo[KNN_MAX_INDEXED], hi[KNN_MAX_INDEXED], cur[KNN_MAX_INDEXED];
-    for (uint32_t j = 0; j < g.nd; j++) {
+    const uint32_t nd = nd_of<ND>(g);
+    // per-dim arrays over a constant bound (registers, not private scratch)
+    uint32_t lo[nd_cap<ND>()], hi[nd_cap<ND>()], cur[nd_cap<ND>()];
+#pragma unroll
+    for (uint32_t j = 0; j < nd_cap<ND>(); j++) {
+      if (j >= nd) break;
       const T xj = xs[(size_t)j * n + p];  // cell-sorted copy: the query's own coordinates, coalesced
       lo[j] = cell_coord<T>(next_down(xj - r), (T)g.minArr[j], eps, g.nCells[j]);
       hi[j] = cell_coord<T>(next_up(xj + r), (T)g.minArr[j], eps, g.nCells[j]);
       cur[j] = lo[j];
     }
-    for (;;) {
-      uint64_t rowBase = g.keyBase;
-      for (uint32_t j = 1; j < g.nd; j++) rowBase += (uint64_t)cur[j] * g.mult[j];
+    do {
+      const uint64_t rowBase = row_base<ND>(g, cur);
       cnt += dense[rowBase + hi[0] + 1] - dense[rowBase + lo[0]];
-      uint32_t j = 1;
-      while (j < g.nd) {
-        if (cur[j] < hi[j]) {
-          cur[j]++;
-          break;
-        }
-        cur[j] = lo[j];
-        j++;
-      }
-      if (j >= g.nd) break;
-    }
+    } while (next_row<ND>(cur, lo, hi, nd));
   }
   if (cellMax) {
     // cellOf is non-decreasing in p, so a cell's lanes are one run of the wave: the run's
@@ -431,6 +430,9 @@ void gather_rows(const T *in, const uint32_t *ids, uint32_t n, uint32_t d, const
                  hipStream_t s) {
   if (n) k_gather_rows<T><<<(n + 255) / 256, 256, 0, s>>>(in, ids, n, d, perm, xs);
 }
+void gather_u32(const uint32_t *src, const uint32_t *idx, uint32_t n, uint32_t *dst, hipStream_t s) {
+  if (n) k_gather_u32<<<(n + 255) / 256, 256, 0, s>>>(src, idx, n, dst);
+}
 void dense_cell_counts(const uint64_t *cellKeys, const uint32_t *cellStart, uint32_t ncell,
                        uint32_t *dense, hipStream_t s) {
   if (ncell) k_dense_counts<<<(ncell + 255) / 256, 256, 0, s>>>(cellKeys, cellStart, ncell, dense);
@@ -449,9 +451,16 @@ void prefilter(const T *xs, uint32_t n, const typename GridOf<T>::type &g, const
                T rAll, const uint32_t *dense, const uint32_t *sortedIds, const uint8_t *activeSrc,
                uint8_t *failed, uint32_t k1, const uint32_t *cellOf, uint32_t *cellMax,
                uint32_t *flags, hipStream_t s) {
-  if (n)
-    k_prefilter<T, typename GridOf<T>::type><<<(n + 255) / 256, 256, 0, s>>>(
-        xs, n, g, radius, rAll, dense, sortedIds, activeSrc, failed, k1, cellOf, cellMax, flags);
+  if (!n) return;
+  using G = typename GridOf<T>::type;
+  const dim3 grid((n + 255) / 256), block(256);
+  switch (g.nd) {
+    case 1: k_prefilter<T, G, 1><<<grid, block, 0, s>>>(xs, n, g, radius, rAll, dense, sortedIds, activeSrc, failed, k1, cellOf, cellMax, flags); break;
+    case 2: k_prefilter<T, G, 2><<<grid, block, 0, s>>>(xs, n, g, radius, rAll, dense, sortedIds, activeSrc, failed, k1, cellOf, cellMax, flags); break;
+    case 3: k_prefilter<T, G, 3><<<grid, block, 0, s>>>(xs, n, g, radius, rAll, dense, sortedIds, activeSrc, failed, k1, cellOf, cellMax, flags); break;
+    case 4: k_prefilter<T, G, 4><<<grid, block, 0, s>>>(xs, n, g, radius, rAll, dense, sortedIds, activeSrc, failed, k1, cellOf, cellMax, flags); break;
+    default: k_prefilter<T, G, 0><<<grid, block, 0, s>>>(xs, n, g, radius, rAll, dense, sortedIds, activeSrc, failed, k1, cellOf, cellMax, flags); break;
+  }
 }
 void cell_bytes(const uint32_t *cellMax, uint32_t ncell, unsigned long long *out, hipStream_t s) {
   if (ncell) {

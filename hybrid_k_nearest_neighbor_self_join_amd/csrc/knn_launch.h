@@ -80,6 +80,8 @@ void compact_cells_u64(const uint64_t *keys, const uint32_t *flags, const uint32
 // xs[j*n + p] = x[j*n + ids[p]]
 template <typename T>
 void gather_soa(const T *x, const uint32_t *ids, uint32_t n, uint32_t d, T *xs, hipStream_t s);
+// dst[p] = src[idx[p]]
+void gather_u32(const uint32_t *src, const uint32_t *idx, uint32_t n, uint32_t *dst, hipStream_t s);
 // xs[j*n + p] = in[ids[p]*d + perm[j]] from the row-major input (one cache line per point)
 struct DimPerm {
   uint32_t v[KNN_MAXDIM];

@@ -202,18 +202,20 @@ __device__ __forceinline__ void row_range(const JoinArgs &a, uint64_t rowBase, u
 }
 
 // Advances the odometer cur over indexed dims 1..nd-1 within [lo, hi]; false when done.
+// Written as selects on every digit (no early exit), so each element keeps a fixed index and
+// the array stays in registers.
 template <int ND>
 __device__ __forceinline__ bool next_row(uint32_t *cur, const uint32_t *lo, const uint32_t *hi, uint32_t nd) {
+  bool carry = true;
 #pragma unroll
   for (uint32_t j = 1; j < nd_cap<ND>(); j++) {
     if (j >= nd) break;
-    if (cur[j] < hi[j]) {
-      cur[j]++;
-      return true;
-    }
-    cur[j] = lo[j];
+    const bool wrap = cur[j] >= hi[j];
+    const uint32_t nxt = wrap ? lo[j] : cur[j] + 1;
+    cur[j] = carry ? nxt : cur[j];
+    carry = carry && wrap;
   }
-  return false;
+  return !carry;
 }
 
 template <int ND, typename G>

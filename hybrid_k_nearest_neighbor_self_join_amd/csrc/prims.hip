@@ -10,8 +10,12 @@ namespace knn {
 namespace {
 
 constexpr int SCAN_BLOCK = 256;
-constexpr int SCAN_ITEMS = 8;
-constexpr uint32_t SCAN_TILE = SCAN_BLOCK * SCAN_ITEMS;  // 2048
+constexpr int SCAN_ITEMS = 16;
+constexpr uint32_t SCAN_TILE = SCAN_BLOCK * SCAN_ITEMS;  // 4096
+// Up to this many tiles, each block of the down-sweep sums the tile totals before its own
+// (at most 32 KB, read from L2) instead of a recursive scan of the totals: two launches per
+// scan and no inter-block waiting.
+constexpr uint32_t SCAN_DIRECT_TILES = 8192;
 
 // inclusive scan of v over the 256 threads of a block; returns the exclusive prefix and
 // sets *total.  Uses __shfl_up inside a wave and 4 LDS slots across waves.
@@ -38,39 +42,73 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t *t
   return woff + inc - v;
 }
 
+// The thread's SCAN_ITEMS consecutive inputs of the tile (16-byte loads when the tile is
+// whole), zero past n.
+__device__ __forceinline__ void load_items(const uint32_t *in, uint32_t n, uint32_t base, uint32_t (&v)[SCAN_ITEMS]) {
+  if (base + SCAN_ITEMS <= n && (reinterpret_cast<uintptr_t>(in + base) & 15u) == 0) {
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; i += 4) {
+      const uint4 q = *reinterpret_cast<const uint4 *>(in + base + i);
+      v[i] = q.x;
+      v[i + 1] = q.y;
+      v[i + 2] = q.z;
+      v[i + 3] = q.w;
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; i++) v[i] = base + i < n ? in[base + i] : 0u;
+  }
+}
+
 __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_reduce(const uint32_t *in, uint32_t n,
                                                              uint32_t *sums) {
   const uint32_t base = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
+  uint32_t v[SCAN_ITEMS];
+  load_items(in, n, base, v);
   uint32_t acc = 0;
 #pragma unroll
-  for (int i = 0; i < SCAN_ITEMS; i++) {
-    uint32_t idx = base + i;
-    acc += idx < n ? in[idx] : 0u;
-  }
+  for (int i = 0; i < SCAN_ITEMS; i++) acc += v[i];
   uint32_t tot;
   (void)block_exclusive_scan(acc, &tot);
   if (threadIdx.x == 0) sums[blockIdx.x] = tot;
 }
 
+// Down-sweep of one tile.  The tile's offset is offs[blockIdx.x] (a scanned array of tile
+// totals), or, with direct set, the sum of sums[0 .. blockIdx.x) (raw tile totals).
 __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_down(const uint32_t *in, uint32_t *out,
-                                                           uint32_t n, const uint32_t *offs) {
+                                                           uint32_t n, const uint32_t *offs, int direct) {
   const uint32_t base = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
   uint32_t v[SCAN_ITEMS];
+  load_items(in, n, base, v);
+  uint32_t tileOff = 0;
+  if (direct) {
+    uint32_t a = 0;
+    for (uint32_t i = threadIdx.x; i < blockIdx.x; i += SCAN_BLOCK) a += offs[i];
+    uint32_t tot;
+    (void)block_exclusive_scan(a, &tot);
+    tileOff = tot;
+  } else if (offs) {
+    tileOff = offs[blockIdx.x];
+  }
   uint32_t acc = 0;
 #pragma unroll
-  for (int i = 0; i < SCAN_ITEMS; i++) {
-    uint32_t idx = base + i;
-    v[i] = idx < n ? in[idx] : 0u;
-    acc += v[i];
-  }
+  for (int i = 0; i < SCAN_ITEMS; i++) acc += v[i];
   uint32_t tot;
-  uint32_t ex = block_exclusive_scan(acc, &tot);
-  uint32_t off = (offs ? offs[blockIdx.x] : 0u) + ex;
+  uint32_t off = tileOff + block_exclusive_scan(acc, &tot);
+  uint32_t o[SCAN_ITEMS];
 #pragma unroll
   for (int i = 0; i < SCAN_ITEMS; i++) {
-    uint32_t idx = base + i;
-    if (idx < n) out[idx] = off;
+    o[i] = off;
     off += v[i];
+  }
+  if (base + SCAN_ITEMS <= n && (reinterpret_cast<uintptr_t>(out + base) & 15u) == 0) {
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; i += 4)
+      *reinterpret_cast<uint4 *>(out + base + i) = make_uint4(o[i], o[i + 1], o[i + 2], o[i + 3]);
+  } else {
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; i++)
+      if (base + i < n) out[base + i] = o[i];
   }
   if (blockIdx.x == gridDim.x - 1 && threadIdx.x == SCAN_BLOCK - 1) out[n] = off;
 }
@@ -96,13 +134,17 @@ void scan_exclusive_u32(const uint32_t *in, uint32_t *out, uint32_t n, uint32_t 
   }
   uint32_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;
   if (nb == 1) {
-    k_scan_down<<<1, SCAN_BLOCK, 0, s>>>(in, out, n, nullptr);
+    k_scan_down<<<1, SCAN_BLOCK, 0, s>>>(in, out, n, nullptr, 0);
     return;
   }
   uint32_t *sums = scratch;  // nb + 1 words
   k_scan_reduce<<<nb, SCAN_BLOCK, 0, s>>>(in, n, sums);
+  if (nb <= SCAN_DIRECT_TILES) {
+    k_scan_down<<<nb, SCAN_BLOCK, 0, s>>>(in, out, n, sums, 1);
+    return;
+  }
   scan_exclusive_u32(sums, sums, nb, scratch + nb + 1, s);
-  k_scan_down<<<nb, SCAN_BLOCK, 0, s>>>(in, out, n, sums);
+  k_scan_down<<<nb, SCAN_BLOCK, 0, s>>>(in, out, n, sums, 0);
 }
 
 // ------------------------------------------------------------------------- radix sort
