@@ -96,6 +96,7 @@ __global__ __launch_bounds__(64) void k_select(JoinArgs a, G g0) {
   uint32_t *prT = prX + SEL_RMAX * 64;
   uint32_t *prR = prT + 64;
   uint32_t *qc = prR + 64;  // the collecting query's slot cursors, one 32-bit word per bucket
+  double *cbuf = reinterpret_cast<double *>(prB);  // pass 1: the round's candidates (64 x D)
   const uint32_t SM = a.smax;
   double *skey = reinterpret_cast<double *>(smem + SEL_FIXED_WORDS * sizeof(uint32_t));
   uint32_t *sid = reinterpret_cast<uint32_t *>(skey + SM);
@@ -116,6 +117,11 @@ __global__ __launch_bounds__(64) void k_select(JoinArgs a, G g0) {
   // FP64 work (5M points, K = 32, plain schedule: 2 of 4 dims indexed, uniform 855 -> 668 ms,
   // skewed 2658 -> 2164 ms; 4 of 6 dims indexed, uniform 1261 -> 1157 ms).
   constexpr bool kPrefilter = std::is_same<T, float>::value && (ND == 0 || D > ND);
+  // Pass 1 broadcasts each round's candidates as FP64 coordinates through LDS (one uniform-
+  // address read per candidate instead of D readlanes and D conversions per lane); the round
+  // buffer uses the pass-2 row tables (prB/prX, 2.5 KB), which are free until the scan.
+  constexpr bool kLdsBcast = !kPrefilter && D <= 4;
+  static_assert(!kLdsBcast || SEL_RMAX * 2 * 64 * 4 >= 64 * D * 8, "round buffer exceeds the row tables");
   // pass-1 histogram, lane-pair layout: the counter of bucket b of lane L is the (L & 1) half
   // of word b * 32 + (L >> 1), so a lane's update address is one shift-add of its bucket
   uint32_t *hcol = H + (lane >> 1);
@@ -164,12 +170,52 @@ __global__ __launch_bounds__(64) void k_select(JoinArgs a, G g0) {
         double key = 0.0;
 #pragma unroll
         for (int j = 0; j < D; j++) {
-          const double dx = (double)readlane<T>(cv[j], (int)jj) - L.qd[j];
+          double cj;
+          if constexpr (kLdsBcast)
+            cj = cbuf[jj * D + j];
+          else
+            cj = (double)readlane<T>(cv[j], (int)jj);
+          const double dx = cj - L.qd[j];
           key = key + dx * dx;
         }
         const uint32_t b = kbucket(key, scale);
         atomicAdd(hcol + b * 32u, hval);
         return b;
+      }
+    };
+    // a full round with the LDS broadcast: the coordinates of the next GRP candidates are read
+    // before the current GRP's histogram updates are issued (LDS reads and atomics keep their
+    // program order, so a read issued after an update would wait for it)
+    auto add64 = [&]() {
+      constexpr int GRP = D <= 2 ? 4 : 2;
+      double nx[GRP][D];
+#pragma unroll
+      for (int k = 0; k < GRP; k++)
+#pragma unroll
+        for (int j = 0; j < D; j++) nx[k][j] = cbuf[k * D + j];
+#pragma unroll
+      for (int g = 0; g < 64 / GRP; g++) {
+        double cu[GRP][D];
+#pragma unroll
+        for (int k = 0; k < GRP; k++)
+#pragma unroll
+          for (int j = 0; j < D; j++) cu[k][j] = nx[k][j];
+        if (g + 1 < 64 / GRP) {
+#pragma unroll
+          for (int k = 0; k < GRP; k++)
+#pragma unroll
+            for (int j = 0; j < D; j++) nx[k][j] = cbuf[((g + 1) * GRP + k) * D + j];
+        }
+#pragma unroll
+        for (int k = 0; k < GRP; k++) {
+          double key = 0.0;
+#pragma unroll
+          for (int j = 0; j < D; j++) {
+            const double dx = cu[k][j] - L.qd[j];
+            key = key + dx * dx;
+          }
+          atomicAdd(hcol + kbucket(key, scale) * 32u, hval);
+        }
       }
     };
     auto hist_round = [&](const T (&cv)[D], uint32_t m) {
@@ -180,10 +226,20 @@ __global__ __launch_bounds__(64) void k_select(JoinArgs a, G g0) {
         for (uint32_t b = 0; b < (uint32_t)SEL_NBIN; b++) inF += (hcol[b * 32u] >> hsh) & 0xffffu;
       }
       seen += m;
+      if constexpr (kLdsBcast) {
+        wave_lds_sync();  // the previous round's readers are done
+#pragma unroll
+        for (int j = 0; j < D; j++) cbuf[lane * D + j] = (double)cv[j];
+        wave_lds_sync();
+      }
       if (!counting) {
         if (m == 64u) {
+          if constexpr (kLdsBcast) {
+            add64();
+          } else {
 #pragma unroll 16
-          for (uint32_t jj = 0; jj < 64u; jj++) add1(cv, jj);
+            for (uint32_t jj = 0; jj < 64u; jj++) add1(cv, jj);
+          }
         } else {
           for (uint32_t jj = 0; jj < m; jj++) add1(cv, jj);
         }
