@@ -97,6 +97,7 @@ __global__ __launch_bounds__(64) void k_select(JoinArgs a, G g0) {
   uint32_t *prR = prT + 64;
   uint32_t *qc = prR + 64;  // the collecting query's slot cursors, one 32-bit word per bucket
   double *cbuf = reinterpret_cast<double *>(prB);  // pass 1: the round's candidates (64 x D)
+  float *cbufF = reinterpret_cast<float *>(prB);
   const uint32_t SM = a.smax;
   double *skey = reinterpret_cast<double *>(smem + SEL_FIXED_WORDS * sizeof(uint32_t));
   uint32_t *sid = reinterpret_cast<uint32_t *>(skey + SM);
@@ -114,14 +115,18 @@ __global__ __launch_bounds__(64) void k_select(JoinArgs a, G g0) {
   const char *idb = reinterpret_cast<const char *>(a.sortedIds);
   // With more data dims than indexed dims (float), the grid prunes only a projection and most
   // candidates of a block lie outside a query's ball: their FP32 sum decides that before any
-  // FP64 work (5M points, K = 32, plain schedule: 2 of 4 dims indexed, uniform 855 -> 668 ms,
-  // skewed 2658 -> 2164 ms; 4 of 6 dims indexed, uniform 1261 -> 1157 ms).
+  // FP64 work (5M points, K = 32, plain schedule, with the LDS broadcast below: 2 of 4 dims
+  // indexed, uniform 855 -> 597 ms, skewed 2658 -> 2044 ms; 4 of 6 dims indexed, uniform
+  // 1261 -> 1074 ms).
   constexpr bool kPrefilter = std::is_same<T, float>::value && (ND == 0 || D > ND);
   // Pass 1 broadcasts each round's candidates as FP64 coordinates through LDS (one uniform-
   // address read per candidate instead of D readlanes and D conversions per lane); the round
   // buffer uses the pass-2 row tables (prB/prX, 2.5 KB), which are free until the scan.
   constexpr bool kLdsBcast = !kPrefilter && D <= 4;
   static_assert(!kLdsBcast || SEL_RMAX * 2 * 64 * 4 >= 64 * D * 8, "round buffer exceeds the row tables");
+  // with the pre-filter, the round's FP32 coordinates go through LDS the same way
+  constexpr bool kLdsBcastF = kPrefilter && D <= 8;
+  static_assert(!kLdsBcastF || SEL_RMAX * 2 * 64 * 4 >= 64 * D * 4, "round buffer exceeds the row tables");
   // pass-1 histogram, lane-pair layout: the counter of bucket b of lane L is the (L & 1) half
   // of word b * 32 + (L >> 1), so a lane's update address is one shift-add of its bucket
   uint32_t *hcol = H + (lane >> 1);
@@ -218,6 +223,51 @@ __global__ __launch_bounds__(64) void k_select(JoinArgs a, G g0) {
         }
       }
     };
+    // the same with the pre-filter: the FP32 sum (fused multiply-adds: within 2^-19 of the
+    // key, like the reported-distance sum) decides whether the FP64 key is needed
+    auto add64f = [&]() {
+      constexpr int GRP = 2;
+      float qf[D];
+#pragma unroll
+      for (int j = 0; j < D; j++) qf[j] = (float)L.qc[j];
+      float nx[GRP][D];
+#pragma unroll
+      for (int k = 0; k < GRP; k++)
+#pragma unroll
+        for (int j = 0; j < D; j++) nx[k][j] = cbufF[k * D + j];
+#pragma unroll
+      for (int g = 0; g < 64 / GRP; g++) {
+        float cu[GRP][D];
+#pragma unroll
+        for (int k = 0; k < GRP; k++)
+#pragma unroll
+          for (int j = 0; j < D; j++) cu[k][j] = nx[k][j];
+        if (g + 1 < 64 / GRP) {
+#pragma unroll
+          for (int k = 0; k < GRP; k++)
+#pragma unroll
+            for (int j = 0; j < D; j++) nx[k][j] = cbufF[((g + 1) * GRP + k) * D + j];
+        }
+#pragma unroll
+        for (int k = 0; k < GRP; k++) {
+          float sf = 0.0f;
+#pragma unroll
+          for (int j = 0; j < D; j++) {
+            const float dj = cu[k][j] - qf[j];
+            sf = __builtin_fmaf(dj, dj, sf);
+          }
+          if (sf <= (float)L.F) {
+            double key = 0.0;
+#pragma unroll
+            for (int j = 0; j < D; j++) {
+              const double dx = (double)cu[k][j] - L.qd[j];
+              key = key + dx * dx;
+            }
+            atomicAdd(hcol + kbucket(key, scale) * 32u, hval);
+          }
+        }
+      }
+    };
     auto hist_round = [&](const T (&cv)[D], uint32_t m) {
       if (!counting && seen + m >= 65536u) {
         counting = true;
@@ -231,11 +281,18 @@ __global__ __launch_bounds__(64) void k_select(JoinArgs a, G g0) {
 #pragma unroll
         for (int j = 0; j < D; j++) cbuf[lane * D + j] = (double)cv[j];
         wave_lds_sync();
+      } else if constexpr (kLdsBcastF) {
+        wave_lds_sync();
+#pragma unroll
+        for (int j = 0; j < D; j++) cbufF[lane * D + j] = (float)cv[j];
+        wave_lds_sync();
       }
       if (!counting) {
         if (m == 64u) {
           if constexpr (kLdsBcast) {
             add64();
+          } else if constexpr (kLdsBcastF) {
+            add64f();
           } else {
 #pragma unroll 16
             for (uint32_t jj = 0; jj < 64u; jj++) add1(cv, jj);
