@@ -445,67 +445,5 @@ __device__ __forceinline__ uint32_t wave_incl_sum(uint32_t v, uint32_t lane) {
   return v;
 }
 
-// Calls f(p, valid) for every sorted position p of the block [lo, hi] (rows with nothing
-// within sqrt(r2) of qd dropped when chord is set), 64 positions per round, lane = position;
-// every lane of the wave makes each round's call (lanes past the block's end with valid =
-// false and p = the block's last position), so f may use wave-wide operations.  The rows'
-// ranges are looked up in parallel (lane = row, 64 rows per batch) and concatenated through a
-// wave prefix sum, so a round mixes rows.  rowB / rowX: 64-entry LDS arrays of the wave.
-// Wave-uniform arguments.
-template <typename T, int D, int ND, typename G, typename F>
-__device__ __forceinline__ void for_each_point_flat(const JoinArgs &a, const G &g, const uint32_t *lo,
-                                                    const uint32_t *hi, bool chord, const double *qd,
-                                                    double r2, uint32_t lane, uint32_t *rowB,
-                                                    uint32_t *rowX, F &&f) {
-  const uint64_t R = block_rows<ND>(g, lo, hi);
-  for (uint64_t rb = 0; rb < R; rb += 64) {
-    uint32_t pB, len;
-    row_batch<T, D, ND, G>(a, g, lo, hi, chord, qd, r2, R, rb + lane, pB, len);
-    const uint32_t inc = wave_incl_sum(len, lane);
-    const uint32_t total = (uint32_t)__builtin_amdgcn_readlane((int)inc, 63);
-    const uint32_t excl = inc - len;  // exclusive offset, non-decreasing over the lanes
-    const uint64_t left = R - rb;
-    if (left <= 8) {
-      // few rows (the usual case with <= 2 indexed dims): offsets and starts in scalar
-      // registers, a candidate's row found by compares
-      uint32_t X[8], Bs[8];
-#pragma unroll
-      for (int k = 0; k < 8; k++) {
-        X[k] = (uint32_t)__builtin_amdgcn_readlane((int)excl, k);
-        Bs[k] = (uint32_t)__builtin_amdgcn_readlane((int)pB, k);
-      }
-      const uint32_t nr = (uint32_t)left;
-      for (uint32_t f0 = 0; f0 < total; f0 += 64) {
-        const bool valid = f0 + lane < total;
-        const uint32_t fi = valid ? f0 + lane : total - 1;
-        uint32_t b = Bs[0], x = X[0];
-#pragma unroll
-        for (int k = 1; k < 8; k++)
-          if ((uint32_t)k < nr && fi >= X[k]) {
-            b = Bs[k];
-            x = X[k];
-          }
-        f(b + (fi - x), valid);
-      }
-      continue;
-    }
-    rowB[lane] = pB;
-    rowX[lane] = excl;
-    wave_lds_sync();
-    for (uint32_t f0 = 0; f0 < total; f0 += 64) {
-      const bool valid = f0 + lane < total;
-      const uint32_t fi = valid ? f0 + lane : total - 1;
-      // last row whose exclusive offset is <= fi (rows of length 0 share the next offset)
-      uint32_t l = 0, h = 64;
-      while (h - l > 1) {
-        const uint32_t mid = (l + h) >> 1;
-        if (rowX[mid] <= fi) l = mid; else h = mid;
-      }
-      f(rowB[l] + (fi - rowX[l]), valid);
-    }
-    wave_lds_sync();
-  }
-}
-
 }  // namespace
 }  // namespace knn

@@ -15,10 +15,10 @@ namespace knn {
 namespace {
 
 // ====================================================================== collect kernel
-// Streams the points of the block [lo, hi] 64 per round like for_each_point_flat, but every
-// lane calls f(c, p, valid) in every round (c: the coordinates of sorted position p, valid or
-// not), so f may use wave-wide operations; the coordinates of PR rounds are loaded before the
-// first of them is processed.
+// Streams the points of the block [lo, hi] 64 per round, its rows (lane = row, 64 per batch)
+// concatenated through a wave prefix sum; every lane calls f(c, p, valid) in every round (c:
+// the coordinates of sorted position p, valid or not), so f may use wave-wide operations; the
+// coordinates of PR rounds are loaded before the first of them is processed.
 template <typename T, int D, typename G, typename F>
 __device__ __forceinline__ void stream_block(const JoinArgs &a, const G &g, const uint32_t *lo,
                                              const uint32_t *hi, bool chord, const double *qd,

@@ -453,6 +453,9 @@ __global__ __launch_bounds__(64) void k_select(JoinArgs a, G g0) {
 #define KNN_SELECT_PR 3
 #endif
     constexpr int PR = D <= 4 ? KNN_SELECT_PR : (D <= 8 ? 2 : 1);
+    // rows averaging this many points or more are streamed one at a time (the partly empty last
+    // round of each then costs less than looking up every candidate's row)
+    constexpr uint32_t kLongRow = 128;
     // a fast-path query's sub-block rows: candidate fi of the concatenated rows sits at
     // fi + dlt0 + the steps stp[k] of the rows k it has passed (X[k] <= fi; rows past the
     // query's count never start)
@@ -476,11 +479,10 @@ __global__ __launch_bounds__(64) void k_select(JoinArgs a, G g0) {
       }
       return w;
     };
-    // positions and loads of rounds f0 .. f0 + 64 PR - 1 (lanes past the end load the last
-    // candidate again; rounds past the end are loaded too: guarding them costs more than the
-    // redundant loads)
-    auto load_group = [&](const Rows &w, uint32_t f0, T (&cv)[PR][D], uint32_t (&cid)[PR], bool (&val)[PR]) {
-      uint32_t pv[PR];
+    // a fast-path query's positions of rounds f0 .. f0 + 64 PR - 1 (lanes past the end take the
+    // last candidate again; rounds past the end are loaded too: guarding them costs more than
+    // the redundant loads)
+    auto fast_pos = [&](const Rows &w, uint32_t f0, uint32_t (&pv)[PR], bool (&val)[PR]) {
 #pragma unroll
       for (int u = 0; u < PR; u++) {
         uint32_t fi = f0 + 64u * u + lane;
@@ -491,6 +493,8 @@ __global__ __launch_bounds__(64) void k_select(JoinArgs a, G g0) {
         for (uint32_t k = 1; k < SEL_RMAX; k++) p += fi >= w.X[k] ? w.stp[k] : 0u;
         pv[u] = p;
       }
+    };
+    auto load_group = [&](const uint32_t (&pv)[PR], T (&cv)[PR][D], uint32_t (&cid)[PR]) {
       if constexpr (ND > 0) {
 #pragma unroll
         for (int u = 0; u < PR; u++) {
@@ -542,62 +546,98 @@ __global__ __launch_bounds__(64) void k_select(JoinArgs a, G g0) {
       }
       bool ovf = false;
       uint32_t nStaged = 0;  // wave-uniform: staging runs on every lane, valid or not
-      if ((fastMask >> i) & 1ull) {
-        const Rows w = rows_of(i);
-        for (uint32_t f0 = 0; f0 < w.tot; f0 += 64u * PR) {
-          T cv[PR][D];
-          uint32_t cidv[PR];
-          bool val[PR];
-          load_group(w, f0, cv, cidv, val);
-          consume(cv, cidv);
-          // the rounds' keys and buckets, then every slot claim, then the staging writes: the
-          // PR rounds' LDS atomics are in flight together
-          double keyv[PR];
-          uint32_t bv[PR], slot[PR];
-          bool take[PR];
-          T sv[PR];
+      // stages the candidates at positions pv of PR rounds (round u counts when
+      // f0 + 64 u < tot): loads, then the rounds' keys and buckets, then every slot claim, then
+      // the staging writes, so the PR rounds' loads and LDS atomics are in flight together
+      auto stage_group = [&](const uint32_t (&pv)[PR], const bool (&val)[PR], uint32_t f0, uint32_t tot) {
+        T cv[PR][D];
+        uint32_t cidv[PR];
+        load_group(pv, cv, cidv);
+        consume(cv, cidv);
+        double keyv[PR];
+        uint32_t bv[PR], slot[PR];
+        bool take[PR];
+        T sv[PR];
 #pragma unroll
-          for (int u = 0; u < PR; u++) {
-            take[u] = false;
-            if (f0 + 64u * u < w.tot) {  // wave-uniform
-              if constexpr (kPrefilter) {
-                sv[u] = Num<T>::template sq<D>(cv[u], qi);
-                keyv[u] = 0.0;
-                bv[u] = (uint32_t)SEL_NB;
-                if (val[u] && sv[u] <= Fe) {
-                  keyv[u] = fp64_key<D, T>(cv[u], qdi);
-                  bv[u] = kbucket(keyv[u], sci);
-                }
-              } else {
+        for (int u = 0; u < PR; u++) {
+          take[u] = false;
+          if (f0 + 64u * u < tot) {  // wave-uniform
+            if constexpr (kPrefilter) {
+              sv[u] = Num<T>::template sq<D>(cv[u], qi);
+              keyv[u] = 0.0;
+              bv[u] = (uint32_t)SEL_NB;
+              if (val[u] && sv[u] <= Fe) {
                 keyv[u] = fp64_key<D, T>(cv[u], qdi);
                 bv[u] = kbucket(keyv[u], sci);
               }
-              take[u] = val[u] && bv[u] <= bsi;
-              nStaged += (uint32_t)__popcll(__ballot(take[u]));
+            } else {
+              keyv[u] = fp64_key<D, T>(cv[u], qdi);
+              bv[u] = kbucket(keyv[u], sci);
             }
+            take[u] = val[u] && bv[u] <= bsi;
+            nStaged += (uint32_t)__popcll(__ballot(take[u]));
           }
+        }
 #pragma unroll
-          for (int u = 0; u < PR; u++) {
-            slot[u] = 0;
-            if (take[u]) slot[u] = atomicAdd(&qc[bv[u]], 1u);
-          }
+        for (int u = 0; u < PR; u++) {
+          slot[u] = 0;
+          if (take[u]) slot[u] = atomicAdd(&qc[bv[u]], 1u);
+        }
 #pragma unroll
-          for (int u = 0; u < PR; u++) {
-            if (take[u]) {
-              if (slot[u] < SM) {
-                skey[slot[u]] = keyv[u];
-                sid[slot[u]] = cidv[u];
-                if constexpr (kPrefilter)
-                  ss[slot[u]] = sv[u];
-                else
-                  ss[slot[u]] = Num<T>::template sq<D>(cv[u], qi);
-              } else {
-                ovf = true;
-              }
+        for (int u = 0; u < PR; u++) {
+          if (take[u]) {
+            if (slot[u] < SM) {
+              skey[slot[u]] = keyv[u];
+              sid[slot[u]] = cidv[u];
+              if constexpr (kPrefilter)
+                ss[slot[u]] = sv[u];
+              else
+                ss[slot[u]] = Num<T>::template sq<D>(cv[u], qi);
+            } else {
+              ovf = true;
             }
           }
         }
+      };
+      // one row of len points from sorted position pB, PR rounds at a time: no position lookup
+      // per candidate, but the row's last round is partly empty
+      auto stage_row = [&](uint32_t pB, uint32_t len) {
+        for (uint32_t f0 = 0; f0 < len; f0 += 64u * PR) {
+          uint32_t pv[PR];
+          bool val[PR];
+#pragma unroll
+          for (int u = 0; u < PR; u++) {
+            const uint32_t fi = f0 + 64u * u + lane;
+            val[u] = fi < len;
+            pv[u] = pB + (val[u] ? fi : len - 1u);
+          }
+          stage_group(pv, val, f0, len);
+        }
+      };
+      if ((fastMask >> i) & 1ull) {
+        const Rows w = rows_of(i);
+        const uint32_t Rq = prR[i];
+        if (w.tot >= Rq * kLongRow) {
+          // long rows (projected grids): row by row
+          for (uint32_t k = 0; k < Rq; k++) {
+            const uint32_t xk = prX[k * 64 + i];
+            const uint32_t xe = k + 1 < Rq ? prX[(k + 1) * 64 + i] : w.tot;
+            if (xe > xk) stage_row(prB[k * 64 + i], xe - xk);
+          }
+        } else {
+          for (uint32_t f0 = 0; f0 < w.tot; f0 += 64u * PR) {
+            uint32_t pv[PR];
+            bool val[PR];
+            fast_pos(w, f0, pv, val);
+            stage_group(pv, val, f0, w.tot);
+          }
+        }
       } else {
+        // the sub-block's rows, 64 per batch (lane = row), concatenated.  A candidate's row is
+        // found from a bitmap of the row starts (rowX: 2048 positions per chunk, one bit each):
+        // the rows begun before its window of 64 positions, plus the starts in the window at or
+        // below its lane (mbcnt), index the rows' position offsets (rowB, in the order of the
+        // rows with points).
         const T ri = readlane<T>(riL, (int)i);
         uint32_t slo[nd_cap<ND>()], shi[nd_cap<ND>()];
 #pragma unroll
@@ -608,31 +648,66 @@ __global__ __launch_bounds__(64) void k_select(JoinArgs a, G g0) {
           shi[j] = cell_coord<T>(next_up(xj + ri), (T)g.minArr[j], eps, g.nCells[j]);
         }
         const double ri2 = (double)ri * (double)ri * (1.0 + 0x1p-15);
-        for_each_point_flat<T, D, ND, G>(a, g, slo, shi, nd >= 2, qdi, ri2, lane, rowB, rowX,
-                                         [&](uint32_t p, bool valid) {
-          T c[D];
-#pragma unroll
-          for (int j = 0; j < D; j++) c[j] = xs[(size_t)j * n + p];
-          const uint32_t cid = a.sortedIds[p];  // with the coordinates: one memory round trip
-          double key = 0.0;
-          uint32_t b = (uint32_t)SEL_NB;
-          if (!kPrefilter || Num<T>::template sq<D>(c, qi) <= Fe) {
-            key = fp64_key<D, T>(c, qdi);
-            b = kbucket(key, sci);
+        constexpr uint32_t CAP = 64u * 32u;
+        const uint64_t R = block_rows<ND>(g, slo, shi);
+        for (uint64_t rb = 0; rb < R; rb += 64) {
+          uint32_t pB, len;
+          row_batch<T, D, ND, G>(a, g, slo, shi, nd >= 2, qdi, ri2, R, rb + lane, pB, len);
+          const uint32_t inc = wave_incl_sum(len, lane);
+          const uint32_t total = (uint32_t)__builtin_amdgcn_readlane((int)inc, 63);
+          if (total == 0) continue;
+          const uint32_t excl = inc - len;
+          uint64_t nz = __ballot(len > 0);
+          if (total >= (uint32_t)__popcll(nz) * kLongRow) {
+            // long rows: row by row, in lane order
+            while (nz) {
+              const int k = __builtin_ctzll(nz);
+              nz &= nz - 1;
+              stage_row((uint32_t)__builtin_amdgcn_readlane((int)pB, k),
+                        (uint32_t)__builtin_amdgcn_readlane((int)len, k));
+            }
+            continue;
           }
-          const bool take = valid && b <= bsi;
-          nStaged += (uint32_t)__popcll(__ballot(take));
-          if (take) {
-            const uint32_t slot = atomicAdd(&qc[b], 1u);
-            if (slot < SM) {
-              skey[slot] = key;
-              sid[slot] = cid;
-              ss[slot] = Num<T>::template sq<D>(c, qi);
-            } else {
-              ovf = true;
+          const uint32_t rk = __builtin_amdgcn_mbcnt_hi((uint32_t)(nz >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)nz, 0u));
+          if (len > 0) rowB[rk] = pB - excl;
+          for (uint32_t c0 = 0; c0 < total; c0 += CAP) {
+            rowX[lane] = 0u;
+            wave_lds_sync();
+            if (len > 0 && excl >= c0 && excl - c0 < CAP)
+              atomicOr(&rowX[(excl - c0) >> 5], 1u << ((excl - c0) & 31u));
+            // rows with points begun before the chunk (>= 1 past the first chunk)
+            uint32_t run = (uint32_t)__popcll(__ballot(len > 0 && excl < c0));
+            wave_lds_sync();
+            const uint32_t ctot = total - c0 < CAP ? total - c0 : CAP;
+            const uint32_t nwin = (ctot + 63u) >> 6;
+            for (uint32_t f0 = 0; f0 < ctot; f0 += 64u * PR) {
+              uint32_t pv[PR];
+              bool val[PR];
+#pragma unroll
+              for (int u = 0; u < PR; u++) {
+                const uint32_t wdx = (f0 >> 6) + (uint32_t)u;
+                // windows past the chunk's end have no starts: their lanes take the last row
+                uint64_t msk = 0;
+                if (wdx < nwin) {
+                  const uint32_t lo32 = (uint32_t)__builtin_amdgcn_readfirstlane((int)rowX[2u * wdx]);
+                  const uint32_t hi32 = (uint32_t)__builtin_amdgcn_readfirstlane((int)rowX[2u * wdx + 1u]);
+                  msk = ((uint64_t)hi32 << 32) | lo32;
+                }
+                const uint32_t fi = f0 + 64u * u + lane;
+                val[u] = fi < ctot;
+                // a lane past the end takes the chunk's last position, whose row is the last one
+                // begun at or below it: no start lies past ctot - 1 in the chunk
+                const uint32_t fc = val[u] ? fi : ctot - 1u;
+                const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(msk >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)msk, 0u));
+                const uint32_t row = run + below + (uint32_t)((msk >> lane) & 1ull) - 1u;
+                run += (uint32_t)__popcll(msk);
+                pv[u] = c0 + fc + rowB[row];
+              }
+              stage_group(pv, val, f0, ctot);
             }
           }
-        });
+          wave_lds_sync();
+        }
       }
       wave_lds_sync();
       // the staged count equals pass 1's cum(b*) (same keys, same buckets); anything else
