@@ -157,6 +157,9 @@ static void grid_dims(const T *mn, const T *mx, uint32_t nd, T eps, typename Gri
     g.mult[j] = tot;
   }
   g.keyBase = 0;
+  g.smin = 0;
+  g.sscale = 0;
+  g.sb = 0;
   totalCells = tot;
 }
 
@@ -325,6 +328,13 @@ struct Pipeline {
   void build_index(const T *mn, const T *mx, T eps) {
     grid_dims<T>(mn, mx, nd, eps, g, totalCells);
     const int bits = bits_for(totalCells);
+    // slab order within cells (u32 keys only: the slab code takes the key bits the cell id
+    // leaves, up to 16, and at least 8)
+    if (slab && bits <= 24 && slabMax > slabMin) {
+      g.sb = (uint32_t)std::min(16, 32 - bits);
+      g.smin = slabMin;
+      g.sscale = (T)(std::ldexp(1.0, (int)g.sb) / ((double)slabMax - (double)slabMin));
+    }
     const bool fromPrev = xs != nullptr && sortedIds != nullptr && !g_arena;
     const T *keySrc = fromPrev ? xs : x;
     // the sort's payload: original ids (first build) or previous positions (later builds)
@@ -334,14 +344,14 @@ struct Pipeline {
       uint32_t *keys = buf<uint32_t>(S_KEYS, n);
       uint32_t *kt = buf<uint32_t>(S_KEYS_TMP, n);
       cell_keys_u32<T>(keySrc, n, g, keys, ids, s);
-      radix_sort_pairs_u32(keys, ids, kt, ids_tmp, n, bits, scratch, s);
+      radix_sort_pairs_u32(keys, ids, kt, ids_tmp, n, bits + (int)g.sb, scratch, s);
       uint32_t *flags = buf<uint32_t>(S_FLAGS, n + 1);
       uint32_t *pos = buf<uint32_t>(S_POS, n + 1);
-      boundary_flags_u32(keys, n, flags, s);
+      boundary_flags_u32(keys, n, g.sb, flags, s);
       scan_exclusive_u32(flags, pos, n, scratch, s);
       cellKeys = buf<uint64_t>(S_CELLKEYS, n);
       cellStart = buf<uint32_t>(S_CELLSTART, n + 1);
-      compact_cells_u32(keys, flags, pos, n, cellKeys, cellStart, s);
+      compact_cells_u32(keys, flags, pos, n, g.sb, cellKeys, cellStart, s);
       cellOf = buf<uint32_t>(S_CELLOF, n);
       cell_of(flags, pos, n, cellOf, s);
       ncellDev = pos + n;
@@ -412,6 +422,28 @@ struct Pipeline {
     if (tBuild) tBuild->stop_async();
     sync();
     ncell = pin[PW_NCELL];
+  }
+  // slab order (fewer indexed than data dims): on when index_minmax has found the range of
+  // the first non-indexed dim; the stage entry points leave it off
+  bool slab = false;
+  T slabMin = 0, slabMax = 0;
+  // min/max of the indexed dims into hmn/hmx (nd values each), and of the first non-indexed
+  // dim for the slab order (KNN_SLAB=0 turns the slab order off)
+  void index_minmax(T *hmn, T *hmx) {
+    const char *ev = std::getenv("KNN_SLAB");
+    const bool want = nd < d && nd < KNN_MAX_INDEXED && !(ev && ev[0] == '0');
+    const uint32_t nde = want ? nd + 1 : nd;
+    T *mm = buf<T>(S_PART, 2 * KNN_MAX_INDEXED + minmax_scratch_elems(n, nde));
+    minmax_soa<T>(x, n, nde, mm, mm + KNN_MAX_INDEXED, mm + 2 * KNN_MAX_INDEXED, s);
+    T lo[KNN_MAX_INDEXED], hi[KNN_MAX_INDEXED];
+    HIP_CHECK(hipMemcpyAsync(lo, mm, sizeof(T) * nde, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipMemcpyAsync(hi, mm + KNN_MAX_INDEXED, sizeof(T) * nde, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    std::memcpy(hmn, lo, sizeof(T) * nd);
+    std::memcpy(hmx, hi, sizeof(T) * nd);
+    slab = want;
+    slabMin = want ? lo[nd] : (T)0;
+    slabMax = want ? hi[nd] : (T)0;
   }
   const uint32_t *ncellDev = nullptr;  // device copy of ncell (the scan total of the build)
   Slot xsSlot = S_XS;                   // the slot holding the current cell-sorted copy
@@ -1195,12 +1227,8 @@ static void run_plain(Pipeline<T> &P, const knnjoin_params &prm, double beta, kn
   stats.ms_eps = tPh.stop();
 
   // ---- min/max of the indexed dims (main.cu:2439-2456)
-  T *mm = P.template buf<T>(S_PART, 2 * KNN_MAX_INDEXED + minmax_scratch_elems(n, nd));
-  minmax_soa<T>(P.x, n, nd, mm, mm + KNN_MAX_INDEXED, mm + 2 * KNN_MAX_INDEXED, s);
   T hmn[KNN_MAX_INDEXED], hmx[KNN_MAX_INDEXED];
-  HIP_CHECK(hipMemcpyAsync(hmn, mm, sizeof(T) * nd, hipMemcpyDeviceToHost, s));
-  HIP_CHECK(hipMemcpyAsync(hmx, mm + KNN_MAX_INDEXED, sizeof(T) * nd, hipMemcpyDeviceToHost, s));
-  HIP_CHECK(hipStreamSynchronize(s));
+  P.index_minmax(hmn, hmx);
   if (eps0 == (T)0) eps0 = fallback_eps<T>(hmn, hmx, nd, n, K1);
   stats.eps0 = (float)eps0;
 
@@ -1357,12 +1385,8 @@ static void run_learned(Pipeline<T> &P, const knnjoin_params &p, knnjoin_stats &
   const RadiusModel &M = radius_model(P.d, nd);
   if (!M.ok) throw Error(KNN_ERR_MODEL, "no learned-radius model registered for this number of indexed dims");
   // min/max of the indexed dims
-  T *mm = P.template buf<T>(S_PART, 2 * KNN_MAX_INDEXED + minmax_scratch_elems(n, nd));
-  minmax_soa<T>(P.x, n, nd, mm, mm + KNN_MAX_INDEXED, mm + 2 * KNN_MAX_INDEXED, s);
   T hmn[KNN_MAX_INDEXED], hmx[KNN_MAX_INDEXED];
-  HIP_CHECK(hipMemcpyAsync(hmn, mm, sizeof(T) * nd, hipMemcpyDeviceToHost, s));
-  HIP_CHECK(hipMemcpyAsync(hmx, mm + KNN_MAX_INDEXED, sizeof(T) * nd, hipMemcpyDeviceToHost, s));
-  HIP_CHECK(hipStreamSynchronize(s));
+  P.index_minmax(hmn, hmx);
 
   // ---- features + MLP -> per-query radius
   tPh.start();
@@ -2294,9 +2318,9 @@ static void batch_index(BatchGroup &G, hipStream_t s) {
   batch_grids(G.sets, G.nsets, G.mnmx, G.width, G.bound, G.status, s);
   batch_cell_keys(G.x, N, G.sets, G.setOf, G.keys, G.ids, s);
   radix_sort_pairs_u32(G.keys, G.ids, G.keysTmp, G.idsTmp, N, G.bits, G.scratch, s);
-  boundary_flags_u32(G.keys, N, G.cflags, s);
+  boundary_flags_u32(G.keys, N, 0u, G.cflags, s);
   scan_exclusive_u32(G.cflags, G.cpos, N, G.scratch, s);
-  compact_cells_u32(G.keys, G.cflags, G.cpos, N, G.cellKeys, G.cellStart, s);
+  compact_cells_u32(G.keys, G.cflags, G.cpos, N, 0u, G.cellKeys, G.cellStart, s);
   gather_soa<float>(G.x, G.ids, N, G.d, G.xs, s);
   fill_u32(G.dense, 0u, G.bound + 1, s);
   batch_dense_counts(G.cellKeys, G.cellStart, G.cpos + N, N, G.dense, s);

@@ -21,24 +21,28 @@ __global__ void k_cell_keys(const T *__restrict__ x, uint32_t n, G g, K *keys, u
     uint32_t c = cell_coord<T>(x[(size_t)j * n + i], (T)g.minArr[j], (T)g.eps, g.nCells[j]);
     id += (uint64_t)c * g.mult[j];
   }
+  // slab order: the cell id above the slab code of the first non-indexed dim (one sort gives
+  // cell order, and slab order within each cell)
+  if (g.sb) id = (id << g.sb) | slab_q<T>(x[(size_t)g.nd * n + i], g);
   keys[i] = (K)id;
   ids[i] = i;
 }
 
+// sb: the slab code's bits below the cell id in the keys (0 without slab order)
 template <typename K>
-__global__ void k_boundary_flags(const K *keys, uint32_t n, uint32_t *flags) {
+__global__ void k_boundary_flags(const K *keys, uint32_t n, uint32_t sb, uint32_t *flags) {
   const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
-  flags[p] = (p == 0 || keys[p] != keys[p - 1]) ? 1u : 0u;
+  flags[p] = (p == 0 || (keys[p] >> sb) != (keys[p - 1] >> sb)) ? 1u : 0u;
 }
 
 template <typename K>
 __global__ void k_compact_cells(const K *keys, const uint32_t *flags, const uint32_t *pos,
-                                uint32_t n, uint64_t *cellKeys, uint32_t *cellStart) {
+                                uint32_t n, uint32_t sb, uint64_t *cellKeys, uint32_t *cellStart) {
   const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
   if (flags[p]) {
-    cellKeys[pos[p]] = (uint64_t)keys[p];
+    cellKeys[pos[p]] = (uint64_t)(keys[p] >> sb);
     cellStart[pos[p]] = p;
   }
   if (p == n - 1) cellStart[pos[n]] = n;
@@ -407,19 +411,19 @@ void cell_keys_u64(const T *x, uint32_t n, const typename GridOf<T>::type &g, ui
                    uint32_t *ids, hipStream_t s) {
   if (n) k_cell_keys<T, uint64_t><<<(n + 255) / 256, 256, 0, s>>>(x, n, g, keys, ids);
 }
-void boundary_flags_u32(const uint32_t *keys, uint32_t n, uint32_t *flags, hipStream_t s) {
-  if (n) k_boundary_flags<uint32_t><<<(n + 255) / 256, 256, 0, s>>>(keys, n, flags);
+void boundary_flags_u32(const uint32_t *keys, uint32_t n, uint32_t sb, uint32_t *flags, hipStream_t s) {
+  if (n) k_boundary_flags<uint32_t><<<(n + 255) / 256, 256, 0, s>>>(keys, n, sb, flags);
 }
 void boundary_flags_u64(const uint64_t *keys, uint32_t n, uint32_t *flags, hipStream_t s) {
-  if (n) k_boundary_flags<uint64_t><<<(n + 255) / 256, 256, 0, s>>>(keys, n, flags);
+  if (n) k_boundary_flags<uint64_t><<<(n + 255) / 256, 256, 0, s>>>(keys, n, 0u, flags);
 }
 void compact_cells_u32(const uint32_t *keys, const uint32_t *flags, const uint32_t *pos,
-                       uint32_t n, uint64_t *cellKeys, uint32_t *cellStart, hipStream_t s) {
-  if (n) k_compact_cells<uint32_t><<<(n + 255) / 256, 256, 0, s>>>(keys, flags, pos, n, cellKeys, cellStart);
+                       uint32_t n, uint32_t sb, uint64_t *cellKeys, uint32_t *cellStart, hipStream_t s) {
+  if (n) k_compact_cells<uint32_t><<<(n + 255) / 256, 256, 0, s>>>(keys, flags, pos, n, sb, cellKeys, cellStart);
 }
 void compact_cells_u64(const uint64_t *keys, const uint32_t *flags, const uint32_t *pos,
                        uint32_t n, uint64_t *cellKeys, uint32_t *cellStart, hipStream_t s) {
-  if (n) k_compact_cells<uint64_t><<<(n + 255) / 256, 256, 0, s>>>(keys, flags, pos, n, cellKeys, cellStart);
+  if (n) k_compact_cells<uint64_t><<<(n + 255) / 256, 256, 0, s>>>(keys, flags, pos, n, 0u, cellKeys, cellStart);
 }
 template <typename T>
 void gather_soa(const T *x, const uint32_t *ids, uint32_t n, uint32_t d, T *xs, hipStream_t s) {

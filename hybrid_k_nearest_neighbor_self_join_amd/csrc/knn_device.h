@@ -28,6 +28,10 @@ struct GridParams {
   uint64_t keyBase;                 // added to every linear id (one key range per set)
   float eps;
   uint32_t nd;                      // indexed dims
+  // slab order (sb > 0, single-set grids with fewer indexed than data dims): within a cell the
+  // points are sorted by slab_q of dim nd, the first non-indexed dim, quantised to sb bits
+  float smin, sscale;
+  uint32_t sb;
 };
 
 struct GridParamsD {
@@ -37,6 +41,8 @@ struct GridParamsD {
   uint64_t keyBase;
   double eps;
   uint32_t nd;
+  double smin, sscale;
+  uint32_t sb;
 };
 
 // One point set of a batched join: its grid, its points [pStart, pStart + n) of the group
@@ -63,6 +69,18 @@ __device__ __forceinline__ uint32_t cell_coord(T x, T mn, T eps, uint32_t ncell)
   if (t >= (T)4294967295.0) return ncell - 1;
   uint32_t c = (uint32_t)t;
   return c >= ncell ? ncell - 1 : c;
+}
+
+// Slab code of a coordinate of the first non-indexed dim: floor((x - smin) * sscale) clamped to
+// [0, 2^sb - 1], in T arithmetic; non-decreasing in x, so the points of a cell whose codes lie
+// in [code(a), code(b)] include every point with a <= x <= b.
+template <typename T, typename G>
+__device__ __forceinline__ uint32_t slab_q(T x, const G &g) {
+  const T t = (x - (T)g.smin) * (T)g.sscale;
+  const uint32_t top = (1u << g.sb) - 1u;
+  if (!(t > (T)0)) return 0u;
+  if (t >= (T)top) return top;
+  return (uint32_t)t;
 }
 
 // IEEE successor / predecessor of a finite value (nextafter toward +-inf); used to round the

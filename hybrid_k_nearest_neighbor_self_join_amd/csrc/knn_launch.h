@@ -68,13 +68,13 @@ void cell_keys_u32(const T *x, uint32_t n, const typename GridOf<T>::type &g, ui
 template <typename T>
 void cell_keys_u64(const T *x, uint32_t n, const typename GridOf<T>::type &g, uint64_t *keys,
                    uint32_t *ids, hipStream_t s);
-// flag[p] = (p == 0 || key[p] != key[p-1])
-void boundary_flags_u32(const uint32_t *keys, uint32_t n, uint32_t *flags, hipStream_t s);
+// flag[p] = (p == 0 || cell[p] != cell[p-1]), cell = key >> sb (sb: slab-code bits, else 0)
+void boundary_flags_u32(const uint32_t *keys, uint32_t n, uint32_t sb, uint32_t *flags, hipStream_t s);
 void boundary_flags_u64(const uint64_t *keys, uint32_t n, uint32_t *flags, hipStream_t s);
-// cellKeys[pos[p]] = key[p], cellStart[pos[p]] = p for boundary p; cellStart[C] = n.
+// cellKeys[pos[p]] = key[p] >> sb, cellStart[pos[p]] = p for boundary p; cellStart[C] = n.
 // Keys are widened to u64 in cellKeys.
 void compact_cells_u32(const uint32_t *keys, const uint32_t *flags, const uint32_t *pos,
-                       uint32_t n, uint64_t *cellKeys, uint32_t *cellStart, hipStream_t s);
+                       uint32_t n, uint32_t sb, uint64_t *cellKeys, uint32_t *cellStart, hipStream_t s);
 void compact_cells_u64(const uint64_t *keys, const uint32_t *flags, const uint32_t *pos,
                        uint32_t n, uint64_t *cellKeys, uint32_t *cellStart, hipStream_t s);
 // xs[j*n + p] = x[j*n + ids[p]]

@@ -254,6 +254,21 @@ __device__ __forceinline__ void for_each_row(const JoinArgs &a, const G &g, cons
   } while (next_row<ND>(cur, lo, hi, nd));
 }
 
+// for_each_row without the point ranges: f(rowBase, l0, h0) for each row kept by narrow
+// (uniform across the wave when lo/hi are).
+template <int ND, typename G, typename R, typename F>
+__device__ __forceinline__ void for_each_row_span(const G &g, const uint32_t *lo, const uint32_t *hi,
+                                                  R &&narrow, F &&f) {
+  uint32_t cur[nd_cap<ND>()];
+  const uint32_t nd = nd_of<ND>(g);
+#pragma unroll
+  for (uint32_t j = 0; j < nd_cap<ND>(); j++) cur[j] = lo[j < nd ? j : 0];
+  do {
+    uint32_t l0 = lo[0], h0 = hi[0];
+    if (narrow(cur, l0, h0)) f(row_base<ND>(g, cur), l0, h0);
+  } while (next_row<ND>(cur, lo, hi, nd));
+}
+
 // Distance^2 from the query to the slab of a row's cells in dims 1..nd-1 (each slab widened
 // by 2^-18 of its coordinates for the rounding of cell_coord).
 template <typename T, int D, int ND, typename G>
@@ -421,6 +436,62 @@ __device__ __forceinline__ void row_batch(const JoinArgs &a, const G &g, const u
     row_range(a, row_base<ND>(g, cur), l0, h0, pB, pE);
     len = pE - pB;
   }
+}
+
+// Narrows the sorted positions [pB, pE) of one cell to its points whose slab code lies in
+// [qlo, qhi]: the cell's points are in slab order, so two binary searches on the coordinates
+// xsd of the first non-indexed dim (both probes of a step loaded together).
+template <typename T, typename G>
+__device__ __forceinline__ void slab_narrow(const T *__restrict__ xsd, const G &g, uint32_t qlo, uint32_t qhi,
+                                            uint32_t &pB, uint32_t &pE) {
+  uint32_t l1 = pB, h1 = pE, l2 = pB, h2 = pE;
+  while (l1 < h1 || l2 < h2) {
+    const uint32_t m1 = (l1 + h1) >> 1, m2 = (l2 + h2) >> 1;
+    // a finished search probes pB again (pB < pE here), so every probe stays in the cell
+    const uint32_t c1 = slab_q<T>(xsd[l1 < h1 ? m1 : pB], g);
+    const uint32_t c2 = slab_q<T>(xsd[l2 < h2 ? m2 : pB], g);
+    if (l1 < h1) {
+      if (c1 < qlo) l1 = m1 + 1; else h1 = m1;
+    }
+    if (l2 < h2) {
+      if (c2 <= qhi) l2 = m2 + 1; else h2 = m2;
+    }
+  }
+  pB = l1;
+  pE = l2 > l1 ? l2 : l1;
+}
+
+// Segments of the block [lo, hi] with slab order: segment rr (of R rows x S0 = hi[0]-lo[0]+1
+// cells, row-major) is the cell lo[0] + rr % S0 of row rr / S0, narrowed to the slab codes
+// [qlo, qhi]; pB / len as row_batch (len 0 for cells outside the row's chord or past RS).
+template <typename T, int D, int ND, typename G>
+__device__ __forceinline__ void seg_batch(const JoinArgs &a, const G &g, const uint32_t *lo, const uint32_t *hi,
+                                          bool chord, const double *qd, double r2, uint64_t RS, uint64_t rr,
+                                          uint32_t qlo, uint32_t qhi, const T *__restrict__ xsd,
+                                          uint32_t &pB, uint32_t &len) {
+  pB = 0;
+  len = 0;
+  if (rr >= RS) return;
+  const uint32_t S0 = hi[0] - lo[0] + 1;
+  const uint64_t row = rr / S0;
+  const uint32_t x0 = lo[0] + (uint32_t)(rr - row * S0);
+  uint32_t cur[nd_cap<ND>()];
+  const uint32_t nd = nd_of<ND>(g);
+  uint64_t t = row;
+#pragma unroll
+  for (uint32_t j = 1; j < nd_cap<ND>(); j++) {
+    if (j >= nd) break;
+    const uint32_t span = hi[j] - lo[j] + 1;
+    cur[j] = lo[j] + (uint32_t)(t % span);
+    t /= span;
+  }
+  uint32_t l0 = lo[0], h0 = hi[0];
+  const bool keep = chord ? row_chord_lane<T, D, ND, G>(g, cur, qd, r2, l0, h0) : true;
+  if (!keep || x0 < l0 || x0 > h0) return;
+  uint32_t pE;
+  row_range(a, row_base<ND>(g, cur), x0, x0, pB, pE);
+  if (pE > pB) slab_narrow<T>(xsd, g, qlo, qhi, pB, pE);
+  len = pE - pB;
 }
 
 template <int ND, typename G>

@@ -147,6 +147,18 @@ __global__ __launch_bounds__(64) void k_select(JoinArgs a, G g0) {
     uint32_t lo[nd_cap<ND>()], hi[nd_cap<ND>()];
     tile_setup<T, D, ND, G>(a, g, tl, lane, L, lo, hi);
     const double scale = L.Tq > 0.0 ? (double)SEL_NBIN / L.Tq : __builtin_inf();
+    // slab order: the union of the lanes' slab-code ranges (radius r, as the cell ranges)
+    // (wave-uniform; the runtime-dims build streams whole cells: narrowing is optional, every
+    // cut point lies outside the radius)
+    const uint32_t sb = ND > 0 ? g.sb : 0u;
+    const T *__restrict__ xsd = xs + (size_t)nd * n;  // read only when sb > 0 (nd < D then)
+    uint32_t uql = 0, uqh = 0;
+    if (sb) {
+      const T r = a.radius ? reinterpret_cast<const T *>(a.radius)[L.q] : (T)a.radiusAll;
+      const T xq = pick_t<T, D>(L.qc, nd);
+      uql = wave_min_u32(L.active ? slab_q<T>(next_down(xq - r), g) : 0xffffffffu);
+      uqh = wave_max_u32(L.active ? slab_q<T>(next_up(xq + r), g) : 0u);
+    }
     wave_lds_sync();  // the previous tile's readers of H and the staging arrays are done
 #pragma unroll 8
     for (uint32_t w = 0; w < SEL_NB / 2; w++) H[w * 64 + lane] = 0u;
@@ -304,48 +316,57 @@ __global__ __launch_bounds__(64) void k_select(JoinArgs a, G g0) {
         for (uint32_t jj = 0; jj < m; jj++) inF += add1(cv, jj) < (uint32_t)SEL_NBIN ? 1u : 0u;
       }
     };
-    if constexpr (ND == 1 || ND == 2) {
-      // the union's rows (no chords below 3 indexed dims) concatenated: rounds mix rows
-      const uint64_t R = block_rows<ND>(g, lo, hi);
-      for (uint64_t rb = 0; rb < R; rb += 64) {
-        uint32_t pB, len;
-        row_batch<T, D, ND, G>(a, g, lo, hi, false, L.qd, 0.0, R, rb + lane, pB, len);
-        const uint32_t inc = wave_incl_sum(len, lane);
-        const uint32_t total = (uint32_t)__builtin_amdgcn_readlane((int)inc, 63);
-        const uint32_t excl = inc - len;
-        const uint64_t left = R - rb;
-        if (left <= 8) {
-          uint32_t X[8], Bs[8];
+    // 64 segments of sorted positions (lane = segment: start pB, length len; lanes from
+    // `left` on empty) concatenated and streamed
+    auto stream_batch = [&](uint32_t pB, uint32_t len, uint64_t left) {
+      const uint32_t inc = wave_incl_sum(len, lane);
+      const uint32_t total = (uint32_t)__builtin_amdgcn_readlane((int)inc, 63);
+      const uint32_t excl = inc - len;
+      if (left <= 8) {
+        uint32_t X[8], Bs[8];
 #pragma unroll
-          for (int k = 0; k < 8; k++) {
-            X[k] = (uint32_t)__builtin_amdgcn_readlane((int)excl, k);
-            Bs[k] = (uint32_t)__builtin_amdgcn_readlane((int)pB, k);
-          }
-          const uint32_t nr = (uint32_t)left;
-          stream_rounds<T, D>(xs, n, total, lane, [&](uint32_t fi) {
-            uint32_t b = Bs[0], x = X[0];
-#pragma unroll
-            for (int k = 1; k < 8; k++)
-              if ((uint32_t)k < nr && fi >= X[k]) {
-                b = Bs[k];
-                x = X[k];
-              }
-            return b + (fi - x);
-          }, hist_round);
-        } else {
-          rowB[lane] = pB;
-          rowX[lane] = excl;
-          wave_lds_sync();
-          stream_rounds<T, D>(xs, n, total, lane, [&](uint32_t fi) {
-            uint32_t l = 0, h = 64;
-            while (h - l > 1) {
-              const uint32_t mid = (l + h) >> 1;
-              if (rowX[mid] <= fi) l = mid; else h = mid;
-            }
-            return rowB[l] + (fi - rowX[l]);
-          }, hist_round);
-          wave_lds_sync();
+        for (int k = 0; k < 8; k++) {
+          X[k] = (uint32_t)__builtin_amdgcn_readlane((int)excl, k);
+          Bs[k] = (uint32_t)__builtin_amdgcn_readlane((int)pB, k);
         }
+        const uint32_t nr = (uint32_t)left;
+        stream_rounds<T, D>(xs, n, total, lane, [&](uint32_t fi) {
+          uint32_t b = Bs[0], x = X[0];
+#pragma unroll
+          for (int k = 1; k < 8; k++)
+            if ((uint32_t)k < nr && fi >= X[k]) {
+              b = Bs[k];
+              x = X[k];
+            }
+          return b + (fi - x);
+        }, hist_round);
+      } else {
+        rowB[lane] = pB;
+        rowX[lane] = excl;
+        wave_lds_sync();
+        stream_rounds<T, D>(xs, n, total, lane, [&](uint32_t fi) {
+          uint32_t l = 0, h = 64;
+          while (h - l > 1) {
+            const uint32_t mid = (l + h) >> 1;
+            if (rowX[mid] <= fi) l = mid; else h = mid;
+          }
+          return rowB[l] + (fi - rowX[l]);
+        }, hist_round);
+        wave_lds_sync();
+      }
+    };
+    if constexpr (ND == 1 || ND == 2) {
+      // the union's rows (no chords below 3 indexed dims) concatenated: rounds mix rows; with
+      // slab order, the rows' cells, each cut to the union's slab-code range
+      const uint64_t R = block_rows<ND>(g, lo, hi);
+      const uint64_t RS = sb ? R * (uint64_t)(hi[0] - lo[0] + 1) : R;
+      for (uint64_t rb = 0; rb < RS; rb += 64) {
+        uint32_t pB, len;
+        if (sb)
+          seg_batch<T, D, ND, G>(a, g, lo, hi, false, L.qd, 0.0, RS, rb + lane, uql, uqh, xsd, pB, len);
+        else
+          row_batch<T, D, ND, G>(a, g, lo, hi, false, L.qd, 0.0, R, rb + lane, pB, len);
+        stream_batch(pB, len, RS - rb);
       }
     } else {
       // rows one at a time, narrowed to the wave's union of chords of radius sqrt(Tq)
@@ -353,9 +374,27 @@ __global__ __launch_bounds__(64) void k_select(JoinArgs a, G g0) {
       auto narrow1 = [&](const uint32_t *cur, uint32_t &l0, uint32_t &h0) {
         return nd < 3 ? true : row_chord<T, D, ND, G>(g, cur, L.qd, rF2, L.active, l0, h0);
       };
-      for_each_row<ND>(a, g, lo, hi, narrow1, [&](uint32_t pBeg, uint32_t pEnd) {
-        stream_rounds<T, D>(xs, n, pEnd - pBeg, lane, [&](uint32_t fi) { return pBeg + fi; }, hist_round);
-      });
+      if (sb) {
+        // with slab order: the row's cells (lane = cell), each cut to the union's slab range
+        for_each_row_span<ND>(g, lo, hi, narrow1, [&](uint64_t rowBase, uint32_t l0, uint32_t h0) {
+          for (uint32_t c0 = l0;; c0 += 64) {
+            const uint32_t x0 = c0 + lane;
+            uint32_t pB = 0, len = 0;
+            if (x0 <= h0) {
+              uint32_t pE;
+              row_range(a, rowBase, x0, x0, pB, pE);
+              if (pE > pB) slab_narrow<T>(xsd, g, uql, uqh, pB, pE);
+              len = pE - pB;
+            }
+            stream_batch(pB, len, (uint64_t)(h0 - c0) + 1);
+            if (h0 - c0 < 64) break;
+          }
+        });
+      } else {
+        for_each_row<ND>(a, g, lo, hi, narrow1, [&](uint32_t pBeg, uint32_t pEnd) {
+          stream_rounds<T, D>(xs, n, pEnd - pBeg, lane, [&](uint32_t fi) { return pBeg + fi; }, hist_round);
+        });
+      }
     }
 
     // ---- scan: b* per query; each lane's counts become exclusive slot offsets (saturated past
@@ -393,11 +432,11 @@ __global__ __launch_bounds__(64) void k_select(JoinArgs a, G g0) {
 
     // ---- pass 2 set-up (lane = query): the rows of each query's sub-block and their point
     // ranges, for the whole tile at once (one memory round trip per tile instead of two per
-    // query).  Queries with more than RMAX rows, or grids without the dense start table, take
-    // the per-query path.
+    // query).  Queries with more than RMAX rows, grids without the dense start table and grids
+    // in slab order take the per-query path.
     const T riL = go ? (T)next_up((T)sqrt(edge)) : (T)0;
     bool fastL = false;
-    if (go && a.denseStart) {
+    if (go && a.denseStart && !sb) {
       uint32_t sl[nd_cap<ND>()], sh[nd_cap<ND>()];
       uint32_t R = 1;
 #pragma unroll
@@ -649,10 +688,21 @@ __global__ __launch_bounds__(64) void k_select(JoinArgs a, G g0) {
         }
         const double ri2 = (double)ri * (double)ri * (1.0 + 0x1p-15);
         constexpr uint32_t CAP = 64u * 32u;
+        // with slab order: the sub-block's cells (segments), each cut to the query's slab range
         const uint64_t R = block_rows<ND>(g, slo, shi);
-        for (uint64_t rb = 0; rb < R; rb += 64) {
+        const uint64_t RS = sb ? R * (uint64_t)(shi[0] - slo[0] + 1) : R;
+        uint32_t sql = 0, sqh = 0;
+        if (sb) {
+          const T xq = pick_t<T, D>(qi, nd);
+          sql = slab_q<T>(next_down(xq - ri), g);
+          sqh = slab_q<T>(next_up(xq + ri), g);
+        }
+        for (uint64_t rb = 0; rb < RS; rb += 64) {
           uint32_t pB, len;
-          row_batch<T, D, ND, G>(a, g, slo, shi, nd >= 2, qdi, ri2, R, rb + lane, pB, len);
+          if (sb)
+            seg_batch<T, D, ND, G>(a, g, slo, shi, nd >= 2, qdi, ri2, RS, rb + lane, sql, sqh, xsd, pB, len);
+          else
+            row_batch<T, D, ND, G>(a, g, slo, shi, nd >= 2, qdi, ri2, R, rb + lane, pB, len);
           const uint32_t inc = wave_incl_sum(len, lane);
           const uint32_t total = (uint32_t)__builtin_amdgcn_readlane((int)inc, 63);
           if (total == 0) continue;
