@@ -1345,15 +1345,18 @@ static void run_plain(Pipeline<T> &P, const knnjoin_params &prm, double beta, kn
 }
 
 // Cells per radius-class upper bound R in the learned schedule's grids: with <= 2 indexed
-// dims 3 up to K+1 = 80 and 4 above (with the wider classes below), 1 with more indexed dims;
+// dims 3 up to K+1 = 80 and 4 above (with the wider classes below), 1 with more indexed dims,
+// 4 on a projected grid (fewer indexed than data dims; see learned_class_bins);
 // KNN_LEARNED_GRID_DIV (1..8) overrides it for tuning.
-static double learned_grid_div(uint32_t nd, uint32_t k1) {
+static double learned_grid_div(uint32_t d, uint32_t nd, uint32_t k1) {
   static const double env = []() {
     const char *v = std::getenv("KNN_LEARNED_GRID_DIV");
     const double x = v ? std::atof(v) : 0.0;
     return (x >= 1.0 && x <= 8.0) ? x : 0.0;
   }();
-  return env > 0.0 ? env : (nd <= 2 ? (k1 <= 80 ? 3.0 : 4.0) : 1.0);
+  if (env > 0.0) return env;
+  if (nd < d) return 4.0;
+  return nd <= 2 ? (k1 <= 80 ? 3.0 : 4.0) : 1.0;
 }
 
 // Quarter-octave bins per radius class with <= 2 indexed dims: 8 (two octaves) up to
@@ -1362,14 +1365,21 @@ static double learned_grid_div(uint32_t nd, uint32_t k1) {
 // 28.9 / 28.2, K = 32 30.4 / 30.2, K = 64 39.4 / 40.4, K = 96 56.0 / 50.2, K = 128 66.5 /
 // 59.7, K = 256 128.1 / 117.6.  Two-octave classes on cells of R/3 beat one-octave classes
 // on cells of R/2 everywhere (K = 64: 41.2 against 47.3 ms), since each class costs an index
-// build over all N points.  KNN_LEARNED_CLASS_BINS (1..16) overrides it for tuning.
-static int learned_class_bins(uint32_t nd, uint32_t k1) {
+// build over all N points.  On a projected grid (fewer indexed than data dims) 6 bins on
+// cells of R/4; measured on 5M points, K = 32, learned ms (bins / cells: default of the
+// indexed dims, 4 / R/3, 6 / R/4): 4-D with 2 indexed, uniform 126 / 90 / 87, skewed
+// 202 / 213 / 191; 6-D with 4 indexed, uniform 270 / 204 / 202, skewed 1819 / 505 / 537 (one
+// bin on cells of R stays best with every dim indexed: 4-D uniform 28.8 against 84.5 and
+// 68.6 ms).  KNN_LEARNED_CLASS_BINS (1..16) overrides it for tuning.
+static int learned_class_bins(uint32_t d, uint32_t nd, uint32_t k1) {
   static const int env = []() {
     const char *v = std::getenv("KNN_LEARNED_CLASS_BINS");
     const int x = v ? std::atoi(v) : 0;
     return (x >= 1 && x <= 16) ? x : 0;
   }();
-  return env > 0 ? env : (nd <= 2 ? (k1 <= 80 ? 8 : 10) : 1);
+  if (env > 0) return env;
+  if (nd < d) return 6;
+  return nd <= 2 ? (k1 <= 80 ? 8 : 10) : 1;
 }
 
 // The learned schedule: per-query initial radius from the MLP, then passes over radius
@@ -1455,9 +1465,9 @@ static void run_learned(Pipeline<T> &P, const knnjoin_params &p, knnjoin_stats &
       passes++;
       break;
     }
-    // wide classes with <= 2 indexed dims; single bins above, where a cell up to 2x the radius
-    // would cost up to 2^nd in block volume
-    const int cHi = std::min(c + learned_class_bins(nd, P.K1) - 1, 63);
+    // wide classes with <= 2 indexed dims and on projected grids; single bins above, where a
+    // cell up to 2x the radius would cost up to 2^nd in block volume
+    const int cHi = std::min(c + learned_class_bins(P.d, nd, P.K1) - 1, 63);
     uint32_t nq = 0;
     for (int b = c; b <= cHi; b++) nq += h[b];
     double R;
@@ -1480,9 +1490,9 @@ static void run_learned(Pipeline<T> &P, const knnjoin_params &p, knnjoin_stats &
       }
       R = R * (1.0 + 1e-6);
     }
-    // cell width R/3 with <= 2 indexed dims; above, R (<= 3 cells per dim), since every extra
-    // cell of reach multiplies the rows of a block
-    T w = (T)(R / learned_grid_div(nd, P.K1));
+    // cell width R/3 with <= 2 indexed dims (R/4 on projected grids); above, R (<= 3 cells per
+    // dim), since every extra cell of reach multiplies the rows of a block
+    T w = (T)(R / learned_grid_div(P.d, nd, P.K1));
     if (!(w > (T)0)) w = F.wf;
     for (;;) {  // keep the grid within 2^30 cells
       typename GridOf<T>::type gt;
