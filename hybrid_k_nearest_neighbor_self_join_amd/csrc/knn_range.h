@@ -516,5 +516,92 @@ __device__ __forceinline__ uint32_t wave_incl_sum(uint32_t v, uint32_t lane) {
   return v;
 }
 
+// Segments whose lengths average this many positions or more are streamed one at a time (the
+// partly empty last round of each then costs less than looking up every candidate's segment).
+constexpr uint32_t SEG_LONG = 128;
+
+// One run of len sorted positions from pB, PR rounds of 64 at a time, lane = candidate:
+// group(pv, val, f0, len) gets the PR rounds' positions (lanes past the end repeat the last
+// position with val false; round u counts when f0 + 64 u < len).  Wave-uniform arguments.
+template <int PR, typename GF>
+__device__ __forceinline__ void stream_row(uint32_t pB, uint32_t len, uint32_t lane, GF &&group) {
+  for (uint32_t f0 = 0; f0 < len; f0 += 64u * PR) {
+    uint32_t pv[PR];
+    bool val[PR];
+#pragma unroll
+    for (int u = 0; u < PR; u++) {
+      const uint32_t fi = f0 + 64u * u + lane;
+      val[u] = fi < len;
+      pv[u] = pB + (val[u] ? fi : len - 1u);
+    }
+    group(pv, val, f0, len);
+  }
+}
+
+// One batch of 64 segments of sorted positions (lane = segment: first position pB, length len;
+// empty segments anywhere) streamed through group as stream_row does, the segments
+// concatenated.  Long segments go one at a time; otherwise a candidate finds its segment from
+// a bitmap of the segment starts (rowX: 2048 positions per chunk, one bit each): the segments
+// begun before its window of 64 positions, plus `mbcnt` of the window's starts at or below its
+// lane, index the segments' position offsets (rowB, in the order of the non-empty segments).
+// rowB / rowX: 64-word LDS arrays of the wave, free for group to ignore.
+template <int PR, typename GF>
+__device__ __forceinline__ void stream_segments(uint32_t pB, uint32_t len, uint32_t lane, uint32_t *rowB,
+                                                uint32_t *rowX, GF &&group) {
+  const uint32_t inc = wave_incl_sum(len, lane);
+  const uint32_t total = (uint32_t)__builtin_amdgcn_readlane((int)inc, 63);
+  if (total == 0) return;
+  const uint32_t excl = inc - len;
+  uint64_t nz = __ballot(len > 0);
+  if (total >= (uint32_t)__popcll(nz) * SEG_LONG) {
+    while (nz) {
+      const int k = __builtin_ctzll(nz);
+      nz &= nz - 1;
+      stream_row<PR>((uint32_t)__builtin_amdgcn_readlane((int)pB, k),
+                     (uint32_t)__builtin_amdgcn_readlane((int)len, k), lane, group);
+    }
+    return;
+  }
+  const uint32_t rk = __builtin_amdgcn_mbcnt_hi((uint32_t)(nz >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)nz, 0u));
+  if (len > 0) rowB[rk] = pB - excl;
+  constexpr uint32_t CAP = 64u * 32u;
+  for (uint32_t c0 = 0; c0 < total; c0 += CAP) {
+    rowX[lane] = 0u;
+    wave_lds_sync();
+    if (len > 0 && excl >= c0 && excl - c0 < CAP) atomicOr(&rowX[(excl - c0) >> 5], 1u << ((excl - c0) & 31u));
+    // segments with points begun before the chunk (>= 1 past the first chunk)
+    uint32_t run = (uint32_t)__popcll(__ballot(len > 0 && excl < c0));
+    wave_lds_sync();
+    const uint32_t ctot = total - c0 < CAP ? total - c0 : CAP;
+    const uint32_t nwin = (ctot + 63u) >> 6;
+    for (uint32_t f0 = 0; f0 < ctot; f0 += 64u * PR) {
+      uint32_t pv[PR];
+      bool val[PR];
+#pragma unroll
+      for (int u = 0; u < PR; u++) {
+        const uint32_t wdx = (f0 >> 6) + (uint32_t)u;
+        // windows past the chunk's end have no starts: their lanes take the last segment
+        uint64_t msk = 0;
+        if (wdx < nwin) {
+          const uint32_t lo32 = (uint32_t)__builtin_amdgcn_readfirstlane((int)rowX[2u * wdx]);
+          const uint32_t hi32 = (uint32_t)__builtin_amdgcn_readfirstlane((int)rowX[2u * wdx + 1u]);
+          msk = ((uint64_t)hi32 << 32) | lo32;
+        }
+        const uint32_t fi = f0 + 64u * u + lane;
+        val[u] = fi < ctot;
+        // a lane past the end takes the chunk's last position, whose segment is the last one
+        // begun at or below it: no start lies past ctot - 1 in the chunk
+        const uint32_t fc = val[u] ? fi : ctot - 1u;
+        const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(msk >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)msk, 0u));
+        const uint32_t row = run + below + (uint32_t)((msk >> lane) & 1ull) - 1u;
+        run += (uint32_t)__popcll(msk);
+        pv[u] = c0 + fc + rowB[row];
+      }
+      group(pv, val, f0, ctot);
+    }
+  }
+  wave_lds_sync();
+}
+
 }  // namespace
 }  // namespace knn

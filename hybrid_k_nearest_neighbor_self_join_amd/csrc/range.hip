@@ -15,96 +15,18 @@ namespace knn {
 namespace {
 
 // ====================================================================== collect kernel
-// Streams the points of the block [lo, hi] 64 per round, its rows (lane = row, 64 per batch)
-// concatenated through a wave prefix sum; every lane calls f(c, p, valid) in every round (c:
-// the coordinates of sorted position p, valid or not), so f may use wave-wide operations; the
-// coordinates of PR rounds are loaded before the first of them is processed.
-template <typename T, int D, typename G, typename F>
-__device__ __forceinline__ void stream_block(const JoinArgs &a, const G &g, const uint32_t *lo,
-                                             const uint32_t *hi, bool chord, const double *qd,
-                                             double r2, uint32_t lane, uint32_t *rowB, uint32_t *rowX,
-                                             F &&f) {
-  uint64_t R = 1;
-  for (uint32_t j = 1; j < g.nd; j++) R *= (uint64_t)(hi[j] - lo[j] + 1);
-  for (uint64_t rb = 0; rb < R; rb += 64) {
-    const uint64_t rr = rb + lane;
-    uint32_t pB = 0, len = 0;
-    if (rr < R) {
-      uint32_t cur[KNN_MAX_INDEXED];
-      uint64_t t = rr;
-      uint64_t rowBase = g.keyBase;
-      for (uint32_t j = 1; j < g.nd; j++) {
-        const uint32_t span = hi[j] - lo[j] + 1;
-        cur[j] = lo[j] + (uint32_t)(t % span);
-        t /= span;
-        rowBase += (uint64_t)cur[j] * g.mult[j];
-      }
-      uint32_t l0 = lo[0], h0 = hi[0];
-      const bool keep = chord ? row_chord_lane<T, D, 0, G>(g, cur, qd, r2, l0, h0) : true;
-      if (keep) {
-        uint32_t pE;
-        if (a.denseStart) {
-          pB = a.denseStart[rowBase + l0];
-          pE = a.denseStart[rowBase + h0 + 1];
-        } else {
-          const uint32_t cbeg = lower_bound_u64(a.cellKeys, a.ncell, rowBase + l0);
-          const uint32_t cend = upper_bound_u64(a.cellKeys, a.ncell, rowBase + h0);
-          pB = a.cellStart[cbeg];
-          pE = a.cellStart[cend];
-        }
-        len = pE - pB;
-      }
-    }
-    uint32_t inc = len;
-    for (int o = 1; o < 64; o <<= 1) {
-      const uint32_t t = (uint32_t)__shfl_up((int)inc, o, 64);
-      if (lane >= (uint32_t)o) inc += t;
-    }
-    const uint32_t total = (uint32_t)__builtin_amdgcn_readlane((int)inc, 63);
-    rowB[lane] = pB;
-    rowX[lane] = inc - len;
-    wave_lds_sync();
-    constexpr int PR = D <= 8 ? 4 : 2;
-    const T *__restrict__ xs = reinterpret_cast<const T *>(a.xs);
-    for (uint32_t f0 = 0; f0 < total; f0 += 64u * PR) {
-      T cv[PR][D];
-      uint32_t pv[PR];
-      bool val[PR];
-#pragma unroll
-      for (int u = 0; u < PR; u++) {
-        const uint32_t fi = f0 + 64u * u + lane;
-        val[u] = fi < total;
-        uint32_t p = 0;
-        if (val[u]) {
-          uint32_t l = 0, h = 64;
-          while (h - l > 1) {
-            const uint32_t mid = (l + h) >> 1;
-            if (rowX[mid] <= fi) l = mid; else h = mid;
-          }
-          p = rowB[l] + (fi - rowX[l]);
-        }
-        pv[u] = p;
-#pragma unroll
-        for (int j = 0; j < D; j++) cv[u][j] = val[u] ? xs[(size_t)j * a.n + p] : (T)0;
-      }
-#pragma unroll
-      for (int u = 0; u < PR; u++)
-        if (f0 + 64u * u < total) f(cv[u], pv[u], val[u]);
-    }
-    wave_lds_sync();
-  }
-}
-
 // Wave = query, for the queries of a tile in turn: the query's own block (radius r; rows
 // narrowed to the chords of its ball with 3 or more indexed dims) is streamed 64 candidates at
-// a time; the FP32/T sum is compared with the T bound of the collector's threshold (first
-// Tq = r^2 (1 - 2^-20), then the (K+1)-th key held), and only candidates that pass get their
-// FP64 key and id and enter the LDS collector.  The query succeeds when K+1 pairs with key
-// <= Tq were collected: they are then its exact top-(K+1), since every point with key <= Tq
-// lies in the block.  One pass over the candidates, against the select kernel's two (the
-// tile's union, then the query's sub-block): the better choice when a query has many
-// candidates per neighbour, as with 3 or more indexed dims.
-template <typename T, int D, typename G>
+// a time, PR rounds of coordinates and ids loaded together; the FP32/T sum is compared with the
+// T bound of the collector's threshold (first Tq = r^2 (1 - 2^-20), then the (K+1)-th key
+// held), and only candidates that pass get their FP64 key and enter the LDS collector.  The
+// query succeeds when K+1 pairs with key <= Tq were collected: they are then its exact
+// top-(K+1), since every point with key <= Tq lies in the block.  One pass over the
+// candidates, against the select kernel's two (the tile's union, then the query's sub-block):
+// the better choice when a query has many candidates per neighbour, as with 5 or more indexed
+// dims.  Built per number of indexed dims ND where that equals D (every per-dim array then
+// lives in registers), else for the runtime count (ND = 0).
+template <typename T, int D, int ND, typename G>
 __global__ __launch_bounds__(64) void k_range_collect(JoinArgs a, G g0) {
   extern __shared__ __align__(16) unsigned char smem[];
   uint32_t *rowB = reinterpret_cast<uint32_t *>(smem);
@@ -115,6 +37,7 @@ __global__ __launch_bounds__(64) void k_range_collect(JoinArgs a, G g0) {
   const T *__restrict__ x = reinterpret_cast<const T *>(a.x);
   const size_t n = a.n;
   const uint32_t NT = tile_count(a);
+  constexpr int PR = D <= 8 ? 4 : 2;
   WaveTally tally;
   for (;;) {
     const uint32_t t = next_tile(a.tileCounter, lane);
@@ -122,6 +45,7 @@ __global__ __launch_bounds__(64) void k_range_collect(JoinArgs a, G g0) {
     const uint4 tl = a.tiles[t];
     const TileCtx<G> C = tile_ctx<G>(a, g0, tl);
     const G &g = C.g;
+    const uint32_t nd = nd_of<ND>(g);
     const uint32_t K1 = C.k1;
     const T eps = (T)g.eps;
     uint32_t nok = 0;
@@ -138,12 +62,11 @@ __global__ __launch_bounds__(64) void k_range_collect(JoinArgs a, G g0) {
       const T r = a.radius ? reinterpret_cast<const T *>(a.radius)[q] : (T)a.radiusAll;
       const double Tq = (double)r * (double)r * (1.0 - 0x1p-20);
       const T F = Num<T>::bound(Tq);
-      uint32_t lo[KNN_MAX_INDEXED], hi[KNN_MAX_INDEXED];
-      for (uint32_t j = 0; j < g.nd; j++) {
-        T xj = qc[0];
+      uint32_t lo[nd_cap<ND>()], hi[nd_cap<ND>()];
 #pragma unroll
-        for (int jj = 1; jj < D; jj++)
-          if ((uint32_t)jj == j) xj = qc[jj];
+      for (uint32_t j = 0; j < nd_cap<ND>(); j++) {
+        if (j >= nd) break;
+        const T xj = pick_t<T, D>(qc, j);
         lo[j] = cell_coord<T>(next_down(xj - r), (T)g.minArr[j], eps, g.nCells[j]);
         hi[j] = cell_coord<T>(next_up(xj + r), (T)g.minArr[j], eps, g.nCells[j]);
       }
@@ -152,15 +75,40 @@ __global__ __launch_bounds__(64) void k_range_collect(JoinArgs a, G g0) {
       col.limit(Tq);
       T fb = F;
       const double rF2 = (double)F * (1.0 + 0x1p-15);
-      stream_block<T, D, G>(a, g, lo, hi, g.nd >= 3, qd, rF2, lane, rowB, rowX, [&](const T *c, uint32_t p, bool valid) {
-        cand += (unsigned long long)__popcll(__ballot(valid));
-        const bool near = valid && Num<T>::template sq<D>(c, qc) <= fb;
-        if (__ballot(near) == 0) return;
-        const uint32_t cid = near ? a.sortedIds[p] : 0u;
-        const double key = near ? fp64_key<D, T>(c, qd) : 0.0;
-        col.offer(near, key, cid, lane);
-        fb = col.nb == K1 ? Num<T>::bound(col.tk) : F;
-      });
+      // PR rounds: every load first (consumed at once, so none is deferred into the filter's
+      // branch), then the rounds in order through the filter and the collector
+      auto group = [&](const uint32_t (&pv)[PR], const bool (&val)[PR], uint32_t f0, uint32_t tot) {
+        T cv[PR][D];
+        uint32_t cid[PR];
+#pragma unroll
+        for (int u = 0; u < PR; u++) {
+#pragma unroll
+          for (int j = 0; j < D; j++) cv[u][j] = xs[(size_t)j * n + pv[u]];
+          cid[u] = a.sortedIds[pv[u]];
+        }
+#pragma unroll
+        for (int u = 0; u < PR; u++) {
+#pragma unroll
+          for (int j = 0; j < D; j++) asm volatile("" ::"v"(cv[u][j]));
+          asm volatile("" ::"v"(cid[u]));
+        }
+#pragma unroll
+        for (int u = 0; u < PR; u++) {
+          if (f0 + 64u * u >= tot) break;  // wave-uniform
+          cand += (unsigned long long)__popcll(__ballot(val[u]));
+          const bool near = val[u] && Num<T>::template sq<D>(cv[u], qc) <= fb;
+          if (__ballot(near) == 0) continue;
+          const double key = near ? fp64_key<D, T>(cv[u], qd) : 0.0;
+          col.offer(near, key, cid[u], lane);
+          fb = col.nb == K1 ? Num<T>::bound(col.tk) : F;
+        }
+      };
+      const uint64_t R = block_rows<ND>(g, lo, hi);
+      for (uint64_t rb = 0; rb < R; rb += 64) {
+        uint32_t pB, len;
+        row_batch<T, D, ND, G>(a, g, lo, hi, nd >= 3, qd, rF2, R, rb + lane, pB, len);
+        stream_segments<PR>(pB, len, lane, rowB, rowX, group);
+      }
       col.merge(lane);
       const bool ok = col.nb == K1;
       if (ok) {
@@ -367,7 +315,14 @@ template <typename T, int D, typename G>
 void launch_collect(const JoinArgs &a, const G &g, hipStream_t s) {
   if (a.ntiles == 0) return;
   const uint32_t grid = a.ntilesDev ? a.ntiles : range_select_grid(a.ntiles);
-  k_range_collect<T, D, G><<<grid, 64, range_collect_lds_bytes(a.k1), s>>>(a, g);
+  const size_t lds = range_collect_lds_bytes(a.k1);
+  if constexpr (D <= KNN_MAX_INDEXED) {
+    if (g.nd == (uint32_t)D) {
+      k_range_collect<T, D, D, G><<<grid, 64, lds, s>>>(a, g);
+      return;
+    }
+  }
+  k_range_collect<T, D, 0, G><<<grid, 64, lds, s>>>(a, g);
 }
 
 }  // namespace

@@ -492,9 +492,6 @@ __global__ __launch_bounds__(64) void k_select(JoinArgs a, G g0) {
 #define KNN_SELECT_PR 3
 #endif
     constexpr int PR = D <= 4 ? KNN_SELECT_PR : (D <= 8 ? 2 : 1);
-    // rows averaging this many points or more are streamed one at a time (the partly empty last
-    // round of each then costs less than looking up every candidate's row)
-    constexpr uint32_t kLongRow = 128;
     // a fast-path query's sub-block rows: candidate fi of the concatenated rows sits at
     // fi + dlt0 + the steps stp[k] of the rows k it has passed (X[k] <= fi; rows past the
     // query's count never start)
@@ -638,30 +635,15 @@ __global__ __launch_bounds__(64) void k_select(JoinArgs a, G g0) {
           }
         }
       };
-      // one row of len points from sorted position pB, PR rounds at a time: no position lookup
-      // per candidate, but the row's last round is partly empty
-      auto stage_row = [&](uint32_t pB, uint32_t len) {
-        for (uint32_t f0 = 0; f0 < len; f0 += 64u * PR) {
-          uint32_t pv[PR];
-          bool val[PR];
-#pragma unroll
-          for (int u = 0; u < PR; u++) {
-            const uint32_t fi = f0 + 64u * u + lane;
-            val[u] = fi < len;
-            pv[u] = pB + (val[u] ? fi : len - 1u);
-          }
-          stage_group(pv, val, f0, len);
-        }
-      };
       if ((fastMask >> i) & 1ull) {
         const Rows w = rows_of(i);
         const uint32_t Rq = prR[i];
-        if (w.tot >= Rq * kLongRow) {
-          // long rows (projected grids): row by row
+        if (w.tot >= Rq * SEG_LONG) {
+          // long rows: row by row
           for (uint32_t k = 0; k < Rq; k++) {
             const uint32_t xk = prX[k * 64 + i];
             const uint32_t xe = k + 1 < Rq ? prX[(k + 1) * 64 + i] : w.tot;
-            if (xe > xk) stage_row(prB[k * 64 + i], xe - xk);
+            if (xe > xk) stream_row<PR>(prB[k * 64 + i], xe - xk, lane, stage_group);
           }
         } else {
           for (uint32_t f0 = 0; f0 < w.tot; f0 += 64u * PR) {
@@ -672,11 +654,8 @@ __global__ __launch_bounds__(64) void k_select(JoinArgs a, G g0) {
           }
         }
       } else {
-        // the sub-block's rows, 64 per batch (lane = row), concatenated.  A candidate's row is
-        // found from a bitmap of the row starts (rowX: 2048 positions per chunk, one bit each):
-        // the rows begun before its window of 64 positions, plus the starts in the window at or
-        // below its lane (mbcnt), index the rows' position offsets (rowB, in the order of the
-        // rows with points).
+        // the sub-block's rows (with slab order: their cells, each cut to the query's slab
+        // range), 64 per batch (lane = row or cell), concatenated
         const T ri = readlane<T>(riL, (int)i);
         uint32_t slo[nd_cap<ND>()], shi[nd_cap<ND>()];
 #pragma unroll
@@ -687,8 +666,6 @@ __global__ __launch_bounds__(64) void k_select(JoinArgs a, G g0) {
           shi[j] = cell_coord<T>(next_up(xj + ri), (T)g.minArr[j], eps, g.nCells[j]);
         }
         const double ri2 = (double)ri * (double)ri * (1.0 + 0x1p-15);
-        constexpr uint32_t CAP = 64u * 32u;
-        // with slab order: the sub-block's cells (segments), each cut to the query's slab range
         const uint64_t R = block_rows<ND>(g, slo, shi);
         const uint64_t RS = sb ? R * (uint64_t)(shi[0] - slo[0] + 1) : R;
         uint32_t sql = 0, sqh = 0;
@@ -703,60 +680,7 @@ __global__ __launch_bounds__(64) void k_select(JoinArgs a, G g0) {
             seg_batch<T, D, ND, G>(a, g, slo, shi, nd >= 2, qdi, ri2, RS, rb + lane, sql, sqh, xsd, pB, len);
           else
             row_batch<T, D, ND, G>(a, g, slo, shi, nd >= 2, qdi, ri2, R, rb + lane, pB, len);
-          const uint32_t inc = wave_incl_sum(len, lane);
-          const uint32_t total = (uint32_t)__builtin_amdgcn_readlane((int)inc, 63);
-          if (total == 0) continue;
-          const uint32_t excl = inc - len;
-          uint64_t nz = __ballot(len > 0);
-          if (total >= (uint32_t)__popcll(nz) * kLongRow) {
-            // long rows: row by row, in lane order
-            while (nz) {
-              const int k = __builtin_ctzll(nz);
-              nz &= nz - 1;
-              stage_row((uint32_t)__builtin_amdgcn_readlane((int)pB, k),
-                        (uint32_t)__builtin_amdgcn_readlane((int)len, k));
-            }
-            continue;
-          }
-          const uint32_t rk = __builtin_amdgcn_mbcnt_hi((uint32_t)(nz >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)nz, 0u));
-          if (len > 0) rowB[rk] = pB - excl;
-          for (uint32_t c0 = 0; c0 < total; c0 += CAP) {
-            rowX[lane] = 0u;
-            wave_lds_sync();
-            if (len > 0 && excl >= c0 && excl - c0 < CAP)
-              atomicOr(&rowX[(excl - c0) >> 5], 1u << ((excl - c0) & 31u));
-            // rows with points begun before the chunk (>= 1 past the first chunk)
-            uint32_t run = (uint32_t)__popcll(__ballot(len > 0 && excl < c0));
-            wave_lds_sync();
-            const uint32_t ctot = total - c0 < CAP ? total - c0 : CAP;
-            const uint32_t nwin = (ctot + 63u) >> 6;
-            for (uint32_t f0 = 0; f0 < ctot; f0 += 64u * PR) {
-              uint32_t pv[PR];
-              bool val[PR];
-#pragma unroll
-              for (int u = 0; u < PR; u++) {
-                const uint32_t wdx = (f0 >> 6) + (uint32_t)u;
-                // windows past the chunk's end have no starts: their lanes take the last row
-                uint64_t msk = 0;
-                if (wdx < nwin) {
-                  const uint32_t lo32 = (uint32_t)__builtin_amdgcn_readfirstlane((int)rowX[2u * wdx]);
-                  const uint32_t hi32 = (uint32_t)__builtin_amdgcn_readfirstlane((int)rowX[2u * wdx + 1u]);
-                  msk = ((uint64_t)hi32 << 32) | lo32;
-                }
-                const uint32_t fi = f0 + 64u * u + lane;
-                val[u] = fi < ctot;
-                // a lane past the end takes the chunk's last position, whose row is the last one
-                // begun at or below it: no start lies past ctot - 1 in the chunk
-                const uint32_t fc = val[u] ? fi : ctot - 1u;
-                const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(msk >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)msk, 0u));
-                const uint32_t row = run + below + (uint32_t)((msk >> lane) & 1ull) - 1u;
-                run += (uint32_t)__popcll(msk);
-                pv[u] = c0 + fc + rowB[row];
-              }
-              stage_group(pv, val, f0, ctot);
-            }
-          }
-          wave_lds_sync();
+          stream_segments<PR>(pB, len, lane, rowB, rowX, stage_group);
         }
       }
       wave_lds_sync();
