@@ -124,6 +124,26 @@ __device__ __forceinline__ double readlane<double>(double v, int lane) {
   return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
 }
 
+// A wave-uniform value moved to scalar registers (the first active lane's copy), so that it
+// holds no vector register.
+template <typename T>
+__device__ __forceinline__ T readfirst(T v);
+template <>
+__device__ __forceinline__ float readfirst<float>(float v) {
+  return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+template <>
+__device__ __forceinline__ uint32_t readfirst<uint32_t>(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_readfirstlane((int)v);
+}
+template <>
+__device__ __forceinline__ double readfirst<double>(double v) {
+  uint64_t b = (uint64_t)__double_as_longlong(v);
+  uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)b);
+  uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(b >> 32));
+  return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+
 __device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) {
   for (int o = 32; o > 0; o >>= 1) {
     uint32_t w = __shfl_xor(v, o, 64);

@@ -55,11 +55,13 @@ __global__ __launch_bounds__(64) void k_range_collect(JoinArgs a, G g0) {
       T qc[D];
       double qd[D];
 #pragma unroll
+      // the query's values are wave-uniform: scalar registers (the vector registers are what
+      // bounds this kernel's residency)
       for (int j = 0; j < D; j++) {
-        qc[j] = x[(size_t)j * n + q];
-        qd[j] = (double)qc[j];
+        qc[j] = readfirst<T>(x[(size_t)j * n + q]);
+        qd[j] = readfirst<double>((double)qc[j]);
       }
-      const T r = a.radius ? reinterpret_cast<const T *>(a.radius)[q] : (T)a.radiusAll;
+      const T r = readfirst<T>(a.radius ? reinterpret_cast<const T *>(a.radius)[q] : (T)a.radiusAll);
       const double Tq = (double)r * (double)r * (1.0 - 0x1p-20);
       const T F = Num<T>::bound(Tq);
       uint32_t lo[nd_cap<ND>()], hi[nd_cap<ND>()];
@@ -67,8 +69,8 @@ __global__ __launch_bounds__(64) void k_range_collect(JoinArgs a, G g0) {
       for (uint32_t j = 0; j < nd_cap<ND>(); j++) {
         if (j >= nd) break;
         const T xj = pick_t<T, D>(qc, j);
-        lo[j] = cell_coord<T>(next_down(xj - r), (T)g.minArr[j], eps, g.nCells[j]);
-        hi[j] = cell_coord<T>(next_up(xj + r), (T)g.minArr[j], eps, g.nCells[j]);
+        lo[j] = readfirst<uint32_t>(cell_coord<T>(next_down(xj - r), (T)g.minArr[j], eps, g.nCells[j]));
+        hi[j] = readfirst<uint32_t>(cell_coord<T>(next_up(xj + r), (T)g.minArr[j], eps, g.nCells[j]));
       }
       Collector col;
       col.init(cbase, K1);
