@@ -24,8 +24,9 @@ namespace {
 // top-(K+1), since every point with key <= Tq lies in the block.  One pass over the
 // candidates, against the select kernel's two (the tile's union, then the query's sub-block):
 // the better choice when a query has many candidates per neighbour, as with 5 or more indexed
-// dims.  Built per number of indexed dims ND where that equals D (every per-dim array then
-// lives in registers), else for the runtime count (ND = 0).
+// dims.  Built per number of indexed dims ND where that equals D or is 6 (every per-dim array
+// then lives in registers), else for the runtime count (ND = 0).  On a grid in slab order the
+// fixed-dims builds cut each cell of the block to the query's slab, as the select kernel does.
 template <typename T, int D, int ND, typename G>
 __global__ __launch_bounds__(64) void k_range_collect(JoinArgs a, G g0) {
   extern __shared__ __align__(16) unsigned char smem[];
@@ -105,10 +106,23 @@ __global__ __launch_bounds__(64) void k_range_collect(JoinArgs a, G g0) {
           fb = col.nb == K1 ? Num<T>::bound(col.tk) : F;
         }
       };
+      // with slab order (projected grids, fixed-dims builds): the block's cells, each cut to
+      // the query's slab range in the first non-indexed dim
+      const uint32_t sb = (ND > 0 && ND < D) ? g.sb : 0u;
       const uint64_t R = block_rows<ND>(g, lo, hi);
-      for (uint64_t rb = 0; rb < R; rb += 64) {
+      const uint64_t RS = sb ? R * (uint64_t)(hi[0] - lo[0] + 1) : R;
+      uint32_t sql = 0, sqh = 0;
+      if (sb) {
+        const T xq = pick_t<T, D>(qc, nd);
+        sql = readfirst<uint32_t>(slab_q<T>(next_down(xq - r), g));
+        sqh = readfirst<uint32_t>(slab_q<T>(next_up(xq + r), g));
+      }
+      for (uint64_t rb = 0; rb < RS; rb += 64) {
         uint32_t pB, len;
-        row_batch<T, D, ND, G>(a, g, lo, hi, nd >= 3, qd, rF2, R, rb + lane, pB, len);
+        if (sb)
+          seg_batch<T, D, ND, G>(a, g, lo, hi, nd >= 3, qd, rF2, RS, rb + lane, sql, sqh, xs + (size_t)nd * n, pB, len);
+        else
+          row_batch<T, D, ND, G>(a, g, lo, hi, nd >= 3, qd, rF2, R, rb + lane, pB, len);
         stream_segments<PR>(pB, len, lane, rowB, rowX, group);
       }
       col.merge(lane);
@@ -321,6 +335,13 @@ void launch_collect(const JoinArgs &a, const G &g, hipStream_t s) {
   if constexpr (D <= KNN_MAX_INDEXED) {
     if (g.nd == (uint32_t)D) {
       k_range_collect<T, D, D, G><<<grid, 64, lds, s>>>(a, g);
+      return;
+    }
+  }
+  // 6 indexed dims of more data dims: the reference's default NUMINDEXEDDIM above 6 dims
+  if constexpr (D > 6) {
+    if (g.nd == 6u) {
+      k_range_collect<T, D, 6, G><<<grid, 64, lds, s>>>(a, g);
       return;
     }
   }

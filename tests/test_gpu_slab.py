@@ -3,7 +3,8 @@ sorted by the quantised first non-indexed coordinate and the select kernel cuts 
 block to the query's (or the tile's) slab range by binary searches.  Only the candidate streams
 change, never the rows: the join equals the oracle's, and equals the join with the slab order
 turned off (KNN_SLAB=0), bit for bit, for every number of indexed dims the select kernel builds
-(1 to 4), lattice data full of ties, and a non-indexed dim of zero range."""
+(1 to 4), 6 indexed dims on the collect kernel, lattice data full of ties, and a non-indexed
+dim of zero range."""
 import numpy as np
 import pytest
 
@@ -32,8 +33,10 @@ def _points(d, gen, n, seed):
     raise ValueError(gen)
 
 
+# (8, 6) and (7, 6) run on the collect kernel (6 indexed dims, the reference's default above
+# 6 data dims), the others on the select kernel
 ORACLE_CASES = [(3, 1, "uniform", 16), (4, 2, "skewed", 16), (5, 3, "uniform", 8), (6, 4, "uniform", 16),
-                (4, 2, "lattice", 16), (4, 3, "flat", 8)]
+                (4, 2, "lattice", 16), (4, 3, "flat", 8), (8, 6, "uniform", 16), (7, 6, "skewed", 8)]
 
 
 @pytest.mark.parametrize("d,nd,gen,k", ORACLE_CASES)
@@ -46,7 +49,8 @@ def test_slab_order_join_equals_oracle(oracle, d, nd, gen, k):
     assert np.array_equal(bits(dist), bits(d2))
 
 
-@pytest.mark.parametrize("d,nd,gen", [(4, 2, "uniform"), (4, 2, "skewed"), (6, 4, "uniform"), (5, 3, "lattice")])
+@pytest.mark.parametrize("d,nd,gen", [(4, 2, "uniform"), (4, 2, "skewed"), (6, 4, "uniform"), (5, 3, "lattice"),
+                                      (8, 6, "uniform")])
 def test_slab_order_off_gives_same_rows(monkeypatch, d, nd, gen):
     pts = _points(d, gen, 200_000, seed=d * 100 + nd + 7)
     loaded = learned_radius.register_default_models()
