@@ -65,3 +65,14 @@ def test_slab_order_off_gives_same_rows(monkeypatch, d, nd, gen):
         assert np.array_equal(bits(a[1]), bits(off_p[1]))
     # the cut streams fewer candidates than whole cells
     assert on_p[2].candidates < off_p[2].candidates
+
+
+@pytest.mark.parametrize("d,nd", [(4, 2), (6, 4), (8, 6)])
+def test_slab_order_double(oracle, d, nd):
+    # the slab codes in FP64 arithmetic (the DTYPE=double build of the grid)
+    pts = synthetic.skewed_cube(20_000, d, seed=d * 7 + nd).astype(np.float64)
+    ids, dist, st = _lib.knnjoin_ex(pts, 8, num_indexed_dims=nd, beta=0.025)
+    rc, i2, d2, s2 = oracle.knnjoin(pts, 8, nd=nd, beta=0.025)
+    assert rc == 0
+    assert np.array_equal(ids, i2)
+    assert np.array_equal(dist.view(np.uint64), d2.view(np.uint64))
