@@ -438,6 +438,11 @@ __device__ __forceinline__ void row_batch(const JoinArgs &a, const G &g, const u
   }
 }
 
+// Blocks at most this many cells wide in dim 0 take the slab cut (a cell lookup and two
+// binary searches per cell instead of one lookup per row: wide blocks of a large radius over
+// sparse cells would pay more for the lookups than the cut saves).
+constexpr uint32_t SLAB_MAX_SPAN = 64;
+
 // Narrows the sorted positions [pB, pE) of one cell to its points whose slab code lies in
 // [qlo, qhi]: the cell's points are in slab order, so two binary searches on the coordinates
 // xsd of the first non-indexed dim (both probes of a step loaded together).

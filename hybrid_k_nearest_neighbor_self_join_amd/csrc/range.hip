@@ -108,18 +108,18 @@ __global__ __launch_bounds__(64) void k_range_collect(JoinArgs a, G g0) {
       };
       // with slab order (projected grids, fixed-dims builds): the block's cells, each cut to
       // the query's slab range in the first non-indexed dim
-      const uint32_t sb = (ND > 0 && ND < D) ? g.sb : 0u;
+      const bool cut = (ND > 0 && ND < D) && g.sb && hi[0] - lo[0] < SLAB_MAX_SPAN;
       const uint64_t R = block_rows<ND>(g, lo, hi);
-      const uint64_t RS = sb ? R * (uint64_t)(hi[0] - lo[0] + 1) : R;
+      const uint64_t RS = cut ? R * (uint64_t)(hi[0] - lo[0] + 1) : R;
       uint32_t sql = 0, sqh = 0;
-      if (sb) {
+      if (cut) {
         const T xq = pick_t<T, D>(qc, nd);
         sql = readfirst<uint32_t>(slab_q<T>(next_down(xq - r), g));
         sqh = readfirst<uint32_t>(slab_q<T>(next_up(xq + r), g));
       }
       for (uint64_t rb = 0; rb < RS; rb += 64) {
         uint32_t pB, len;
-        if (sb)
+        if (cut)
           seg_batch<T, D, ND, G>(a, g, lo, hi, nd >= 3, qd, rF2, RS, rb + lane, sql, sqh, xs + (size_t)nd * n, pB, len);
         else
           row_batch<T, D, ND, G>(a, g, lo, hi, nd >= 3, qd, rF2, R, rb + lane, pB, len);

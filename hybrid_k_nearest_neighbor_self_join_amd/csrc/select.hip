@@ -358,11 +358,12 @@ __global__ __launch_bounds__(64) void k_select(JoinArgs a, G g0) {
     if constexpr (ND == 1 || ND == 2) {
       // the union's rows (no chords below 3 indexed dims) concatenated: rounds mix rows; with
       // slab order, the rows' cells, each cut to the union's slab-code range
+      const bool cut = sb && hi[0] - lo[0] < SLAB_MAX_SPAN;
       const uint64_t R = block_rows<ND>(g, lo, hi);
-      const uint64_t RS = sb ? R * (uint64_t)(hi[0] - lo[0] + 1) : R;
+      const uint64_t RS = cut ? R * (uint64_t)(hi[0] - lo[0] + 1) : R;
       for (uint64_t rb = 0; rb < RS; rb += 64) {
         uint32_t pB, len;
-        if (sb)
+        if (cut)
           seg_batch<T, D, ND, G>(a, g, lo, hi, false, L.qd, 0.0, RS, rb + lane, uql, uqh, xsd, pB, len);
         else
           row_batch<T, D, ND, G>(a, g, lo, hi, false, L.qd, 0.0, R, rb + lane, pB, len);
@@ -374,7 +375,7 @@ __global__ __launch_bounds__(64) void k_select(JoinArgs a, G g0) {
       auto narrow1 = [&](const uint32_t *cur, uint32_t &l0, uint32_t &h0) {
         return nd < 3 ? true : row_chord<T, D, ND, G>(g, cur, L.qd, rF2, L.active, l0, h0);
       };
-      if (sb) {
+      if (sb && hi[0] - lo[0] < SLAB_MAX_SPAN) {
         // with slab order: the row's cells (lane = cell), each cut to the union's slab range
         for_each_row_span<ND>(g, lo, hi, narrow1, [&](uint64_t rowBase, uint32_t l0, uint32_t h0) {
           for (uint32_t c0 = l0;; c0 += 64) {
@@ -667,16 +668,17 @@ __global__ __launch_bounds__(64) void k_select(JoinArgs a, G g0) {
         }
         const double ri2 = (double)ri * (double)ri * (1.0 + 0x1p-15);
         const uint64_t R = block_rows<ND>(g, slo, shi);
-        const uint64_t RS = sb ? R * (uint64_t)(shi[0] - slo[0] + 1) : R;
+        const bool cut = sb && shi[0] - slo[0] < SLAB_MAX_SPAN;
+        const uint64_t RS = cut ? R * (uint64_t)(shi[0] - slo[0] + 1) : R;
         uint32_t sql = 0, sqh = 0;
-        if (sb) {
+        if (cut) {
           const T xq = pick_t<T, D>(qi, nd);
           sql = slab_q<T>(next_down(xq - ri), g);
           sqh = slab_q<T>(next_up(xq + ri), g);
         }
         for (uint64_t rb = 0; rb < RS; rb += 64) {
           uint32_t pB, len;
-          if (sb)
+          if (cut)
             seg_batch<T, D, ND, G>(a, g, slo, shi, nd >= 2, qdi, ri2, RS, rb + lane, sql, sqh, xsd, pB, len);
           else
             row_batch<T, D, ND, G>(a, g, slo, shi, nd >= 2, qdi, ri2, R, rb + lane, pB, len);
