@@ -208,47 +208,49 @@ constexpr uint32_t NBUCKETS = 10000;
 // kernel.cu:450-523: sample s = blockIdx.x / KD_PARTS (point s * offset) against the first
 // 256 points of every 2048, +8 per hit in bucket floor(d / bw) for d < avg (self excluded).
 // The counts are integers, so the partition is free: each sample's 2048-point strides are
-// dealt to KD_PARTS blocks, and a thread loads KD_UNROLL strides before it bins any.
+// dealt to KD_PARTS blocks, and a thread loads KD_UNROLL strides before it bins any.  Built
+// per dimensionality D (the coordinates in registers, no per-dim predication); a pair whose
+// squared distance is at or above avg^2 (1 + 2^-40) has a rounded root >= avg and skips the
+// root and the division.
 constexpr uint32_t KD_PARTS = 2, KD_UNROLL = 4;
-template <typename T>
-__global__ __launch_bounds__(256) void k_kdist(const T *__restrict__ x, uint32_t n, uint32_t d,
-                                               uint32_t offset, double avg, double bw,
-                                               uint32_t *hist) {
+template <typename T, int D>
+__global__ __launch_bounds__(256) void k_kdist(const T *__restrict__ x, uint32_t n, uint32_t offset,
+                                               double avg, double bw, uint32_t *hist) {
   __shared__ uint32_t lh[NBUCKETS];
   for (uint32_t b = threadIdx.x; b < NBUCKETS; b += 256) lh[b] = 0;
   __syncthreads();
   const uint64_t tid = (uint64_t)(blockIdx.x / KD_PARTS) * offset;
   const uint32_t part = blockIdx.x % KD_PARTS;
+  const double cut2 = avg * avg * (1.0 + 0x1p-40);
   if (tid < n) {
-    T q[KNN_MAXDIM];
+    T q[D];
 #pragma unroll
-    for (uint32_t j = 0; j < KNN_MAXDIM; j++) q[j] = j < d ? x[(size_t)j * n + tid] : (T)0;
+    for (int j = 0; j < D; j++) q[j] = x[(size_t)j * n + tid];
     const uint64_t step = 2048ull * KD_PARTS;
     for (uint64_t i0 = 2048ull * part; i0 < n; i0 += step * KD_UNROLL) {
-      T c[KD_UNROLL][KNN_MAXDIM];
+      T c[KD_UNROLL][D];
       bool ok[KD_UNROLL];
 #pragma unroll
       for (uint32_t u = 0; u < KD_UNROLL; u++) {
         const uint64_t p = i0 + step * u + threadIdx.x;
         ok[u] = p < n;
+        const uint64_t pc = ok[u] ? p : (uint64_t)n - 1;
 #pragma unroll
-        for (uint32_t j = 0; j < KNN_MAXDIM; j++) c[u][j] = (ok[u] && j < d) ? x[(size_t)j * n + p] : (T)0;
+        for (int j = 0; j < D; j++) c[u][j] = x[(size_t)j * n + pc];
       }
 #pragma unroll
       for (uint32_t u = 0; u < KD_UNROLL; u++) {
         const uint64_t p = i0 + step * u + threadIdx.x;
-        if (!ok[u]) continue;
         double run = 0;
 #pragma unroll
-        for (uint32_t j = 0; j < KNN_MAXDIM; j++) {
-          if (j < d) {
-            T df = c[u][j] - q[j];
-            T sq = df * df;
-            run += (double)sq;
-          }
+        for (int j = 0; j < D; j++) {
+          T df = c[u][j] - q[j];
+          T sq = df * df;
+          run += (double)sq;
         }
+        if (!ok[u] || !(run < cut2) || p == tid) continue;
         run = sqrt(run);
-        if (run < avg && p != tid) {
+        if (run < avg) {
           uint32_t bucket = (uint32_t)(run / bw);
           if (bucket < NBUCKETS) atomicAdd(&lh[bucket], 8u);
         }
@@ -333,7 +335,16 @@ template <typename T>
 void kdist_hist_soa(const T *x, uint32_t n, uint32_t d, uint32_t offset, uint32_t nsample,
                     double avg, double bw, uint32_t *hist, hipStream_t s) {
   fill_u32(hist, 0u, NBUCKETS, s);
-  if (nsample) k_kdist<T><<<nsample * KD_PARTS, 256, 0, s>>>(x, n, d, offset, avg, bw, hist);
+  if (!nsample) return;
+  const uint32_t grid = nsample * KD_PARTS;
+  switch (d) {
+#define KD_CASE(DD) \
+    case DD: k_kdist<T, DD><<<grid, 256, 0, s>>>(x, n, offset, avg, bw, hist); break;
+    KD_CASE(1) KD_CASE(2) KD_CASE(3) KD_CASE(4) KD_CASE(5) KD_CASE(6) KD_CASE(7) KD_CASE(8)
+    KD_CASE(9) KD_CASE(10) KD_CASE(11) KD_CASE(12) KD_CASE(13) KD_CASE(14) KD_CASE(15) KD_CASE(16)
+#undef KD_CASE
+    default: break;
+  }
 }
 
 #define INST(T)                                                                            \
