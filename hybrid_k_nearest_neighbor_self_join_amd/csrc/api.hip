@@ -621,6 +621,21 @@ static bool use_collect(uint32_t nd, uint32_t k1) {
   return env >= 0 ? env == 1 : nd >= 5;
 }
 
+// Tile ranges of a range-kernel launch: one per XCD for single sets of 3 or more data dims,
+// one queue for 2-D sets and batched groups.  Measured on one MI355X, each pair on one box
+// (ms per join, one queue against one range per XCD): 10M mixed 2-D points K=64, plain 35.0 /
+// 35.9, learned 21.3 / 21.5; 5M points, K=32, plain (learned): 4-D uniform 30.8 / 29.8 (32.7
+// / 32.0), skewed 54.6 / 57.6 (52.3 / 53.1); 4-D on 2 indexed dims uniform 92.3 / 79.5, skewed
+// 206 / 216; 6-D uniform (collect kernel) 132 / 106 (120 / 98), skewed 188 / 178; 6-D on 4
+// indexed dims uniform 286 / 272, skewed 1382 / 1412 (626 / 655); the 256-set batched join
+// 166 / 183 (178 with one range per XCD above 2 dims).  KNN_TILE_RANGES=1|8 overrides it.
+static uint32_t tile_ranges(uint32_t d, bool batched) {
+  const char *v = std::getenv("KNN_TILE_RANGES");  // read per launch: tuning runs switch it in-process
+  if (v && std::atoi(v) == 1) return 1u;
+  if (v && std::atoi(v) == (int)TILE_QUEUES) return TILE_QUEUES;
+  return (d >= 3 && !batched) ? TILE_QUEUES : 1u;
+}
+
 // Builds tiles of <= 64 active queries per tile block: activeFailed[q] != 0 selects q (all
 // when null); act/tiles point into the cache, the tile count is left in device memory.
 // Tile blocks are windows of `span` cells along dim 0 within one row; the span is chosen on the
@@ -673,10 +688,11 @@ PassResult Pipeline<T>::pass(const uint8_t *activeFailed, uint32_t nq, uint8_t *
                              T *dDists, unsigned long long *ctr, Timer &tK, Timer &tPass,
                              const std::function<void()> &beforeWait, const std::function<void()> &afterHeap) {
   // ctr layout (u64 words): [0] pairs, [1] algorithmic bytes, [2] fail|overflow counters,
-  // [3] tile queue head, [4..5] pre-check accounting (candidate points, query cells)
+  // [4..5] pre-check accounting (candidate points, query cells), [16..19] the tile queue
+  // heads (TILE_QUEUES u32)
   uint32_t *counters = reinterpret_cast<uint32_t *>(ctr + 2);
   uint32_t *overflowCount = counters + 1;
-  uint32_t *tileCounter = counters + 2;
+  uint32_t *tileCounter = reinterpret_cast<uint32_t *>(ctr + 16);
   uint32_t *act;
   uint4 *tiles;
   const uint32_t *ntDev;
@@ -686,6 +702,7 @@ PassResult Pipeline<T>::pass(const uint8_t *activeFailed, uint32_t nq, uint8_t *
   uint8_t *overflow = buf<uint8_t>(S_OVERFLOW, n);
   fill_u8(overflow, 0, n, s);
   fill_u32(counters, 0u, 4, s);
+  fill_u32(tileCounter, 0u, TILE_QUEUES, s);
 
   JoinArgs a{};
   a.x = x;
@@ -710,6 +727,7 @@ PassResult Pipeline<T>::pass(const uint8_t *activeFailed, uint32_t nq, uint8_t *
   a.failCount = counters;
   a.candCount = ctr;
   a.tileCounter = tileCounter;
+  a.tileRanges = tile_ranges(d, false);
   // LDS staging slots per query: K+1 and 64 more (the items of the K+1-th neighbour's
   // bucket beyond K+1), at most 512, within the 64 KB a workgroup may allocate
   a.smax = std::min<uint32_t>(K1 + 64, 512u);
@@ -756,7 +774,7 @@ PassResult Pipeline<T>::pass(const uint8_t *activeFailed, uint32_t nq, uint8_t *
     const uint32_t *nt2;
     const uint32_t A1 = lastActive;
     make_tiles(overflow, act2, tiles2, nt2, nullptr, nullptr);
-    fill_u32(tileCounter, 0u, 1, s);
+    fill_u32(tileCounter, 0u, TILE_QUEUES, s);
     a.act = act2;
     a.tiles = tiles2;
     a.ntilesDev = nt2;
@@ -2268,7 +2286,7 @@ static void batch_build(BatchPlan &B, uint32_t nsets, const uint32_t *n, const u
     G.scratch = B.alloc<uint32_t>(sw);
     G.status = B.alloc<uint32_t>(4);
     G.setMaxBits = B.alloc<uint32_t>(G.nsets);
-    G.ctr = B.alloc<uint32_t>(8);
+    G.ctr = B.alloc<uint32_t>(8 + TILE_QUEUES);  // [8..]: the tile queue heads
     G.cand = B.alloc<unsigned long long>(4);
     G.failed = B.alloc<uint8_t>(N);
     G.overflow = B.alloc<uint8_t>(N);
@@ -2310,7 +2328,8 @@ static JoinArgs batch_args(const BatchPlan &B, const BatchGroup &G) {
   a.failed = G.failed;
   a.failCount = G.ctr + 1;
   a.candCount = G.cand;
-  a.tileCounter = G.ctr;
+  a.tileCounter = G.ctr + 8;
+  a.tileRanges = tile_ranges(G.d, true);
   a.heapScratch = G.heap;
   a.smax = G.smax;
   a.overflow = G.overflow;
@@ -2355,7 +2374,7 @@ static void batch_pass(const BatchPlan &B, BatchGroup &G, hipStream_t s) {
   scan_exclusive_u32(G.aflags, G.apos, N, G.scratch, s);
   compact_active(G.ids, G.aflags, G.apos, N, G.act, s);
   batch_tiles(G.cellStart, G.apos, G.cpos + N, N, B.tq, G.ntiles, G.tileOff, G.scratch, G.tiles, s);
-  fill_u32(G.ctr, 0u, 4, s);
+  fill_u32(G.ctr, 0u, 8 + TILE_QUEUES, s);
   if (use_collect(G.hsets.empty() ? 0u : G.hsets[0].nd, G.k1max))
     range_collect<float>(a, g0, s);
   else
@@ -2365,7 +2384,7 @@ static void batch_pass(const BatchPlan &B, BatchGroup &G, hipStream_t s) {
   scan_exclusive_u32(G.aflags, G.apos, N, G.scratch, s);
   compact_active(G.ids, G.aflags, G.apos, N, G.act, s);
   batch_tiles(G.cellStart, G.apos, G.cpos + N, N, B.tq, G.ntiles, G.tileOff, G.scratch, G.tiles, s);
-  fill_u32(G.ctr, 0u, 1, s);
+  fill_u32(G.ctr + 8, 0u, TILE_QUEUES, s);
   range_topk<float>(a, g0, s);
   fill_u8(G.overflow, 0, N, s);
 

@@ -137,6 +137,10 @@ void block_flags_dev(const uint64_t *cellKeys, uint32_t ncell, uint32_t ncell0, 
 void block_tiles_dev(const uint32_t *blockStart, const uint32_t *pos, const uint32_t *nblocks, uint32_t ncell,
                      uint32_t *ntiles, uint32_t *tileOff, uint32_t *scratch, uint4 *tiles, hipStream_t s);
 
+// Tile queues of the persistent range kernels: one per XCD (the chip's 8 L2 domains), each
+// over a contiguous eighth of the tiles.
+constexpr uint32_t TILE_QUEUES = 8;
+
 struct JoinArgs {
   const void *x;         // reordered dim-major points x[j*n + i]
   const void *xs;        // cell-sorted copy xs[j*n + p]
@@ -156,7 +160,8 @@ struct JoinArgs {
   uint8_t *failed;       // per point: 1 = failed this pass
   uint32_t *failCount;   // device counter (added to)
   unsigned long long *candCount;  // [0] (query, candidate) pairs, [1] algorithmic bytes (added to)
-  uint32_t *tileCounter; // dynamic tile queue head (zeroed before the launch)
+  uint32_t *tileCounter; // TILE_QUEUES dynamic tile queue heads (zeroed before the launch)
+  uint32_t tileRanges;   // 1: one queue over all tiles; TILE_QUEUES: one range per XCD
   void *heapScratch;     // heap kernel: global heap storage when K+1 does not fit in LDS
   uint32_t smax;         // select kernel: LDS staging slots per query
   uint8_t *overflow;     // select kernel: per point, 1 = needs the heap kernel

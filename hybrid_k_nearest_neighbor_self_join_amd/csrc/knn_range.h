@@ -331,11 +331,30 @@ __device__ __forceinline__ bool row_chord_lane(const G &g, const uint32_t *cur, 
   return l0 <= h0;
 }
 
-__device__ __forceinline__ uint32_t next_tile(uint32_t *counter, uint32_t lane) {
-  uint32_t t = 0;
-  if (lane == 0) t = atomicAdd(counter, 1u);
-  return (uint32_t)__shfl((int)t, 0, 64);
-}
+// XCD-aware tile queue of a persistent kernel.  Workgroups are dispatched to the 8 XCDs in
+// turn (workgroup b on XCD b % 8), each XCD with its own L2, and neighbouring tiles (cell
+// order) read mostly the same candidates: the tiles are dealt in TILE_QUEUES contiguous ranges,
+// a workgroup takes tiles from its XCD's range first (its L2 then serves neighbouring tiles)
+// and, once that is empty, from the other ranges in turn.  Wave-uniform.
+struct TileQueue {
+  uint32_t nr, r, left;  // ranges (1 or TILE_QUEUES), current range, ranges left to try
+  __device__ explicit TileQueue(uint32_t ranges)
+      : nr(ranges == TILE_QUEUES ? TILE_QUEUES : 1u), r(blockIdx.x % nr), left(nr) {}
+  // next tile index, NT when every range is empty
+  __device__ __forceinline__ uint32_t next(uint32_t *heads, uint32_t NT, uint32_t lane) {
+    while (left) {
+      const uint32_t b = (uint32_t)((uint64_t)NT * r / nr);
+      const uint32_t e = (uint32_t)((uint64_t)NT * (r + 1) / nr);
+      uint32_t t = 0;
+      if (lane == 0) t = atomicAdd(&heads[r], 1u);
+      t = (uint32_t)__shfl((int)t, 0, 64);
+      if (t < e - b) return b + t;
+      r = r + 1 == nr ? 0u : r + 1;
+      left--;
+    }
+    return NT;
+  }
+};
 
 // The grid, K+1 and row addressing of one tile: the kernel's own for a single set; the
 // tile's set (all queries of a tile share one cell, hence one set) in a batched join.
