@@ -621,6 +621,15 @@ static bool use_collect(uint32_t nd, uint32_t k1) {
   return env >= 0 ? env == 1 : nd >= 5;
 }
 
+// Kernel for the queries the select kernel cannot stage (staging overflow, or K+1 beyond its
+// LDS slots): the collect kernel (wave = query, LDS collector) up to K+1 = BRUTE_MAX_K1, the
+// per-lane heap kernel above.  KNN_OVERFLOW_KERNEL=heap|collect overrides it for tuning.
+static bool overflow_by_collect(uint32_t k1) {
+  const char *v = std::getenv("KNN_OVERFLOW_KERNEL");
+  if (k1 > BRUTE_MAX_K1) return false;
+  return !(v && std::strcmp(v, "heap") == 0);
+}
+
 // Tile ranges of a range-kernel launch: one per XCD for single sets of 3 or more data dims,
 // one queue for 2-D sets and batched groups.  Measured on one MI355X, each pair on one box
 // (ms per join, one queue against one range per XCD): 10M mixed 2-D points K=64, plain 35.0 /
@@ -740,11 +749,15 @@ PassResult Pipeline<T>::pass(const uint8_t *activeFailed, uint32_t nq, uint8_t *
   if (use_collect(nd, K1)) {
     range_collect<T>(a, g, s);
   } else if (!(a.smax > K1 && range_select_lds_bytes<T>(a.smax) <= 65536)) {
-    // K+1 too large for LDS staging: the heap kernel takes every query of the pass
-    a.ntiles = kHeapGrid;
-    size_t hs = range_heap_scratch_bytes<T>(kHeapGrid, K1);
-    a.heapScratch = hs ? g_cache.get(S_HEAP, hs) : nullptr;
-    range_topk<T>(a, g, s);
+    // K+1 too large for LDS staging: the collect (or heap) kernel takes every query of the pass
+    if (overflow_by_collect(K1)) {
+      range_collect<T>(a, g, s);
+    } else {
+      a.ntiles = kHeapGrid;
+      size_t hs = range_heap_scratch_bytes<T>(kHeapGrid, K1);
+      a.heapScratch = hs ? g_cache.get(S_HEAP, hs) : nullptr;
+      range_topk<T>(a, g, s);
+    }
   } else {
     range_select<T>(a, g, s);
     selectK = true;
@@ -768,7 +781,7 @@ PassResult Pipeline<T>::pass(const uint8_t *activeFailed, uint32_t nq, uint8_t *
   lastSpan = pin[PW_SPAN];
   r.noverflow = selectK ? pin[PW_OVF] : 0u;
   if (r.noverflow) {
-    // queries whose candidates overflow the staging slots: per-lane heap kernel
+    // queries whose candidates overflow the staging slots: the collect (or heap) kernel
     uint32_t *act2;
     uint4 *tiles2;
     const uint32_t *nt2;
@@ -778,12 +791,18 @@ PassResult Pipeline<T>::pass(const uint8_t *activeFailed, uint32_t nq, uint8_t *
     a.act = act2;
     a.tiles = tiles2;
     a.ntilesDev = nt2;
-    a.ntiles = kHeapGrid;
-    size_t hs = range_heap_scratch_bytes<T>(kHeapGrid, K1);
-    a.heapScratch = hs ? g_cache.get(S_HEAP, hs) : nullptr;
+    const bool byCollect = overflow_by_collect(K1);
+    a.ntiles = byCollect ? kPassGrid : kHeapGrid;
+    if (!byCollect) {
+      size_t hs = range_heap_scratch_bytes<T>(kHeapGrid, K1);
+      a.heapScratch = hs ? g_cache.get(S_HEAP, hs) : nullptr;
+    }
     Timer tH(s, 5);
     tH.start();
-    range_topk<T>(a, g, s);
+    if (byCollect)
+      range_collect<T>(a, g, s);
+    else
+      range_topk<T>(a, g, s);
     LAUNCH_CHECK();
     tH.stop_async();
     if (afterHeap) afterHeap();
@@ -2385,7 +2404,10 @@ static void batch_pass(const BatchPlan &B, BatchGroup &G, hipStream_t s) {
   compact_active(G.ids, G.aflags, G.apos, N, G.act, s);
   batch_tiles(G.cellStart, G.apos, G.cpos + N, N, B.tq, G.ntiles, G.tileOff, G.scratch, G.tiles, s);
   fill_u32(G.ctr + 8, 0u, TILE_QUEUES, s);
-  range_topk<float>(a, g0, s);
+  if (overflow_by_collect(G.k1max))
+    range_collect<float>(a, g0, s);
+  else
+    range_topk<float>(a, g0, s);
   fill_u8(G.overflow, 0, N, s);
 
   batch_grow_class(G.sets, G.setOf, G.actq, G.failed, N, G.radius, G.bin, G.ctr + 3, s);

@@ -73,7 +73,10 @@ def test_batched_pass_counts(graph_passes):
     _check_against_single(sets, ids, dists)
 
 
-def test_batched_edge_sets():
+@pytest.mark.parametrize("kern", ["collect", "heap"])
+def test_batched_edge_sets(monkeypatch, kern):
+    # the staging-overflow queries go to the collect kernel (default) or the heap kernel
+    monkeypatch.setenv("KNN_OVERFLOW_KERNEL", kern)
     rng = np.random.default_rng(3)
     sets = [
         (np.ones((40, 3), np.float32), 5),                               # all duplicates

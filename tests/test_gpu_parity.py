@@ -175,9 +175,11 @@ def test_errors_and_edges(oracle):
     assert np.array_equal(ids, bi) and np.array_equal(bits(d), bits(bd))
 
 
-def test_join_heavy_duplicates_use_heap_fallback(oracle):
+@pytest.mark.parametrize("kern", ["collect", "heap"])
+def test_join_heavy_duplicates_overflow_kernels(oracle, monkeypatch, kern):
     # 3000 copies of one point inside a uniform background: those queries' staged lists
-    # overflow and are finished by the heap kernel
+    # overflow and are finished by the collect kernel (default) or the heap kernel
+    monkeypatch.setenv("KNN_OVERFLOW_KERNEL", kern)
     rng = np.random.default_rng(12)
     pts = np.concatenate([np.full((3000, 2), 0.5, np.float32),
                           rng.uniform(0, 1, (20000, 2)).astype(np.float32)])[rng.permutation(23000)]
