@@ -145,8 +145,12 @@ def suite_highdim(args):
 
     models = learned_radius.register_default_models()
     n = args.n if args.n != 10_000_000 else 5_000_000
-    for d, nd, gen in [(4, 4, "uniform"), (4, 2, "uniform"), (4, 4, "skewed"), (4, 2, "skewed"),
-                       (6, 6, "uniform"), (6, 4, "uniform"), (6, 6, "skewed"), (6, 4, "skewed")]:
+    cases = [(4, 4, "uniform"), (4, 2, "uniform"), (4, 4, "skewed"), (4, 2, "skewed"),
+             (6, 6, "uniform"), (6, 4, "uniform"), (6, 6, "skewed"), (6, 4, "skewed")]
+    if args.case:
+        d0, nd0, gen0 = args.case.split(",")
+        cases = [(int(d0), int(nd0), gen0)]
+    for d, nd, gen in cases:
         pts = (synthetic.unit_cube if gen == "uniform" else synthetic.skewed_cube)(n, d, seed=0x5EED0100 + d * 10 + nd)
         out = {"suite": "highdim", "n": n, "d": d, "nd": nd, "data": gen, "k": 32}
         res = {}
@@ -314,6 +318,8 @@ def main():
     ap.add_argument("--suite", choices=["highdim", "batched", "brute", "hybrid"], default=None,
                     help="instead of the headline line: the 4-D/6-D cases, the 256-set batch, or "
                          "the brute-force path")
+    ap.add_argument("--case", default=None,
+                    help="--suite highdim: one case only, as d,nd,uniform|skewed (for profiling)")
     ap.add_argument("--batched-only", action="store_true",
                     help="--suite batched: time the batched join only (no one-by-one comparison), "
                          "for profiling")
