@@ -1397,7 +1397,7 @@ static double learned_grid_div(uint32_t d, uint32_t nd, uint32_t k1) {
 }
 
 // Quarter-octave bins per radius class with <= 2 indexed dims: 8 (two octaves) up to
-// K+1 = 80, 10 above; 1 with more indexed dims.  Measured on MI355X, 10M mixed 2-D points
+// K+1 = 80, 10 above; 2 with more indexed dims.  Measured on MI355X, 10M mixed 2-D points
 // (ms per join, classes of 8 bins on cells of R/3 against 10 bins on cells of R/4): K = 16
 // 28.9 / 28.2, K = 32 30.4 / 30.2, K = 64 39.4 / 40.4, K = 96 56.0 / 50.2, K = 128 66.5 /
 // 59.7, K = 256 128.1 / 117.6.  Two-octave classes on cells of R/3 beat one-octave classes
@@ -1405,9 +1405,11 @@ static double learned_grid_div(uint32_t d, uint32_t nd, uint32_t k1) {
 // build over all N points.  On a projected grid (fewer indexed than data dims) 6 bins on
 // cells of R/4; measured on 5M points, K = 32, learned ms (bins / cells: default of the
 // indexed dims, 4 / R/3, 6 / R/4): 4-D with 2 indexed, uniform 126 / 90 / 87, skewed
-// 202 / 213 / 191; 6-D with 4 indexed, uniform 270 / 204 / 202, skewed 1819 / 505 / 537 (one
-// bin on cells of R stays best with every dim indexed: 4-D uniform 28.8 against 84.5 and
-// 68.6 ms).  KNN_LEARNED_CLASS_BINS (1..16) overrides it for tuning.
+// 202 / 213 / 191; 6-D with 4 indexed, uniform 270 / 204 / 202, skewed 1819 / 505 / 537.
+// With 3 or more dims, every one indexed: 2 bins on cells of R (1 / 2 / 3 bins, ms: 4-D
+// uniform 32.2 / 31.5 / 34.4, skewed 52.8 / 49.5 / 48.2; 6-D uniform 97.9 / 97.3 / 118, skewed
+// 146 / 144 / 209; 4 bins on cells of R/3: 4-D uniform 84.5).  KNN_LEARNED_CLASS_BINS (1..16)
+// overrides it for tuning.
 static int learned_class_bins(uint32_t d, uint32_t nd, uint32_t k1) {
   static const int env = []() {
     const char *v = std::getenv("KNN_LEARNED_CLASS_BINS");
@@ -1416,7 +1418,7 @@ static int learned_class_bins(uint32_t d, uint32_t nd, uint32_t k1) {
   }();
   if (env > 0) return env;
   if (nd < d) return 6;
-  return nd <= 2 ? (k1 <= 80 ? 8 : 10) : 1;
+  return nd <= 2 ? (k1 <= 80 ? 8 : 10) : 2;
 }
 
 // The learned schedule: per-query initial radius from the MLP, then passes over radius
@@ -1453,7 +1455,8 @@ static void run_learned(Pipeline<T> &P, const knnjoin_params &p, knnjoin_stats &
 
   // ---- radius classes: the pending queries whose radius lies in the lowest occupied class
   // of bins [c, c + learned_class_bins) (two or two and a half octaves with <= 2 indexed dims,
-  // a quarter octave above) are joined on a grid matched to R (cell width R / learned_grid_div);
+  // one and a half on projected grids, half an octave above) are joined on a grid matched to R
+  // (cell width R / learned_grid_div);
   // queries that fail grow their radius (x2, or x2^(1/4) with 3 or more indexed dims), which
   // moves them to a later class.  One index build per occupied class.
   unsigned long long *ctr = P.template buf<unsigned long long>(S_CTR, 64);
@@ -1502,7 +1505,7 @@ static void run_learned(Pipeline<T> &P, const knnjoin_params &p, knnjoin_stats &
       passes++;
       break;
     }
-    // wide classes with <= 2 indexed dims and on projected grids; single bins above, where a
+    // wide classes with <= 2 indexed dims and on projected grids; half an octave above, where a
     // cell up to 2x the radius would cost up to 2^nd in block volume
     const int cHi = std::min(c + learned_class_bins(P.d, nd, P.K1) - 1, 63);
     uint32_t nq = 0;
