@@ -336,25 +336,24 @@ __device__ __forceinline__ bool row_chord_lane(const G &g, const uint32_t *cur, 
 // order) read mostly the same candidates: the tiles are dealt in TILE_QUEUES contiguous ranges,
 // a workgroup takes tiles from its XCD's range first (its L2 then serves neighbouring tiles)
 // and, once that is empty, from the other ranges in turn.  Wave-uniform.
-struct TileQueue {
-  uint32_t nr, r, left;  // ranges (1 or TILE_QUEUES), current range, ranges left to try
-  __device__ explicit TileQueue(uint32_t ranges)
-      : nr(ranges == TILE_QUEUES ? TILE_QUEUES : 1u), r(blockIdx.x % nr), left(nr) {}
-  // next tile index, NT when every range is empty
-  __device__ __forceinline__ uint32_t next(uint32_t *heads, uint32_t NT, uint32_t lane) {
-    while (left) {
-      const uint32_t b = (uint32_t)((uint64_t)NT * r / nr);
-      const uint32_t e = (uint32_t)((uint64_t)NT * (r + 1) / nr);
-      uint32_t t = 0;
-      if (lane == 0) t = atomicAdd(&heads[r], 1u);
-      t = (uint32_t)__shfl((int)t, 0, 64);
-      if (t < e - b) return b + t;
-      r = r + 1 == nr ? 0u : r + 1;
-      left--;
-    }
-    return NT;
+static_assert(TILE_QUEUES == 8, "tile ranges are split by a shift of 3");
+// Next tile index of the workgroup, NT when every range is empty.  ranges: 1 or TILE_QUEUES.
+// No state is kept between calls: a workgroup whose ranges are empty re-reads their heads (at
+// most TILE_QUEUES atomics per call once its own range is empty).
+__device__ __forceinline__ uint32_t next_tile(uint32_t *heads, uint32_t NT, uint32_t ranges, uint32_t lane) {
+  const uint32_t sh = ranges == TILE_QUEUES ? 3u : 0u;
+  const uint32_t mask = (1u << sh) - 1u;
+  for (uint32_t j = 0; j <= mask; j++) {
+    const uint32_t r = (blockIdx.x + j) & mask;
+    const uint32_t b = (uint32_t)(((uint64_t)NT * r) >> sh);
+    const uint32_t e = (uint32_t)(((uint64_t)NT * (r + 1)) >> sh);
+    uint32_t t = 0;
+    if (lane == 0) t = atomicAdd(&heads[r], 1u);
+    t = (uint32_t)__shfl((int)t, 0, 64);
+    if (t < e - b) return b + t;
   }
-};
+  return NT;
+}
 
 // The grid, K+1 and row addressing of one tile: the kernel's own for a single set; the
 // tile's set (all queries of a tile share one cell, hence one set) in a batched join.

@@ -40,9 +40,8 @@ __global__ __launch_bounds__(64) void k_range_collect(JoinArgs a, G g0) {
   const uint32_t NT = tile_count(a);
   constexpr int PR = D <= 8 ? 4 : 2;
   WaveTally tally;
-  TileQueue queue(a.tileRanges);
   for (;;) {
-    const uint32_t t = queue.next(a.tileCounter, NT, lane);
+    const uint32_t t = next_tile(a.tileCounter, NT, a.tileRanges, lane);
     if (t >= NT) break;
     const uint4 tl = a.tiles[t];
     const TileCtx<G> C = tile_ctx<G>(a, g0, tl);
@@ -235,9 +234,8 @@ __global__ __launch_bounds__(64) void k_range_heap(JoinArgs a, G g0) {
   const uint32_t NT = tile_count(a);
 
   WaveTally tally;
-  TileQueue queue(a.tileRanges);
   for (;;) {
-    const uint32_t t = queue.next(a.tileCounter, NT, lane);
+    const uint32_t t = next_tile(a.tileCounter, NT, a.tileRanges, lane);
     if (t >= NT) break;
     const uint4 tl = a.tiles[t];
     const TileCtx<G> C = tile_ctx<G>(a, g0, tl);

@@ -134,9 +134,8 @@ __global__ __launch_bounds__(64) void k_select(JoinArgs a, G g0) {
   const uint32_t hsh = (lane & 1u) << 4;
 
   WaveTally tally;
-  TileQueue queue(a.tileRanges);
   for (;;) {
-    const uint32_t t = queue.next(a.tileCounter, NT, lane);
+    const uint32_t t = next_tile(a.tileCounter, NT, a.tileRanges, lane);
     if (t >= NT) break;
     const uint4 tl = a.tiles[t];
     const TileCtx<G> C = tile_ctx<G>(a, g0, tl);
