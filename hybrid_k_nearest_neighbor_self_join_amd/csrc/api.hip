@@ -631,13 +631,13 @@ static bool overflow_by_collect(uint32_t k1) {
 }
 
 // Tile ranges of a range-kernel launch: one per XCD for single sets of 3 or more data dims,
-// one queue for 2-D sets and batched groups.  Measured on one MI355X, each pair on one box
+// one queue for 2-D sets and batched groups.  Measured on one MI355X, each pair in one process
 // (ms per join, one queue against one range per XCD): 10M mixed 2-D points K=64, plain 35.0 /
-// 35.9, learned 21.3 / 21.5; 5M points, K=32, plain (learned): 4-D uniform 30.8 / 29.8 (32.7
-// / 32.0), skewed 54.6 / 57.6 (52.3 / 53.1); 4-D on 2 indexed dims uniform 92.3 / 79.5, skewed
-// 206 / 216; 6-D uniform (collect kernel) 132 / 106 (120 / 98), skewed 188 / 178; 6-D on 4
-// indexed dims uniform 286 / 272, skewed 1382 / 1412 (626 / 655); the 256-set batched join
-// 166 / 183 (178 with one range per XCD above 2 dims).  KNN_TILE_RANGES=1|8 overrides it.
+// 35.9, learned 21.3 / 21.5; 5M points, K=32, plain unless marked: 4-D uniform 26.7 / 25.9,
+// skewed 49.3 / 52.2; 4-D on 2 indexed dims uniform 92.1 / 79.8, skewed (learned) 189 / 187;
+// 6-D on 4 indexed dims uniform 233 / 221, skewed (learned) 532 / 552; 6-D (collect kernel)
+// uniform 132 / 106, skewed 189 / 177; the 256-set batched join 166 / 183.
+// KNN_TILE_RANGES=1|8 overrides the choice for tuning.
 static uint32_t tile_ranges(uint32_t d, bool batched) {
   const char *v = std::getenv("KNN_TILE_RANGES");  // read per launch: tuning runs switch it in-process
   if (v && std::atoi(v) == 1) return 1u;
