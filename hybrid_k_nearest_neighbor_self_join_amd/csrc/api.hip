@@ -636,7 +636,7 @@ static bool overflow_by_collect(uint32_t k1) {
 // 35.9, learned 21.3 / 21.5; 5M points, K=32, plain unless marked: 4-D uniform 26.7 / 25.9,
 // skewed 49.3 / 52.2; 4-D on 2 indexed dims uniform 92.1 / 79.8, skewed (learned) 189 / 187;
 // 6-D on 4 indexed dims uniform 233 / 221, skewed (learned) 532 / 552; 6-D (collect kernel)
-// uniform 132 / 106, skewed 189 / 177; the 256-set batched join 166 / 183.
+// uniform 132 / 106, skewed 189 / 177; the 256-set batched join 144 / 164.
 // KNN_TILE_RANGES=1|8 overrides the choice for tuning.
 static uint32_t tile_ranges(uint32_t d, bool batched) {
   const char *v = std::getenv("KNN_TILE_RANGES");  // read per launch: tuning runs switch it in-process
