@@ -164,7 +164,7 @@ struct JoinArgs {
   uint32_t tileRanges;   // 1: one queue over all tiles; TILE_QUEUES: one range per XCD
   void *heapScratch;     // heap kernel: global heap storage when K+1 does not fit in LDS
   uint32_t smax;         // select kernel: LDS staging slots per query
-  uint8_t *overflow;     // select kernel: per point, 1 = needs the heap kernel
+  uint8_t *overflow;     // select kernel: per point, 1 = needs the overflow (collect or heap) kernel
   uint32_t *overflowCount;
   // batched joins: per-set grids / K+1 / row offsets (sets[setOf[q]]), and the tile count
   // read from device memory at kernel start (null: single set, `ntiles`)
@@ -173,7 +173,7 @@ struct JoinArgs {
   const uint32_t *ntilesDev;
 };
 // Two-pass select kernel (histogram, then bucket-ordered collect); queries whose candidates
-// do not fit in `smax` staging slots are flagged in `overflow` for the heap kernel.
+// do not fit in `smax` staging slots are flagged in `overflow` for the overflow kernel.
 template <typename T>
 void range_select(const JoinArgs &a, const typename GridOf<T>::type &g, hipStream_t s);
 template <typename T>
@@ -184,7 +184,8 @@ uint32_t range_select_grid(uint32_t ntiles);
 template <typename T>
 void range_collect(const JoinArgs &a, const typename GridOf<T>::type &g, hipStream_t s);
 size_t range_collect_lds_bytes(uint32_t k1);
-// Heap kernel (per-lane max-heap on (key, id)): handles any input, used for overflow queries.
+// Heap kernel (per-lane max-heap on (key, id)): handles any input; the overflow kernel above
+// the collector's K+1.
 template <typename T>
 void range_topk(const JoinArgs &a, const typename GridOf<T>::type &g, hipStream_t s);
 // bytes of heap scratch needed in global memory for `ntiles` tiles (0 if LDS is used)

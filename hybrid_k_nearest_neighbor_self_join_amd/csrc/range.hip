@@ -6,7 +6,8 @@
 // k_range_collect (5 or more indexed dims): wave = query, one pass over the query's own block
 //   into the LDS top-(K+1) collector of knn_collect.h, FP32-prefiltered against its threshold.
 // k_range_heap: per-lane max-heap of (key, id), in LDS when K+1 fits, else in global scratch;
-//   takes the select kernel's overflow queries.
+//   takes the select kernel's overflow queries above the collector's K+1 (the collect kernel
+//   takes them below).
 #include "knn_collect.h"
 #include "knn_range.h"
 

@@ -37,7 +37,7 @@
 //           the whole tile at once (lane = query).
 //   A query whose staged list exceeds `smax` LDS slots (e.g. thousands of duplicates in one
 //   bucket), or with 65536 or more candidates within Tq (16-bit counters), is flagged for the
-//   heap kernel (range.hip).
+//   overflow kernel (range.hip: the collect kernel, or the heap kernel above its K+1).
 //
 // This file is compiled once per KNN_SELECT_PART (0..5): float / double x three ranges of D,
 // each instantiating the kernel for every number of indexed dims 1..4 (all per-dim arrays in
@@ -401,7 +401,7 @@ __global__ __launch_bounds__(64) void k_select(JoinArgs a, G g0) {
     // ---- scan: b* per query; each lane's counts become exclusive slot offsets (saturated past
     // b*), written as the query's row of cursors: lane L's row at word L * SEL_RS
     // a lane with 65536 or more candidates within Tq may have carried into its pair partner's
-    // counters (the high half of the same words): both go to the heap kernel
+    // counters (the high half of the same words): both go to the overflow kernel
     const bool bigOwn = counting && inF >= 65536u;
     const bool big = bigOwn || __shfl_xor((int)bigOwn, 1, 64) != 0;
     uint32_t cum = 0, bstar = SEL_NB, cumB = 0;
@@ -423,7 +423,7 @@ __global__ __launch_bounds__(64) void k_select(JoinArgs a, G g0) {
 #pragma unroll
       for (uint32_t w = 0; w < (uint32_t)SEL_NB / 2; w++) myRow[w] = col[w];
     }
-    // buckets <= b* alone exceed the staging slots: the heap kernel takes the query
+    // buckets <= b* alone exceed the staging slots: the overflow kernel takes the query
     const bool preOvf = L.active && !big && bstar < (uint32_t)SEL_NBIN && cumB > SM;
     const bool go = L.active && !big && bstar < (uint32_t)SEL_NBIN && !preOvf;
     // every staged key is below (b* + 1) / scale (1 + 2^-51): the sub-block's reach
@@ -687,7 +687,7 @@ __global__ __launch_bounds__(64) void k_select(JoinArgs a, G g0) {
       }
       wave_lds_sync();
       // the staged count equals pass 1's cum(b*) (same keys, same buckets); anything else
-      // (an overflowing slot) sends the query to the heap kernel
+      // (an overflowing slot) sends the query to the overflow kernel
       if (__ballot(ovf) || nStaged != m) {
         if (lane == i) ovfMine = true;
         wave_lds_sync();
