@@ -630,6 +630,15 @@ static bool overflow_by_collect(uint32_t k1) {
   return !(v && std::strcmp(v, "heap") == 0);
 }
 
+// Positions per chunk of the range kernels' segment-start bitmap, log2: 11 (the whole 2048-bit
+// bitmap); KNN_SEG_CHUNK_LOG2=6..10 makes the chunks smaller so that the tests exercise the
+// chunk boundaries on small inputs (same rows, more chunks).
+static uint32_t seg_cap_log2() {
+  const char *v = std::getenv("KNN_SEG_CHUNK_LOG2");
+  const int c = v ? std::atoi(v) : 11;
+  return (c >= 6 && c <= 11) ? (uint32_t)c : 11u;
+}
+
 // Tile ranges of a range-kernel launch: one per XCD for single sets of 3 or more data dims,
 // one queue for 2-D sets and batched groups.  Measured on one MI355X, each pair in one process
 // (ms per join, one queue against one range per XCD): 10M mixed 2-D points K=64, plain 35.0 /
@@ -737,6 +746,7 @@ PassResult Pipeline<T>::pass(const uint8_t *activeFailed, uint32_t nq, uint8_t *
   a.candCount = ctr;
   a.tileCounter = tileCounter;
   a.tileRanges = tile_ranges(d, false);
+  a.segCapLog2 = seg_cap_log2();
   // LDS staging slots per query: K+1 and 64 more (the items of the K+1-th neighbour's
   // bucket beyond K+1), at most 512, within the 64 KB a workgroup may allocate
   a.smax = std::min<uint32_t>(K1 + 64, 512u);
@@ -2352,6 +2362,7 @@ static JoinArgs batch_args(const BatchPlan &B, const BatchGroup &G) {
   a.candCount = G.cand;
   a.tileCounter = G.ctr + 8;
   a.tileRanges = tile_ranges(G.d, true);
+  a.segCapLog2 = seg_cap_log2();
   a.heapScratch = G.heap;
   a.smax = G.smax;
   a.overflow = G.overflow;

@@ -162,6 +162,7 @@ struct JoinArgs {
   unsigned long long *candCount;  // [0] (query, candidate) pairs, [1] algorithmic bytes (added to)
   uint32_t *tileCounter; // TILE_QUEUES dynamic tile queue heads (zeroed before the launch)
   uint32_t tileRanges;   // 1: one queue over all tiles; TILE_QUEUES: one range per XCD
+  uint32_t segCapLog2;   // positions per chunk of the segment-start bitmap, log2 (6..11; 11 by default)
   void *heapScratch;     // heap kernel: global heap storage when K+1 does not fit in LDS
   uint32_t smax;         // select kernel: LDS staging slots per query
   uint8_t *overflow;     // select kernel: per point, 1 = needs the overflow (collect or heap) kernel

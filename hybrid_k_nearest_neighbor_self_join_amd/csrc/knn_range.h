@@ -567,10 +567,11 @@ __device__ __forceinline__ void stream_row(uint32_t pB, uint32_t len, uint32_t l
 // a bitmap of the segment starts (rowX: 2048 positions per chunk, one bit each): the segments
 // begun before its window of 64 positions, plus `mbcnt` of the window's starts at or below its
 // lane, index the segments' position offsets (rowB, in the order of the non-empty segments).
-// rowB / rowX: 64-word LDS arrays of the wave, free for group to ignore.
+// rowB / rowX: 64-word LDS arrays of the wave, free for group to ignore.  capLog2: log2 of the
+// positions per chunk, 6..11 (11: the whole bitmap; smaller only to exercise the chunking).
 template <int PR, typename GF>
 __device__ __forceinline__ void stream_segments(uint32_t pB, uint32_t len, uint32_t lane, uint32_t *rowB,
-                                                uint32_t *rowX, GF &&group) {
+                                                uint32_t *rowX, uint32_t capLog2, GF &&group) {
   const uint32_t inc = wave_incl_sum(len, lane);
   const uint32_t total = (uint32_t)__builtin_amdgcn_readlane((int)inc, 63);
   if (total == 0) return;
@@ -587,7 +588,7 @@ __device__ __forceinline__ void stream_segments(uint32_t pB, uint32_t len, uint3
   }
   const uint32_t rk = __builtin_amdgcn_mbcnt_hi((uint32_t)(nz >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)nz, 0u));
   if (len > 0) rowB[rk] = pB - excl;
-  constexpr uint32_t CAP = 64u * 32u;
+  const uint32_t CAP = 1u << capLog2;
   for (uint32_t c0 = 0; c0 < total; c0 += CAP) {
     rowX[lane] = 0u;
     wave_lds_sync();

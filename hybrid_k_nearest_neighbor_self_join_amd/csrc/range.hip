@@ -124,7 +124,7 @@ __global__ __launch_bounds__(64) void k_range_collect(JoinArgs a, G g0) {
           seg_batch<T, D, ND, G>(a, g, lo, hi, nd >= 3, qd, rF2, RS, rb + lane, sql, sqh, xs + (size_t)nd * n, pB, len);
         else
           row_batch<T, D, ND, G>(a, g, lo, hi, nd >= 3, qd, rF2, R, rb + lane, pB, len);
-        stream_segments<PR>(pB, len, lane, rowB, rowX, group);
+        stream_segments<PR>(pB, len, lane, rowB, rowX, a.segCapLog2, group);
       }
       col.merge(lane);
       const bool ok = col.nb == K1;

@@ -682,7 +682,7 @@ __global__ __launch_bounds__(64) void k_select(JoinArgs a, G g0) {
             seg_batch<T, D, ND, G>(a, g, slo, shi, nd >= 2, qdi, ri2, RS, rb + lane, sql, sqh, xsd, pB, len);
           else
             row_batch<T, D, ND, G>(a, g, slo, shi, nd >= 2, qdi, ri2, R, rb + lane, pB, len);
-          stream_segments<PR>(pB, len, lane, rowB, rowX, stage_group);
+          stream_segments<PR>(pB, len, lane, rowB, rowX, a.segCapLog2, stage_group);
         }
       }
       wave_lds_sync();
