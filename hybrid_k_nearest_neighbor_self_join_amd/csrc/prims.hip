@@ -335,17 +335,25 @@ __global__ __launch_bounds__(MM_BLOCK) void k_minmax_part(const T *x, uint32_t n
 }
 
 template <typename T>
-__global__ void k_minmax_final(const T *pmin, const T *pmax, uint32_t nb, uint32_t nd, T *mn,
-                               T *mx) {
-  const uint32_t j = threadIdx.x;
+// one wave per dim (block j): lane l folds partials l, l+64, ...; then a wave reduction (min
+// and max are exact in any order)
+__global__ __launch_bounds__(64) void k_minmax_final(const T *pmin, const T *pmax, uint32_t nb, uint32_t nd,
+                                                     T *mn, T *mx) {
+  const uint32_t j = blockIdx.x, l = threadIdx.x;
   if (j >= nd) return;
   T a = pmin[(size_t)j * nb], b = pmax[(size_t)j * nb];
-  for (uint32_t i = 1; i < nb; i++) {
+  for (uint32_t i = l; i < nb; i += 64) {
     a = tmin(a, pmin[(size_t)j * nb + i]);
     b = tmax(b, pmax[(size_t)j * nb + i]);
   }
-  mn[j] = a;
-  mx[j] = b;
+  for (int o = 32; o > 0; o >>= 1) {
+    a = tmin(a, (T)__shfl_xor(a, o, 64));
+    b = tmax(b, (T)__shfl_xor(b, o, 64));
+  }
+  if (l == 0) {
+    mn[j] = a;
+    mx[j] = b;
+  }
 }
 }  // namespace
 
@@ -359,7 +367,7 @@ void minmax_soa(const T *x, uint32_t n, uint32_t nd, T *mn, T *mx, T *scratch, h
   uint32_t chunk = (n + nb - 1) / nb;
   T *pmin = scratch, *pmax = scratch + (size_t)MM_MAXBLOCKS * nd;
   k_minmax_part<T><<<dim3(nb, nd), MM_BLOCK, 0, s>>>(x, n, chunk, pmin, pmax);
-  k_minmax_final<T><<<1, 64, 0, s>>>(pmin, pmax, nb, nd, mn, mx);
+  if (nd) k_minmax_final<T><<<nd, 64, 0, s>>>(pmin, pmax, nb, nd, mn, mx);
 }
 template void minmax_soa<float>(const float *, uint32_t, uint32_t, float *, float *, float *,
                                 hipStream_t);
