@@ -1544,6 +1544,16 @@ static void run_learned(Pipeline<T> &P, const knnjoin_params &p, knnjoin_stats &
     // dim), since every extra cell of reach multiplies the rows of a block
     T w = (T)(R / learned_grid_div(P.d, nd, P.K1));
     if (!(w > (T)0)) w = F.wf;
+    // a small tail of pending queries (at most N/64) keeps the current grid when its blocks
+    // stay within 1024 cells: an index build over all N points costs more than the wider
+    // blocks of the few queries left (10M mixed 2-D points, K=64: the fourth pass, 15.8k
+    // queries, without its build)
+    bool reuse = false;
+    if (curW > (T)0 && (uint64_t)nq * 64u <= (uint64_t)n) {
+      const double reach = std::ceil(R / (double)curW);
+      reuse = std::pow(2.0 * reach + 1.0, (double)nd) <= 1024.0;
+    }
+    if (reuse) w = curW;
     for (;;) {  // keep the grid within 2^30 cells
       typename GridOf<T>::type gt;
       uint64_t tc = 0;
@@ -1568,7 +1578,9 @@ static void run_learned(Pipeline<T> &P, const knnjoin_params &p, knnjoin_stats &
       curW = w;
       if (stats.eps0 == 0.0f) stats.eps0 = (float)w;
     }
-    P.avgReach = 1.5;
+    // typical reach in cells (tile-span cost model): 1.5 on the class's own grid, the half
+    // radius in cells on a kept one
+    P.avgReach = reuse ? std::max(1.5, 0.5 * R / (double)curW) : 1.5;
     radius_class_flags(radius, failed, n, wfF, (uint32_t)c, (uint32_t)cHi, act, s);
     // growth on failure: one class (an octave; a quarter octave with >= 3 indexed dims), then
     // the next histogram, both before the pass's wait; after a heap-kernel pass (overflowing
