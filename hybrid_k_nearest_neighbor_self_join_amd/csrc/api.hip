@@ -639,6 +639,17 @@ static uint32_t seg_cap_log2() {
   return (c >= 6 && c <= 11) ? (uint32_t)c : 11u;
 }
 
+// Pending-query fraction (1/div of N) at or below which the learned schedule keeps the current
+// grid for a pass; KNN_REUSE_TAIL=div overrides the default 64 for tuning.  A larger fraction
+// also keeps the first grid for the second pass (10M mixed 2-D points, 550k queries at K=64):
+// its wider blocks then cost what the skipped build saves (ms per join, div 64 against 8:
+// K=16 16.65 / 17.18, K=64 19.98 / 20.12, K=128 30.09 / 30.00).
+static uint64_t reuse_tail_div() {
+  const char *v = std::getenv("KNN_REUSE_TAIL");
+  const long c = v ? std::atol(v) : 64;
+  return c >= 1 ? (uint64_t)c : 64u;
+}
+
 // Tile ranges of a range-kernel launch: one per XCD for single sets of 3 or more data dims,
 // one queue for 2-D sets and batched groups.  Measured on one MI355X, each pair in one process
 // (ms per join, one queue against one range per XCD): 10M mixed 2-D points K=64, plain 35.0 /
@@ -1549,7 +1560,7 @@ static void run_learned(Pipeline<T> &P, const knnjoin_params &p, knnjoin_stats &
     // blocks of the few queries left (10M mixed 2-D points, K=64: the fourth pass, 15.8k
     // queries, without its build)
     bool reuse = false;
-    if (curW > (T)0 && (uint64_t)nq * 64u <= (uint64_t)n) {
+    if (curW > (T)0 && (uint64_t)nq * reuse_tail_div() <= (uint64_t)n) {
       const double reach = std::ceil(R / (double)curW);
       reuse = std::pow(2.0 * reach + 1.0, (double)nd) <= 1024.0;
     }
