@@ -53,7 +53,7 @@ enum Slot {
   S_PTS, S_X, S_XS, S_KEYS, S_KEYS_TMP, S_IDS, S_IDS_TMP, S_CELLKEYS, S_CELLSTART, S_FLAGS, S_POS,
   S_ACT, S_NTILES, S_TILEOFF, S_TILES, S_FAILED, S_ROWSUM, S_OUTIDS, S_OUTDISTS, S_SCRATCH,
   S_HEAP, S_SMALL, S_PART, S_CTR, S_CSUM, S_HOSTFLAGS, S_OVERFLOW, S_STAGE, S_DENSE, S_BFLAGS,
-  S_BPOS, S_BSTART, S_CELLOF, S_CELLMAX, S_FCNT, S_RADIUS, S_RHIST, S_MODEL, S_TRAIN, S_OCC, S_ACTQ, S_BRUTE_PK, S_BRUTE_PI, S_BLIST, S_TILEMETA, S_HCPU, S_HGPU, S_HLIST, S_HROWI, S_HROWD, S_XS2, S_SIDS, S_SIDS2, S_NSLOTS
+  S_BPOS, S_BSTART, S_CELLOF, S_CELLMAX, S_FCNT, S_RADIUS, S_RHIST, S_MODEL, S_TRAIN, S_OCC, S_ACTQ, S_BRUTE_PK, S_BRUTE_PI, S_BLIST, S_TILEMETA, S_HCPU, S_HGPU, S_HLIST, S_HROWI, S_HROWD, S_XS2, S_SIDS, S_SIDS2, S_BKEY, S_PARTNER, S_NSLOTS
 };
 
 // Slots of one no-allocation call: regions of the static device scratch array handed out in
@@ -478,6 +478,7 @@ struct Pipeline {
   const T *radius = nullptr;  // per-query radii (learned schedule), null = radiusAll
   T radiusAll = (T)0;         // one radius for every query (plain schedule); 0 = the grid's eps
   uint32_t lastSpan = 1;
+  uint32_t lastRows = 1;
   uint32_t lastActive = 0;   // queries left in tiles after the pre-check
   double avgReach = 1.0;     // typical search radius in cells (tile-span cost model)
   // persistent range-kernel workgroups: 11 per CU on 256 CUs (the select kernel's LDS,
@@ -650,6 +651,17 @@ static uint64_t reuse_tail_div() {
   return c >= 1 ? (uint64_t)c : 64u;
 }
 
+// Rows of a tile block with 2 indexed dims: up to 2 (k_choose_span picks 1 or 2 per pass from
+// the active density); KNN_TILE_ROWS=1 keeps single-row blocks, for tuning.  Measured on one
+// MI355X (ms per join, single rows against row pairs, same rows out): 10M mixed 2-D points,
+// K=64, plain 34.90 / 32.45, learned 20.00 / 20.01 (K=16 16.69 / 16.59, K=128 30.10 / 30.28);
+// with 4 indexed dims (5M uniform 4-D, K=32, plain) 25.8 / 26.2 with 15 % more candidates, as
+// the rows' chords narrow less for a union of two rows: above 2 indexed dims blocks keep one.
+static uint32_t tile_rows_max() {
+  const char *v = std::getenv("KNN_TILE_ROWS");  // read per pass: tuning runs switch it in-process
+  return (v && std::atoi(v) == 1) ? 1u : 2u;
+}
+
 // Tile ranges of a range-kernel launch: one per XCD for single sets of 3 or more data dims,
 // one queue for 2-D sets and batched groups.  Measured on one MI355X, each pair in one process
 // (ms per join, one queue against one range per XCD): 10M mixed 2-D points K=64, plain 35.0 /
@@ -697,18 +709,21 @@ void Pipeline<T>::make_tiles(const uint8_t *activeFailed, uint32_t *&act, uint4 
   activeDev = pos + n;
   uint32_t *meta = buf<uint32_t>(S_TILEMETA, 16);
   const uint32_t spanMax = nd == 1 ? 4096u : (nd == 2 ? 256u : 8u);
-  choose_span(activeDev, ncell, avgReach, spanMax, kMinTiles, meta, s);
+  choose_span(activeDev, ncell, avgReach, spanMax, nd == 2 ? tile_rows_max() : 1u, kMinTiles, meta, s);
   spanDev = meta;
   uint32_t *bflags = buf<uint32_t>(S_BFLAGS, (size_t)ncell + 1);
   uint32_t *bpos = buf<uint32_t>(S_BPOS, (size_t)ncell + 1);
   uint32_t *bstart = buf<uint32_t>(S_BSTART, (size_t)ncell + 1);
+  uint64_t *bkey = buf<uint64_t>(S_BKEY, (size_t)ncell + 1);
+  uint32_t *partner = buf<uint32_t>(S_PARTNER, (size_t)ncell + 1);
   block_flags_dev(cellKeys, ncell, g.nCells[0], meta, bflags, s);
   scan_exclusive_u32(bflags, bpos, ncell, scratch, s);
-  compact_blocks(cellStart, bflags, bpos, ncell, n, bstart, s);
+  compact_blocks_keyed(cellKeys, cellStart, bflags, bpos, ncell, g.nCells[0], meta, n, bstart, bkey, s);
   uint32_t *ntiles = buf<uint32_t>(S_NTILES, (size_t)ncell + 1);
   uint32_t *tileOff = buf<uint32_t>(S_TILEOFF, (size_t)ncell + 1);
   tiles = buf<uint4>(S_TILES, n);
-  block_tiles_dev(bstart, pos, bpos + ncell, ncell, ntiles, tileOff, scratch, tiles, s);
+  pair_tiles_dev(bstart, bkey, pos, bpos + ncell, ncell, g.nCells[0], nd >= 2 ? g.nCells[1] : 1u, meta, partner,
+                 ntiles, tileOff, scratch, tiles, s);
   ntilesDev = tileOff + ncell;
 }
 
@@ -790,7 +805,7 @@ PassResult Pipeline<T>::pass(const uint8_t *activeFailed, uint32_t nq, uint8_t *
   auto readback = [&]() {
     HIP_CHECK(hipMemcpyAsync(pin + PW_KFAIL, counters, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipMemcpyAsync(pin + PW_ACTIVE, activeDev, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-    HIP_CHECK(hipMemcpyAsync(pin + PW_SPAN, spanDev, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipMemcpyAsync(pin + PW_SPAN, spanDev, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipMemcpyAsync(pin + PW_CTR, ctr, 6 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
   };
   if (beforeWait) beforeWait();
@@ -800,6 +815,7 @@ PassResult Pipeline<T>::pass(const uint8_t *activeFailed, uint32_t nq, uint8_t *
   r.kernel_ms = tK.elapsed();
   lastActive = pin[PW_ACTIVE];
   lastSpan = pin[PW_SPAN];
+  lastRows = pin[PW_SPAN + 1];
   r.noverflow = selectK ? pin[PW_OVF] : 0u;
   if (r.noverflow) {
     // queries whose candidates overflow the staging slots: the collect (or heap) kernel
@@ -1367,8 +1383,8 @@ static void run_plain(Pipeline<T> &P, const knnjoin_params &prm, double beta, kn
     const uint32_t nfail = pr.nfail;
     joinMs += pr.pass_ms;
     if (P.verbose)
-      printf("[knn] iter %u eps %.9g queries %u failed %u cells %u span %u overflow %u candidates %llu kernel %.3f ms\n",
-             numIter, (double)eps, nq, nfail, P.ncell, P.lastSpan, pr.noverflow,
+      printf("[knn] iter %u eps %.9g queries %u failed %u cells %u span %u rows %u overflow %u candidates %llu kernel %.3f ms\n",
+             numIter, (double)eps, nq, nfail, P.ncell, P.lastSpan, P.lastRows, pr.noverflow,
              (unsigned long long)(pr.pairs - pairs), pr.kernel_ms);
     pairs = pr.pairs;
     frac = (nfail * 1.0) / (nq * 1.0);  // main.cu:402
@@ -1612,8 +1628,8 @@ static void run_learned(Pipeline<T> &P, const knnjoin_params &p, knnjoin_stats &
     prevBytes = pr.bytes_total;
     joinMs += pr.pass_ms;
     if (P.verbose)
-      printf("[knn] learned pass %u octave %d grid %.9g queries %u failed %u cells %u span %u overflow %u candidates %llu kernel %.3f ms\n",
-             passes, c, (double)P.g.eps, nq, pr.nfail, P.ncell, P.lastSpan, pr.noverflow,
+      printf("[knn] learned pass %u octave %d grid %.9g queries %u failed %u cells %u span %u rows %u overflow %u candidates %llu kernel %.3f ms\n",
+             passes, c, (double)P.g.eps, nq, pr.nfail, P.ncell, P.lastSpan, P.lastRows, pr.noverflow,
              (unsigned long long)(pr.pairs - pairs), pr.kernel_ms);
     pairs = pr.pairs;
     passes++;
@@ -1640,7 +1656,7 @@ static void *noalloc_scratch(size_t &bytes);
 // laid out): two coordinate copies, 64-bit keys and ids with their ping-pong copies, flags,
 // scans, cell and block tables, the active list and tiles, failure flags, radii, row sums.
 static size_t noalloc_bound(uint32_t n, uint32_t d, size_t tsz) {
-  const size_t perPoint = 2 * d * tsz + tsz + 16 + 8 + 8 + 8 + 4 + 4 + 4 + 16 + 3 + 8 + 24 + 4;
+  const size_t perPoint = 2 * d * tsz + tsz + 16 + 8 + 8 + 8 + 4 + 4 + 4 + 16 + 3 + 8 + 24 + 4 + 12;
   return (size_t)n * perPoint + (radix_scratch_words(n) + scan_scratch_words(n + 1) + 2048) * 4 +
          avgdist_scratch_doubles(n) * 8 + (2u << 20);
 }

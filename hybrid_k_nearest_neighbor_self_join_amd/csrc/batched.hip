@@ -373,8 +373,10 @@ __global__ void k_b_write_tiles(const uint32_t *cellStart, const uint32_t *pos, 
   if (c >= N || c >= *ncell) return;
   const uint32_t a0 = pos[cellStart[c]], a1 = pos[cellStart[c + 1]];
   uint32_t t = tileOff[c];
-  for (uint32_t a = a0; a < a1; a += tq, t++)
-    tiles[t] = make_uint4(c, a, (a1 - a < tq) ? a1 - a : tq, a == a0 ? 1u : 0u);
+  for (uint32_t a = a0; a < a1; a += tq, t++) {
+    const uint32_t m = (a1 - a < tq) ? a1 - a : tq;
+    tiles[t] = make_uint4(0u, a, m, m);
+  }
 }
 
 }  // namespace

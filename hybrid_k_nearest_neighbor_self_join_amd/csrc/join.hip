@@ -86,22 +86,6 @@ __device__ __forceinline__ uint64_t block_key(uint64_t id, uint32_t ncell0, uint
   return (id / ncell0) * nb0 + (id % ncell0) / span;
 }
 
-__global__ void k_block_flags(const uint64_t *cellKeys, uint32_t ncell, uint32_t ncell0,
-                              uint32_t span, uint32_t *flags) {
-  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= ncell) return;
-  flags[c] = (c == 0 || block_key(cellKeys[c], ncell0, span) != block_key(cellKeys[c - 1], ncell0, span)) ? 1u : 0u;
-}
-
-__global__ void k_compact_blocks(const uint32_t *cellStart, const uint32_t *flags,
-                                 const uint32_t *pos, uint32_t ncell, uint32_t n,
-                                 uint32_t *blockStart) {
-  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= ncell) return;
-  if (flags[c]) blockStart[pos[c]] = cellStart[c];
-  if (c == ncell - 1) blockStart[pos[ncell]] = n;
-}
-
 // Per active query (sorted position p): the number of points in its own cell block (radius
 // r = its own, or rAll for every query, cells [coord(x-r) rounded down, coord(x+r) rounded up]) from
 // the dense cell-start table.  Fewer than K+1 points there means the query cannot succeed
@@ -228,48 +212,35 @@ __global__ void k_compact_active(const uint32_t *sortedIds, const uint32_t *flag
   if (flags[p]) act[pos[p]] = sortedIds[p];
 }
 
-__global__ void k_cell_tile_counts(const uint32_t *cellStart, const uint32_t *pos, uint32_t ncell,
-                                   uint32_t *ntiles) {
-  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= ncell) return;
-  const uint32_t a0 = pos[cellStart[c]], a1 = pos[cellStart[c + 1]];
-  ntiles[c] = (a1 - a0 + 63) / 64;
-}
-
-__global__ void k_write_tiles(const uint32_t *cellStart, const uint32_t *pos,
-                              const uint32_t *tileOff, uint32_t ncell, uint4 *tiles) {
-  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= ncell) return;
-  const uint32_t a0 = pos[cellStart[c]], a1 = pos[cellStart[c + 1]];
-  uint32_t t = tileOff[c];
-  for (uint32_t a = a0; a < a1; a += 64, t++)
-    tiles[t] = make_uint4(c, a, (a1 - a < 64u) ? a1 - a : 64u, a == a0 ? 1u : 0u);
-}
-
-// Tile-block span of a pass from the active count A = *activeCount (read on the device): a
-// tile scans the union of its lanes' blocks, about (span + 2 reach) cells per row, for
-// min(64, span * rho) queries (rho = active queries per non-empty cell), so the cost per
-// query is about (span + 2 reach) / max(1, min(64, span * rho)); a pass with fewer than
-// minTiles tiles leaves the persistent waves idle at its end, which costs in proportion.
+// Tile-block shape of a pass from the active count A = *activeCount (read on the device): a
+// tile of `rows` rows (1, or 2 when rowsMax = 2) scans the union of its lanes' blocks, about
+// (span + 2 reach) x (rows + 2 reach) cells, for min(64, span * rows * rho) queries (rho =
+// active queries per non-empty cell), so the cost per query is about that area over the query
+// count; a pass with fewer than minTiles tiles leaves the persistent waves idle at its end,
+// which costs in proportion.  shape[0] = span, shape[1] = rows.
 __global__ void k_choose_span(const uint32_t *activeCount, uint32_t ncell, double avgReach,
-                              uint32_t spanMax, double minTiles, uint32_t *span) {
+                              uint32_t spanMax, uint32_t rowsMax, double minTiles, uint32_t *shape) {
   if (threadIdx.x || blockIdx.x) return;
   const uint32_t A = *activeCount;
-  uint32_t best = 1;
+  uint32_t best = 1, bestRows = 1;
   if (A && ncell) {
     const double rhoA = (double)A / (double)ncell;
     double bc = 1e300;
-    for (uint32_t w = 1; w <= spanMax; w *= 2) {
-      const double q = fmax(1.0, fmin(64.0, w * rhoA));
-      const double tilesEst = (double)A / q;
-      const double cost = (w + 2.0 * avgReach) / q * fmax(1.0, minTiles / tilesEst);
-      if (cost < bc * 0.97) {
-        bc = cost;
-        best = w;
+    for (uint32_t rows = 1; rows <= rowsMax; rows++) {
+      for (uint32_t w = 1; w <= spanMax; w *= 2) {
+        const double q = fmax(1.0, fmin(64.0, w * rows * rhoA));
+        const double tilesEst = (double)A / q;
+        const double cost = (w + 2.0 * avgReach) * (rows + 2.0 * avgReach) / q * fmax(1.0, minTiles / tilesEst);
+        if (cost < bc * 0.97) {
+          bc = cost;
+          best = w;
+          bestRows = rows;
+        }
       }
     }
   }
-  *span = best;
+  shape[0] = best;
+  shape[1] = bestRows;
 }
 
 __global__ void k_block_flags_dev(const uint64_t *cellKeys, uint32_t ncell, uint32_t ncell0,
@@ -280,27 +251,91 @@ __global__ void k_block_flags_dev(const uint64_t *cellKeys, uint32_t ncell, uint
   flags[c] = (c == 0 || block_key(cellKeys[c], ncell0, w) != block_key(cellKeys[c - 1], ncell0, w)) ? 1u : 0u;
 }
 
-// per tile block b < *nblocks (0 for b up to ncell): tiles of <= 64 active queries
-__global__ void k_block_tile_counts_dev(const uint32_t *blockStart, const uint32_t *pos,
-                                        const uint32_t *nblocks, uint32_t ncell, uint32_t *ntiles) {
+__global__ void k_compact_blocks_keyed(const uint64_t *cellKeys, const uint32_t *cellStart,
+                                       const uint32_t *flags, const uint32_t *pos, uint32_t ncell,
+                                       uint32_t ncell0, const uint32_t *shape, uint32_t n,
+                                       uint32_t *blockStart, uint64_t *blockKey) {
+  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncell) return;
+  if (flags[c]) {
+    blockStart[pos[c]] = cellStart[c];
+    blockKey[pos[c]] = block_key(cellKeys[c], ncell0, shape[0]);
+  }
+  if (c == ncell - 1) blockStart[pos[ncell]] = n;
+}
+
+constexpr uint32_t NO_PARTNER = 0xffffffffu;
+
+// first index in [lo, hi) of the increasing keys with keys[i] >= v
+__device__ __forceinline__ uint32_t lower_bound_keys(const uint64_t *keys, uint32_t lo, uint32_t hi, uint64_t v) {
+  while (lo < hi) {
+    const uint32_t mid = lo + ((hi - lo) >> 1);
+    if (keys[mid] < v) lo = mid + 1; else hi = mid;
+  }
+  return lo;
+}
+
+// per tile block b < *nblocks (0 for b up to ncell): tiles of <= 64 active queries.  With row
+// pairs (shape[1] = 2) a block of an even dim-1 coordinate takes its partner's queries (the
+// block one row up, same window; partner[b]), and a block whose partner one row down exists
+// has no tiles of its own.  Block keys increase with b, so each partner is one binary search.
+__global__ void k_pair_tile_counts(const uint32_t *blockStart, const uint64_t *blockKey, const uint32_t *pos,
+                                   const uint32_t *nblocks, uint32_t ncell, uint32_t ncell0, uint32_t ncell1,
+                                   const uint32_t *shape, uint32_t *partner, uint32_t *ntiles) {
   const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= ncell) return;
-  if (b >= *nblocks) {
+  const uint32_t nb = *nblocks;
+  if (b >= nb) {
     ntiles[b] = 0;
     return;
   }
-  const uint32_t a0 = pos[blockStart[b]], a1 = pos[blockStart[b + 1]];
-  ntiles[b] = (a1 - a0 + 63) / 64;
+  uint32_t cnt = pos[blockStart[b + 1]] - pos[blockStart[b]];
+  uint32_t pj = NO_PARTNER;
+  if (shape[1] == 2u && ncell1 > 1u) {
+    const uint64_t key = blockKey[b];
+    const uint64_t nw = (ncell0 + shape[0] - 1) / shape[0];  // windows per row
+    const uint32_t r1 = (uint32_t)((key / nw) % ncell1);       // the block's dim-1 coordinate
+    if ((r1 & 1u) == 0u) {
+      if (r1 + 1u < ncell1) {
+        const uint32_t j = lower_bound_keys(blockKey, b + 1, nb, key + nw);
+        if (j < nb && blockKey[j] == key + nw) {
+          pj = j;
+          cnt += pos[blockStart[j + 1]] - pos[blockStart[j]];
+        }
+      }
+    } else {
+      const uint32_t j = lower_bound_keys(blockKey, 0, b, key - nw);
+      if (j < b && blockKey[j] == key - nw) cnt = 0;
+    }
+  }
+  partner[b] = pj;
+  ntiles[b] = (cnt + 63) / 64;
 }
 
-__global__ void k_write_tiles_dev(const uint32_t *blockStart, const uint32_t *pos, const uint32_t *tileOff,
-                                  const uint32_t *nblocks, uint32_t ncell, uint4 *tiles) {
+// tiles of block b (and its partner): runs of 64 over the block's active queries followed by
+// the partner's, each tile as {second run start, first run start, count, first run length}
+__global__ void k_write_pair_tiles(const uint32_t *blockStart, const uint32_t *pos, const uint32_t *tileOff,
+                                   const uint32_t *nblocks, const uint32_t *partner, uint32_t ncell,
+                                   uint4 *tiles) {
   const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= ncell || b >= *nblocks) return;
-  const uint32_t a0 = pos[blockStart[b]], a1 = pos[blockStart[b + 1]];
   uint32_t t = tileOff[b];
-  for (uint32_t a = a0; a < a1; a += 64, t++)
-    tiles[t] = make_uint4(b, a, (a1 - a < 64u) ? a1 - a : 64u, a == a0 ? 1u : 0u);
+  if (tileOff[b + 1] == t) return;
+  const uint32_t a0 = pos[blockStart[b]], n0 = pos[blockStart[b + 1]] - a0;
+  const uint32_t j = partner[b];
+  uint32_t a1 = 0, n1 = 0;
+  if (j != NO_PARTNER) {
+    a1 = pos[blockStart[j]];
+    n1 = pos[blockStart[j + 1]] - a1;
+  }
+  const uint32_t tot = n0 + n1;
+  for (uint32_t base = 0; base < tot; base += 64, t++) {
+    const uint32_t m = tot - base < 64u ? tot - base : 64u;
+    const uint32_t m0 = base < n0 ? (n0 - base < m ? n0 - base : m) : 0u;
+    const uint32_t s0 = a0 + (base < n0 ? base : n0);
+    const uint32_t s1 = a1 + (base > n0 ? base - n0 : 0u);
+    tiles[t] = make_uint4(s1, s0, m, m0);
+  }
 }
 
 // ------------------------------------------------------------------ checksum
@@ -443,15 +478,6 @@ void dense_cell_counts(const uint64_t *cellKeys, const uint32_t *cellStart, uint
                        uint32_t *dense, hipStream_t s) {
   if (ncell) k_dense_counts<<<(ncell + 255) / 256, 256, 0, s>>>(cellKeys, cellStart, ncell, dense);
 }
-void block_flags(const uint64_t *cellKeys, uint32_t ncell, uint32_t ncell0, uint32_t span,
-                 uint32_t *flags, hipStream_t s) {
-  if (ncell) k_block_flags<<<(ncell + 255) / 256, 256, 0, s>>>(cellKeys, ncell, ncell0, span, flags);
-}
-void compact_blocks(const uint32_t *cellStart, const uint32_t *flags, const uint32_t *pos,
-                    uint32_t ncell, uint32_t n, uint32_t *blockStart, hipStream_t s) {
-  if (ncell) k_compact_blocks<<<(ncell + 255) / 256, 256, 0, s>>>(cellStart, flags, pos, ncell, n, blockStart);
-}
-
 template <typename T>
 void prefilter(const T *xs, uint32_t n, const typename GridOf<T>::type &g, const T *radius,
                T rAll, const uint32_t *dense, const uint32_t *sortedIds, const uint8_t *activeSrc,
@@ -486,29 +512,30 @@ void compact_active(const uint32_t *sortedIds, const uint32_t *flags, const uint
                     uint32_t n, uint32_t *act, hipStream_t s) {
   if (n) k_compact_active<<<(n + 255) / 256, 256, 0, s>>>(sortedIds, flags, pos, n, act);
 }
-void cell_tile_counts(const uint32_t *cellStart, const uint32_t *pos, uint32_t ncell,
-                      uint32_t *ntiles, hipStream_t s) {
-  if (ncell) k_cell_tile_counts<<<(ncell + 255) / 256, 256, 0, s>>>(cellStart, pos, ncell, ntiles);
-}
-void write_tiles(const uint32_t *cellStart, const uint32_t *pos, const uint32_t *tileOff,
-                 uint32_t ncell, uint4 *tiles, hipStream_t s) {
-  if (ncell) k_write_tiles<<<(ncell + 255) / 256, 256, 0, s>>>(cellStart, pos, tileOff, ncell, tiles);
-}
-
 void choose_span(const uint32_t *activeCount, uint32_t ncell, double avgReach, uint32_t spanMax,
-                 double minTiles, uint32_t *span, hipStream_t s) {
-  k_choose_span<<<1, 64, 0, s>>>(activeCount, ncell, avgReach, spanMax, minTiles, span);
+                 uint32_t rowsMax, double minTiles, uint32_t *shape, hipStream_t s) {
+  k_choose_span<<<1, 64, 0, s>>>(activeCount, ncell, avgReach, spanMax, rowsMax, minTiles, shape);
 }
 void block_flags_dev(const uint64_t *cellKeys, uint32_t ncell, uint32_t ncell0, const uint32_t *span,
                      uint32_t *flags, hipStream_t s) {
   if (ncell) k_block_flags_dev<<<(ncell + 255) / 256, 256, 0, s>>>(cellKeys, ncell, ncell0, span, flags);
 }
-void block_tiles_dev(const uint32_t *blockStart, const uint32_t *pos, const uint32_t *nblocks, uint32_t ncell,
-                     uint32_t *ntiles, uint32_t *tileOff, uint32_t *scratch, uint4 *tiles, hipStream_t s) {
+void compact_blocks_keyed(const uint64_t *cellKeys, const uint32_t *cellStart, const uint32_t *flags,
+                          const uint32_t *pos, uint32_t ncell, uint32_t ncell0, const uint32_t *shape,
+                          uint32_t n, uint32_t *blockStart, uint64_t *blockKey, hipStream_t s) {
+  if (ncell)
+    k_compact_blocks_keyed<<<(ncell + 255) / 256, 256, 0, s>>>(cellKeys, cellStart, flags, pos, ncell, ncell0,
+                                                                shape, n, blockStart, blockKey);
+}
+void pair_tiles_dev(const uint32_t *blockStart, const uint64_t *blockKey, const uint32_t *pos,
+                    const uint32_t *nblocks, uint32_t ncell, uint32_t ncell0, uint32_t ncell1,
+                    const uint32_t *shape, uint32_t *partner, uint32_t *ntiles, uint32_t *tileOff,
+                    uint32_t *scratch, uint4 *tiles, hipStream_t s) {
   if (!ncell) return;
-  k_block_tile_counts_dev<<<(ncell + 255) / 256, 256, 0, s>>>(blockStart, pos, nblocks, ncell, ntiles);
+  k_pair_tile_counts<<<(ncell + 255) / 256, 256, 0, s>>>(blockStart, blockKey, pos, nblocks, ncell, ncell0, ncell1,
+                                                         shape, partner, ntiles);
   scan_exclusive_u32(ntiles, tileOff, ncell, scratch, s);
-  k_write_tiles_dev<<<(ncell + 255) / 256, 256, 0, s>>>(blockStart, pos, tileOff, nblocks, ncell, tiles);
+  k_write_pair_tiles<<<(ncell + 255) / 256, 256, 0, s>>>(blockStart, pos, tileOff, nblocks, partner, ncell, tiles);
 }
 
 template <typename T>

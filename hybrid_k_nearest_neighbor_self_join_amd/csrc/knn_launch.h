@@ -95,13 +95,6 @@ void gather_rows(const T *in, const uint32_t *ids, uint32_t n, uint32_t d, const
 // cell table, then scanned in place.
 void dense_cell_counts(const uint64_t *cellKeys, const uint32_t *cellStart, uint32_t ncell,
                        uint32_t *dense, hipStream_t s);
-// Tile blocks: runs of cells in one row (dims 1.. equal) whose dim-0 coordinate falls in the
-// same window of `span` cells.  blockStart[b] = first sorted position of block b,
-// blockStart[B] = n.  flags/pos are ncell + 1 scratch words.
-void block_flags(const uint64_t *cellKeys, uint32_t ncell, uint32_t ncell0, uint32_t span,
-                 uint32_t *flags, hipStream_t s);
-void compact_blocks(const uint32_t *cellStart, const uint32_t *flags, const uint32_t *pos,
-                    uint32_t ncell, uint32_t n, uint32_t *blockStart, hipStream_t s);
 
 // Active flags per sorted position with the own-block pre-check: queries whose own cell
 // block holds fewer than k1 points get failed[q] = 1 and flag 0.  activeSrc null = all
@@ -119,23 +112,30 @@ void active_flags(const uint32_t *sortedIds, const uint8_t *failed, uint32_t n, 
                   hipStream_t s);
 void compact_active(const uint32_t *sortedIds, const uint32_t *flags, const uint32_t *pos,
                     uint32_t n, uint32_t *act, hipStream_t s);
-// per cell: tiles of up to 64 active queries (pos is the exclusive scan of active flags,
-// with pos[n] = total)
-void cell_tile_counts(const uint32_t *cellStart, const uint32_t *pos, uint32_t ncell,
-                      uint32_t *ntiles, hipStream_t s);
-void write_tiles(const uint32_t *cellStart, const uint32_t *pos, const uint32_t *tileOff,
-                 uint32_t ncell, uint4 *tiles, hipStream_t s);
 
-// The same tile build with every count read on the device (no host round trip): the span from
-// the active count *activeCount (k_choose_span's cost model), block flags at that span, and
-// tiles of the *nblocks blocks (ntiles / tileOff hold ncell + 1 words; tileOff[ncell] = tile
-// count, read by the range kernels through JoinArgs::ntilesDev).
+// Tile blocks: runs of cells in one row (dims 1.. equal) whose dim-0 coordinate falls in the
+// same window of `span` cells (blockStart[b] = first sorted position of block b, blockStart[B]
+// = n; flags/pos are ncell + 1 scratch words), built with every count read on the device (no
+// host round trip; pos is the exclusive scan of the active flags, pos[n] = total): the block
+// shape from the active count *activeCount (k_choose_span's cost model: shape[0] = span,
+// shape[1] = rows), block flags at that span, and tiles of the *nblocks blocks (ntiles /
+// tileOff hold ncell + 1 words; tileOff[ncell] = tile count, read by the range kernels through
+// JoinArgs::ntilesDev).
 void choose_span(const uint32_t *activeCount, uint32_t ncell, double avgReach, uint32_t spanMax,
-                 double minTiles, uint32_t *span, hipStream_t s);
+                 uint32_t rowsMax, double minTiles, uint32_t *shape, hipStream_t s);
 void block_flags_dev(const uint64_t *cellKeys, uint32_t ncell, uint32_t ncell0, const uint32_t *span,
                      uint32_t *flags, hipStream_t s);
-void block_tiles_dev(const uint32_t *blockStart, const uint32_t *pos, const uint32_t *nblocks, uint32_t ncell,
-                     uint32_t *ntiles, uint32_t *tileOff, uint32_t *scratch, uint4 *tiles, hipStream_t s);
+// Row pairs: with shape[1] = 2 (k_choose_span's choice when rowsMax = 2), the tiles of the
+// block at (row r, window w) of an even dim-1 coordinate r also take the active queries of the
+// block at (r + 1, w), so a tile's queries fill a window of two rows; blockKey[b] = row *
+// windows per row + window of block b (compact_blocks_keyed).  partner holds ncell words.
+void compact_blocks_keyed(const uint64_t *cellKeys, const uint32_t *cellStart, const uint32_t *flags,
+                          const uint32_t *pos, uint32_t ncell, uint32_t ncell0, const uint32_t *shape,
+                          uint32_t n, uint32_t *blockStart, uint64_t *blockKey, hipStream_t s);
+void pair_tiles_dev(const uint32_t *blockStart, const uint64_t *blockKey, const uint32_t *pos,
+                    const uint32_t *nblocks, uint32_t ncell, uint32_t ncell0, uint32_t ncell1,
+                    const uint32_t *shape, uint32_t *partner, uint32_t *ntiles, uint32_t *tileOff,
+                    uint32_t *scratch, uint4 *tiles, hipStream_t s);
 
 // Tile queues of the persistent range kernels: one per XCD (the chip's 8 L2 domains), each
 // over a contiguous eighth of the tiles.
@@ -150,7 +150,9 @@ struct JoinArgs {
   uint32_t ncell;
   const uint32_t *denseStart;  // dense cell-start table, or null (binary search of cellKeys)
   const uint32_t *act;   // active query ids, grouped by cell
-  const uint4 *tiles;    // {block, first active position, count, first tile of its block}
+  // tile = up to 64 active queries in at most two runs of act: {start of the second run, start
+  // of the first, query count, length of the first run (= count for a single run)}
+  const uint4 *tiles;
   uint32_t ntiles;       // tile count; with ntilesDev: the persistent grid size (blocks)
   uint32_t n, d, k1;
   const void *radius;    // per-query search radius (T); null = radiusAll for all

@@ -152,6 +152,12 @@ struct Lane {
   T F;        // filter bound on the T sum
 };
 
+// Query i < tl.z of a tile: the first run act[tl.y ..) holds tl.w of them, the second run
+// act[tl.x ..) the rest.
+__device__ __forceinline__ uint32_t tile_query(const JoinArgs &a, const uint4 &tl, uint32_t i) {
+  return a.act[i < tl.w ? tl.y + i : tl.x + (i - tl.w)];
+}
+
 // Lane = query of the tile: its coordinates, radius r (its own, or the pass's one radius) and
 // Tq = r^2 (1 - 2^-20); lo/hi = the wave's union of the lanes' cell ranges per indexed dim.
 template <typename T, int D, int ND, typename G>
@@ -160,7 +166,7 @@ __device__ __forceinline__ void tile_setup(const JoinArgs &a, const G &g, const 
   const T *__restrict__ x = reinterpret_cast<const T *>(a.x);
   const size_t n = a.n;
   L.active = lane < tl.z;
-  L.q = a.act[tl.y + (L.active ? lane : 0u)];
+  L.q = tile_query(a, tl, L.active ? lane : 0u);
 #pragma unroll
   for (int j = 0; j < D; j++) {
     L.qc[j] = x[(size_t)j * n + L.q];
@@ -374,7 +380,7 @@ __device__ __forceinline__ TileCtx<G> tile_ctx(const JoinArgs &a, const G &g, co
   c.rowAdj = 0;
   if constexpr (std::is_same<G, GridParams>::value) {
     if (a.sets) {
-      const SetDesc &sd = a.sets[a.setOf[a.act[tl.y]]];
+      const SetDesc &sd = a.sets[a.setOf[tile_query(a, tl, 0u)]];
       c.g = sd.g;
       c.k1 = sd.k1;
       c.idBase = sd.pStart;

@@ -53,7 +53,7 @@ __global__ __launch_bounds__(64) void k_range_collect(JoinArgs a, G g0) {
     uint32_t nok = 0;
     unsigned long long cand = 0;
     for (uint32_t i = 0; i < tl.z; i++) {
-      const uint32_t q = (uint32_t)__builtin_amdgcn_readfirstlane((int)a.act[tl.y + i]);
+      const uint32_t q = (uint32_t)__builtin_amdgcn_readfirstlane((int)tile_query(a, tl, i));
       T qc[D];
       double qd[D];
 #pragma unroll
