@@ -457,7 +457,14 @@ struct Pipeline {
   // 1 GiB of cell starts at most: filled and scanned once per build (~0.6 ms at 2^28 cells),
   // against a binary search of the cell table for every row of every block without it
   static constexpr uint64_t kDenseMaxCells = 1ull << 28;
-  static constexpr double kMinTiles = 8192.0;
+  // tiles a pass should have before the cost model widens its blocks (2816 persistent waves;
+  // KNN_MIN_TILES overrides it for tuning: 2816, 4096, 8192 and 16384 are within 1 % of each
+  // other on the 10M mixed 2-D joins, K = 16 to 128)
+  static double min_tiles() {
+    const char *v = std::getenv("KNN_MIN_TILES");
+    const double t = v ? std::atof(v) : 8192.0;
+    return t >= 1.0 ? t : 8192.0;
+  }
   const uint32_t *denseStart = nullptr;
   uint32_t *cellOf = nullptr;  // compacted cell index of each sorted position
 
@@ -662,6 +669,18 @@ static uint32_t tile_rows_max() {
   return (v && std::atoi(v) == 1) ? 1u : 2u;
 }
 
+// Density basis of the tile-block cost model: active queries per cell holding any (default),
+// or KNN_SPAN_DENSITY=cells for active queries per non-empty cell, for tuning.  Clustered
+// active queries fill a block's 64 query slots over fewer cells than the average over all
+// non-empty cells says, and further tiles of the block rescan the same union.  Measured on one
+// MI355X with row pairs (ms per join, per non-empty cell against per active cell, same rows
+// out): 10M mixed 2-D points, learned K=16 16.90 / 15.55, K=64 20.22 / 19.92, K=128 30.50 /
+// 29.89, plain K=64 32.54 / 32.07; 5M points, K=32, 3 and 4 indexed dims: equal candidates.
+static bool span_by_active_cells() {
+  const char *v = std::getenv("KNN_SPAN_DENSITY");  // read per pass: tuning runs switch it in-process
+  return !(v && std::strcmp(v, "cells") == 0);
+}
+
 // Tile ranges of a range-kernel launch: one per XCD for single sets of 3 or more data dims,
 // one queue for 2-D sets and batched groups.  Measured on one MI355X, each pair in one process
 // (ms per join, one queue against one range per XCD): 10M mixed 2-D points K=64, plain 35.0 /
@@ -709,7 +728,9 @@ void Pipeline<T>::make_tiles(const uint8_t *activeFailed, uint32_t *&act, uint4 
   activeDev = pos + n;
   uint32_t *meta = buf<uint32_t>(S_TILEMETA, 16);
   const uint32_t spanMax = nd == 1 ? 4096u : (nd == 2 ? 256u : 8u);
-  choose_span(activeDev, ncell, avgReach, spanMax, nd == 2 ? tile_rows_max() : 1u, kMinTiles, meta, s);
+  // the pre-check's count of cells holding active queries (cellAcc[1]) as the density basis
+  const unsigned long long *activeCells = (cellAcc && span_by_active_cells()) ? cellAcc + 1 : nullptr;
+  choose_span(activeDev, activeCells, ncell, avgReach, spanMax, nd == 2 ? tile_rows_max() : 1u, min_tiles(), meta, s);
   spanDev = meta;
   uint32_t *bflags = buf<uint32_t>(S_BFLAGS, (size_t)ncell + 1);
   uint32_t *bpos = buf<uint32_t>(S_BPOS, (size_t)ncell + 1);

@@ -215,16 +215,19 @@ __global__ void k_compact_active(const uint32_t *sortedIds, const uint32_t *flag
 // Tile-block shape of a pass from the active count A = *activeCount (read on the device): a
 // tile of `rows` rows (1, or 2 when rowsMax = 2) scans the union of its lanes' blocks, about
 // (span + 2 reach) x (rows + 2 reach) cells, for min(64, span * rows * rho) queries (rho =
-// active queries per non-empty cell), so the cost per query is about that area over the query
-// count; a pass with fewer than minTiles tiles leaves the persistent waves idle at its end,
-// which costs in proportion.  shape[0] = span, shape[1] = rows.
-__global__ void k_choose_span(const uint32_t *activeCount, uint32_t ncell, double avgReach,
-                              uint32_t spanMax, uint32_t rowsMax, double minTiles, uint32_t *shape) {
+// active queries per cell holding any, *activeCells when given, else per non-empty cell), so
+// the cost per query is about that area over the query count; a pass with fewer than minTiles
+// tiles leaves the persistent waves idle at its end, which costs in proportion.  shape[0] =
+// span, shape[1] = rows.
+__global__ void k_choose_span(const uint32_t *activeCount, const unsigned long long *activeCells,
+                              uint32_t ncell, double avgReach, uint32_t spanMax, uint32_t rowsMax,
+                              double minTiles, uint32_t *shape) {
   if (threadIdx.x || blockIdx.x) return;
   const uint32_t A = *activeCount;
   uint32_t best = 1, bestRows = 1;
   if (A && ncell) {
-    const double rhoA = (double)A / (double)ncell;
+    const unsigned long long ac = activeCells ? *activeCells : 0ull;
+    const double rhoA = (double)A / (ac ? (double)ac : (double)ncell);
     double bc = 1e300;
     for (uint32_t rows = 1; rows <= rowsMax; rows++) {
       for (uint32_t w = 1; w <= spanMax; w *= 2) {
@@ -512,9 +515,10 @@ void compact_active(const uint32_t *sortedIds, const uint32_t *flags, const uint
                     uint32_t n, uint32_t *act, hipStream_t s) {
   if (n) k_compact_active<<<(n + 255) / 256, 256, 0, s>>>(sortedIds, flags, pos, n, act);
 }
-void choose_span(const uint32_t *activeCount, uint32_t ncell, double avgReach, uint32_t spanMax,
-                 uint32_t rowsMax, double minTiles, uint32_t *shape, hipStream_t s) {
-  k_choose_span<<<1, 64, 0, s>>>(activeCount, ncell, avgReach, spanMax, rowsMax, minTiles, shape);
+void choose_span(const uint32_t *activeCount, const unsigned long long *activeCells, uint32_t ncell,
+                 double avgReach, uint32_t spanMax, uint32_t rowsMax, double minTiles, uint32_t *shape,
+                 hipStream_t s) {
+  k_choose_span<<<1, 64, 0, s>>>(activeCount, activeCells, ncell, avgReach, spanMax, rowsMax, minTiles, shape);
 }
 void block_flags_dev(const uint64_t *cellKeys, uint32_t ncell, uint32_t ncell0, const uint32_t *span,
                      uint32_t *flags, hipStream_t s) {

@@ -121,8 +121,9 @@ void compact_active(const uint32_t *sortedIds, const uint32_t *flags, const uint
 // shape[1] = rows), block flags at that span, and tiles of the *nblocks blocks (ntiles /
 // tileOff hold ncell + 1 words; tileOff[ncell] = tile count, read by the range kernels through
 // JoinArgs::ntilesDev).
-void choose_span(const uint32_t *activeCount, uint32_t ncell, double avgReach, uint32_t spanMax,
-                 uint32_t rowsMax, double minTiles, uint32_t *shape, hipStream_t s);
+void choose_span(const uint32_t *activeCount, const unsigned long long *activeCells, uint32_t ncell,
+                 double avgReach, uint32_t spanMax, uint32_t rowsMax, double minTiles, uint32_t *shape,
+                 hipStream_t s);
 void block_flags_dev(const uint64_t *cellKeys, uint32_t ncell, uint32_t ncell0, const uint32_t *span,
                      uint32_t *flags, hipStream_t s);
 // Row pairs: with shape[1] = 2 (k_choose_span's choice when rowsMax = 2), the tiles of the
