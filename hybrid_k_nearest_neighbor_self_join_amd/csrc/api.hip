@@ -681,6 +681,18 @@ static bool span_by_active_cells() {
   return !(v && std::strcmp(v, "cells") == 0);
 }
 
+// Replica grids of the learned features' point counts: up to 16, within 2^24 counters in all
+// (none in a no-allocation call, whose scratch is sized for one grid); KNN_FEATURE_REPS=1..16
+// overrides it for tuning.
+static uint32_t feature_reps(uint64_t cells) {
+  if (g_arena || cells == 0) return 1u;
+  const char *v = std::getenv("KNN_FEATURE_REPS");
+  const long e = v ? std::atol(v) : 0;
+  if (e >= 1 && e <= 16) return (uint32_t)e;
+  const uint64_t r = (1ull << 24) / cells;
+  return r < 1 ? 1u : (r > 16 ? 16u : (uint32_t)r);
+}
+
 // Tile ranges of a range-kernel launch: one per XCD for single sets of 3 or more data dims,
 // one queue for 2-D sets and batched groups.  Measured on one MI355X, each pair in one process
 // (ms per join, one queue against one range per XCD): 10M mixed 2-D points K=64, plain 35.0 /
@@ -1499,9 +1511,10 @@ static void run_learned(Pipeline<T> &P, const knnjoin_params &p, knnjoin_stats &
   tPh.start();
   FeatureGrid<T> F;
   feature_grid<T>(hmn, hmx, nd, n, K1, F);
-  uint32_t *cnt = P.template buf<uint32_t>(S_FCNT, (size_t)F.cells);
-  fill_u32(cnt, 0u, (size_t)F.cells, s);
-  feature_counts<T>(P.x, n, F.g, cnt, s);
+  const uint32_t reps = feature_reps(F.cells);
+  uint32_t *cnt = P.template buf<uint32_t>(S_FCNT, (size_t)F.cells * reps);
+  fill_u32(cnt, 0u, (size_t)F.cells * reps, s);
+  feature_counts<T>(P.x, n, F.g, cnt, F.cells, reps, s);
   feature_occupancy(cnt, F.cells, n, F.logdens, s);
   float *Wd = P.template buf<float>(S_MODEL, MLP_NW);
   HIP_CHECK(hipMemcpyAsync(Wd, M.w.data(), sizeof(float) * MLP_NW, hipMemcpyHostToDevice, s));
@@ -2084,9 +2097,10 @@ int knnjoin_radius_features(const float *points, uint32_t n, uint32_t ndim, uint
     HIP_CHECK(hipStreamSynchronize(s));
     FeatureGrid<float> F;
     feature_grid<float>(hmn, hmx, nd, n, k + 1, F);
-    uint32_t *cnt = P.buf<uint32_t>(S_FCNT, (size_t)F.cells);
-    fill_u32(cnt, 0u, (size_t)F.cells, s);
-    feature_counts<float>(P.x, n, F.g, cnt, s);
+    const uint32_t reps = feature_reps(F.cells);
+    uint32_t *cnt = P.buf<uint32_t>(S_FCNT, (size_t)F.cells * reps);
+    fill_u32(cnt, 0u, (size_t)F.cells * reps, s);
+    feature_counts<float>(P.x, n, F.g, cnt, F.cells, reps, s);
     feature_occupancy(cnt, F.cells, n, F.logdens, s);
     float *fd = features ? P.buf<float>(S_TRAIN, (size_t)n * MLP_IN) : nullptr;
     float *rd = nullptr;

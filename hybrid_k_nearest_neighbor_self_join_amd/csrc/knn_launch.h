@@ -200,10 +200,12 @@ size_t range_heap_scratch_bytes(uint32_t ntiles, uint32_t k1);
 template <typename T>
 void row_sums(const T *dists, uint32_t n, uint32_t k1, double *rowSum, hipStream_t s);
 // ------------------------------------------------------------- learned radius (learned.hip)
-// per-cell point counts of the feature grid (cnt zeroed by the caller, totalCells entries)
+// per-cell point counts of the feature grid: cnt holds reps replica grids of `cells` entries,
+// zeroed by the caller; blocks count into the replica of their index mod reps (a clustered set's
+// densest cells then take reps shorter chains of atomics), summed into the first grid
 template <typename T>
 void feature_counts(const T *x, uint32_t n, const typename GridOf<T>::type &g, uint32_t *cnt,
-                    hipStream_t s);
+                    uint64_t cells, uint32_t reps, hipStream_t s);
 // features (featOut, n x 12, optional) and radius = wf * exp(mlp + margin) (radius, optional),
 // with a 64-bin histogram of 4*log2(radius/wf)+32 (rhist, optional, zeroed by the caller)
 template <typename T>

@@ -101,13 +101,23 @@ __device__ __forceinline__ float mlp_forward_t(const float *W, const float *f) {
 }
 
 template <typename T, typename G>
-__global__ void k_feature_counts(const T *__restrict__ x, uint32_t n, G g, uint32_t *cnt) {
+__global__ void k_feature_counts(const T *__restrict__ x, uint32_t n, G g, uint32_t *cnt, uint64_t cells,
+                                 uint32_t reps) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   uint64_t id = 0;
   for (uint32_t j = 0; j < g.nd; j++)
     id += (uint64_t)cell_coord<T>(x[(size_t)j * n + i], (T)g.minArr[j], (T)g.eps, g.nCells[j]) * g.mult[j];
-  atomicAdd(&cnt[id], 1u);
+  atomicAdd(&cnt[(uint64_t)(blockIdx.x % reps) * cells + id], 1u);
+}
+
+// cnt[c] = the sum of the reps replica grids' counts of cell c (integers: exact in any order)
+__global__ void k_sum_reps(uint32_t *cnt, uint64_t cells, uint32_t reps) {
+  const uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= cells) return;
+  uint32_t t = cnt[c];
+  for (uint32_t r = 1; r < reps; r++) t += cnt[(uint64_t)r * cells + c];
+  cnt[c] = t;
 }
 
 template <typename T, typename G>
@@ -466,8 +476,11 @@ __global__ void k_adam(float *W, const float *G, float *M, float *V, float lr, f
 
 template <typename T>
 void feature_counts(const T *x, uint32_t n, const typename GridOf<T>::type &g, uint32_t *cnt,
-                    hipStream_t s) {
-  if (n) k_feature_counts<T, typename GridOf<T>::type><<<(n + 255) / 256, 256, 0, s>>>(x, n, g, cnt);
+                    uint64_t cells, uint32_t reps, hipStream_t s) {
+  if (!n || !cells) return;
+  if (reps < 1) reps = 1;
+  k_feature_counts<T, typename GridOf<T>::type><<<(n + 255) / 256, 256, 0, s>>>(x, n, g, cnt, cells, reps);
+  if (reps > 1) k_sum_reps<<<(uint32_t)((cells + 255) / 256), 256, 0, s>>>(cnt, cells, reps);
 }
 
 template <typename T>
@@ -547,7 +560,7 @@ void adam_step(float *W, const float *G, float *M, float *V, float lr, float b1,
 
 #define INST(T)                                                                                   \
   template void feature_counts<T>(const T *, uint32_t, const GridOf<T>::type &, uint32_t *,        \
-                                  hipStream_t);                                                   \
+                                  uint64_t, uint32_t, hipStream_t);                               \
   template void radius_kernel<T>(const T *, uint32_t, const GridOf<T>::type &, const uint32_t *,   \
                                  float, float, const float *, float, float, T *, float *,         \
                                  uint32_t *, hipStream_t);                                        \
