@@ -212,32 +212,36 @@ typedef _Float16 half4_t __attribute__((ext_vector_type(4)));
 typedef float float4_t __attribute__((ext_vector_type(4)));
 
 template <typename T, typename G>
-__global__ __launch_bounds__(256) void k_radius_mfma(const T *__restrict__ x, uint32_t n, G g,
+__global__ __launch_bounds__(256, 4) void k_radius_mfma(const T *__restrict__ x, uint32_t n, G g,
                                                      const uint32_t *__restrict__ cnt, float logk1,
                                                      float logdens, const float *__restrict__ Wg,
                                                      float margin, float wf, T *radius, uint32_t *rhist) {
   __shared__ _Float16 fs[4][64][16];
   __shared__ uint32_t lh[64];
+  // the hidden biases and the output weights (64 each), read from LDS where they are used: in
+  // registers they took 48 VGPRs a lane and held the kernel to 3 waves per SIMD
+  __shared__ __align__(16) float sb1[64], sb2[64], sw3[64];
   const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const uint32_t r16 = lane & 15, kq = lane >> 4;
   const float *W1 = Wg, *b1 = Wg + 768, *W2 = Wg + 832, *b2 = Wg + 4928, *w3 = Wg + 4992;
   half4_t a1[4], a2[4][4];
-  float b1v[4][4], b2v[4][4], w3v[4][4];
 #pragma unroll
   for (int mt = 0; mt < 4; mt++) {
 #pragma unroll
     for (int i = 0; i < 4; i++) {
       const uint32_t k = 4 * kq + i;
       a1[mt][i] = (_Float16)(k < (uint32_t)MLP_IN ? W1[(16 * mt + r16) * MLP_IN + k] : 0.0f);
-      b1v[mt][i] = b1[16 * mt + 4 * kq + i];
-      b2v[mt][i] = b2[16 * mt + 4 * kq + i];
-      w3v[mt][i] = w3[16 * mt + 4 * kq + i];
 #pragma unroll
       for (int t = 0; t < 4; t++) a2[mt][t][i] = (_Float16)W2[(16 * mt + r16) * MLP_H + 16 * t + 4 * kq + i];
     }
   }
   const float b3 = Wg[5056];
-  if (threadIdx.x < 64) lh[threadIdx.x] = 0;
+  if (threadIdx.x < 64) {
+    lh[threadIdx.x] = 0;
+    sb1[threadIdx.x] = b1[threadIdx.x];
+    sb2[threadIdx.x] = b2[threadIdx.x];
+    sw3[threadIdx.x] = w3[threadIdx.x];
+  }
   __syncthreads();
   const uint64_t stride = (uint64_t)gridDim.x * 256;
   for (uint64_t base = (uint64_t)blockIdx.x * 256 + wv * 64; base < n; base += stride) {
@@ -266,7 +270,7 @@ __global__ __launch_bounds__(256) void k_radius_mfma(const T *__restrict__ x, ui
         float4_t acc = {0.0f, 0.0f, 0.0f, 0.0f};
         acc = __builtin_amdgcn_mfma_f32_16x16x16f16(a1[mt], bx, acc, 0, 0, 0);
 #pragma unroll
-        for (int i = 0; i < 4; i++) h1[mt][i] = (_Float16)relu(acc[i] + b1v[mt][i]);
+        for (int i = 0; i < 4; i++) h1[mt][i] = (_Float16)relu(acc[i] + sb1[16 * mt + 4 * kq + i]);
       }
       float part = 0.0f;
 #pragma unroll
@@ -275,7 +279,7 @@ __global__ __launch_bounds__(256) void k_radius_mfma(const T *__restrict__ x, ui
 #pragma unroll
         for (int t = 0; t < 4; t++) acc = __builtin_amdgcn_mfma_f32_16x16x16f16(a2[mt][t], h1[t], acc, 0, 0, 0);
 #pragma unroll
-        for (int i = 0; i < 4; i++) part = fmaf(w3v[mt][i], relu(acc[i] + b2v[mt][i]), part);
+        for (int i = 0; i < 4; i++) part = fmaf(sw3[16 * mt + 4 * kq + i], relu(acc[i] + sb2[16 * mt + 4 * kq + i]), part);
       }
       part += __shfl_xor(part, 16, 64);
       part += __shfl_xor(part, 32, 64);
