@@ -53,7 +53,7 @@ enum Slot {
   S_PTS, S_X, S_XS, S_KEYS, S_KEYS_TMP, S_IDS, S_IDS_TMP, S_CELLKEYS, S_CELLSTART, S_FLAGS, S_POS,
   S_ACT, S_NTILES, S_TILEOFF, S_TILES, S_FAILED, S_ROWSUM, S_OUTIDS, S_OUTDISTS, S_SCRATCH,
   S_HEAP, S_SMALL, S_PART, S_CTR, S_CSUM, S_HOSTFLAGS, S_OVERFLOW, S_STAGE, S_DENSE, S_BFLAGS,
-  S_BPOS, S_BSTART, S_CELLOF, S_CELLMAX, S_FCNT, S_RADIUS, S_RHIST, S_MODEL, S_TRAIN, S_OCC, S_ACTQ, S_BRUTE_PK, S_BRUTE_PI, S_BLIST, S_TILEMETA, S_HCPU, S_HGPU, S_HLIST, S_HROWI, S_HROWD, S_XS2, S_SIDS, S_SIDS2, S_BKEY, S_PARTNER, S_NSLOTS
+  S_BPOS, S_BSTART, S_CELLOF, S_CELLMAX, S_FCNT, S_RADIUS, S_RHIST, S_MODEL, S_TRAIN, S_OCC, S_ACTQ, S_BRUTE_PK, S_BRUTE_PI, S_BLIST, S_TILEMETA, S_HCPU, S_HGPU, S_HLIST, S_HROWI, S_HROWD, S_XS2, S_SIDS, S_SIDS2, S_BKEY, S_PARTNER, S_ABITS, S_NSLOTS
 };
 
 // Slots of one no-allocation call: regions of the static device scratch array handed out in
@@ -723,7 +723,12 @@ void Pipeline<T>::make_tiles(const uint8_t *activeFailed, uint32_t *&act, uint4 
     // own-block pre-check (exact early failures) + per-cell candidate accounting
     uint32_t *cellMax = buf<uint32_t>(S_CELLMAX, (size_t)ncell + 1);
     fill_u32(cellMax, 0u, ncell, s);
-    prefilter<T>(xs, n, g, radius, uniform_radius(), denseStart, sortedIds, activeFailed, failed, K1, cellOf, cellMax,
+    uint64_t *activeBits = nullptr;
+    if (activeFailed) {
+      activeBits = buf<uint64_t>(S_ABITS, ((size_t)n + 63) / 64);
+      pack_bits(activeFailed, n, activeBits, s);
+    }
+    prefilter<T>(xs, n, g, radius, uniform_radius(), denseStart, sortedIds, activeBits, failed, K1, cellOf, cellMax,
                  flags, s);
     if (cellAcc) cell_bytes(cellMax, ncell, cellAcc, s);
     all = false;
@@ -1690,7 +1695,7 @@ static void *noalloc_scratch(size_t &bytes);
 // laid out): two coordinate copies, 64-bit keys and ids with their ping-pong copies, flags,
 // scans, cell and block tables, the active list and tiles, failure flags, radii, row sums.
 static size_t noalloc_bound(uint32_t n, uint32_t d, size_t tsz) {
-  const size_t perPoint = 2 * d * tsz + tsz + 16 + 8 + 8 + 8 + 4 + 4 + 4 + 16 + 3 + 8 + 24 + 4 + 12;
+  const size_t perPoint = 2 * d * tsz + tsz + 16 + 8 + 8 + 8 + 4 + 4 + 4 + 16 + 3 + 8 + 24 + 4 + 12 + 1;
   return (size_t)n * perPoint + (radix_scratch_words(n) + scan_scratch_words(n + 1) + 2048) * 4 +
          avgdist_scratch_doubles(n) * 8 + (2u << 20);
 }

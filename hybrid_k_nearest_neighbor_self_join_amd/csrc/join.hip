@@ -86,21 +86,31 @@ __device__ __forceinline__ uint64_t block_key(uint64_t id, uint32_t ncell0, uint
   return (id / ncell0) * nb0 + (id % ncell0) / span;
 }
 
+// bits[w] bit l = (src[64 w + l] != 0), for the ids below n (the last word's bits past n are 0)
+__global__ void k_pack_bits(const uint8_t *__restrict__ src, uint32_t n, uint64_t *__restrict__ bits) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t b = __ballot(i < n && src[i] != 0);
+  if ((threadIdx.x & 63u) == 0 && i < n) bits[i >> 6] = b;
+}
+
 // Per active query (sorted position p): the number of points in its own cell block (radius
 // r = its own, or rAll for every query, cells [coord(x-r) rounded down, coord(x+r) rounded up]) from
 // the dense cell-start table.  Fewer than K+1 points there means the query cannot succeed
 // at this radius: it is marked failed without a distance evaluation.  The per-cell maximum of
 // those counts feeds the algorithmic-bytes accounting (each query cell's block once).
+// The active flags come as a bitmap by point id (bit q & 63 of word q >> 6): its 1/8 of a byte
+// per point stays in the L2 for these reads in sorted order, where a byte per point (10 MB at
+// 10M points) made each read a line from beyond it.
 template <typename T, typename G, int ND>
 __global__ void k_prefilter(const T *__restrict__ xs, uint32_t n, G g, const T *radius, T rAll,
                             const uint32_t *__restrict__ dense, const uint32_t *__restrict__ sortedIds,
-                            const uint8_t *activeSrc, uint8_t *failed, uint32_t k1,
+                            const uint64_t *activeBits, uint8_t *failed, uint32_t k1,
                             const uint32_t *cellOf, uint32_t *cellMax, uint32_t *flags) {
   const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t lane = threadIdx.x & 63;
   const bool inRange = p < n;
   const uint32_t q = inRange ? sortedIds[p] : 0u;
-  const bool active = inRange && !(activeSrc && !activeSrc[q]);
+  const bool active = inRange && !(activeBits && !((activeBits[q >> 6] >> (q & 63u)) & 1ull));
   uint32_t cnt = 0;
   if (active) {
     const T r = radius ? radius[q] : rAll;
@@ -481,20 +491,23 @@ void dense_cell_counts(const uint64_t *cellKeys, const uint32_t *cellStart, uint
                        uint32_t *dense, hipStream_t s) {
   if (ncell) k_dense_counts<<<(ncell + 255) / 256, 256, 0, s>>>(cellKeys, cellStart, ncell, dense);
 }
+void pack_bits(const uint8_t *src, uint32_t n, uint64_t *bits, hipStream_t s) {
+  if (n) k_pack_bits<<<(n + 255) / 256, 256, 0, s>>>(src, n, bits);
+}
 template <typename T>
 void prefilter(const T *xs, uint32_t n, const typename GridOf<T>::type &g, const T *radius,
-               T rAll, const uint32_t *dense, const uint32_t *sortedIds, const uint8_t *activeSrc,
+               T rAll, const uint32_t *dense, const uint32_t *sortedIds, const uint64_t *activeBits,
                uint8_t *failed, uint32_t k1, const uint32_t *cellOf, uint32_t *cellMax,
                uint32_t *flags, hipStream_t s) {
   if (!n) return;
   using G = typename GridOf<T>::type;
   const dim3 grid((n + 255) / 256), block(256);
   switch (g.nd) {
-    case 1: k_prefilter<T, G, 1><<<grid, block, 0, s>>>(xs, n, g, radius, rAll, dense, sortedIds, activeSrc, failed, k1, cellOf, cellMax, flags); break;
-    case 2: k_prefilter<T, G, 2><<<grid, block, 0, s>>>(xs, n, g, radius, rAll, dense, sortedIds, activeSrc, failed, k1, cellOf, cellMax, flags); break;
-    case 3: k_prefilter<T, G, 3><<<grid, block, 0, s>>>(xs, n, g, radius, rAll, dense, sortedIds, activeSrc, failed, k1, cellOf, cellMax, flags); break;
-    case 4: k_prefilter<T, G, 4><<<grid, block, 0, s>>>(xs, n, g, radius, rAll, dense, sortedIds, activeSrc, failed, k1, cellOf, cellMax, flags); break;
-    default: k_prefilter<T, G, 0><<<grid, block, 0, s>>>(xs, n, g, radius, rAll, dense, sortedIds, activeSrc, failed, k1, cellOf, cellMax, flags); break;
+    case 1: k_prefilter<T, G, 1><<<grid, block, 0, s>>>(xs, n, g, radius, rAll, dense, sortedIds, activeBits, failed, k1, cellOf, cellMax, flags); break;
+    case 2: k_prefilter<T, G, 2><<<grid, block, 0, s>>>(xs, n, g, radius, rAll, dense, sortedIds, activeBits, failed, k1, cellOf, cellMax, flags); break;
+    case 3: k_prefilter<T, G, 3><<<grid, block, 0, s>>>(xs, n, g, radius, rAll, dense, sortedIds, activeBits, failed, k1, cellOf, cellMax, flags); break;
+    case 4: k_prefilter<T, G, 4><<<grid, block, 0, s>>>(xs, n, g, radius, rAll, dense, sortedIds, activeBits, failed, k1, cellOf, cellMax, flags); break;
+    default: k_prefilter<T, G, 0><<<grid, block, 0, s>>>(xs, n, g, radius, rAll, dense, sortedIds, activeBits, failed, k1, cellOf, cellMax, flags); break;
   }
 }
 void cell_bytes(const uint32_t *cellMax, uint32_t ncell, unsigned long long *out, hipStream_t s) {
@@ -570,7 +583,7 @@ void sum_rows(const double *rowSum, uint32_t n, double *partial, double *out, hi
                                hipStream_t);                                                   \
   template void row_sums<T>(const T *, uint32_t, uint32_t, double *, hipStream_t);          \
   template void prefilter<T>(const T *, uint32_t, const GridOf<T>::type &, const T *, T,      \
-                             const uint32_t *, const uint32_t *, const uint8_t *, uint8_t *,  \
+                             const uint32_t *, const uint32_t *, const uint64_t *, uint8_t *, \
                              uint32_t, const uint32_t *, uint32_t *, uint32_t *, hipStream_t);
 INST(float)
 INST(double)

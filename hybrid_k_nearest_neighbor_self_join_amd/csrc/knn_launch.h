@@ -97,11 +97,13 @@ void dense_cell_counts(const uint64_t *cellKeys, const uint32_t *cellStart, uint
                        uint32_t *dense, hipStream_t s);
 
 // Active flags per sorted position with the own-block pre-check: queries whose own cell
-// block holds fewer than k1 points get failed[q] = 1 and flag 0.  activeSrc null = all
-// active.  cellOf/cellMax (optional) collect per-cell candidate counts for accounting.
+// block holds fewer than k1 points get failed[q] = 1 and flag 0.  activeBits (pack_bits of the
+// per-id active bytes) null = all active.  cellOf/cellMax (optional) collect per-cell
+// candidate counts for accounting.
+void pack_bits(const uint8_t *src, uint32_t n, uint64_t *bits, hipStream_t s);
 template <typename T>
 void prefilter(const T *xs, uint32_t n, const typename GridOf<T>::type &g, const T *radius,
-               T rAll, const uint32_t *dense, const uint32_t *sortedIds, const uint8_t *activeSrc,
+               T rAll, const uint32_t *dense, const uint32_t *sortedIds, const uint64_t *activeBits,
                uint8_t *failed, uint32_t k1, const uint32_t *cellOf, uint32_t *cellMax,
                uint32_t *flags, hipStream_t s);
 void cell_bytes(const uint32_t *cellMax, uint32_t ncell, unsigned long long *out, hipStream_t s);
