@@ -291,7 +291,8 @@ __device__ __forceinline__ uint32_t lower_bound_keys(const uint64_t *keys, uint3
 // per tile block b < *nblocks (0 for b up to ncell): tiles of <= 64 active queries.  With row
 // pairs (shape[1] = 2) a block of an even dim-1 coordinate takes its partner's queries (the
 // block one row up, same window; partner[b]), and a block whose partner one row down exists
-// has no tiles of its own.  Block keys increase with b, so each partner is one binary search.
+// has no tiles of its own.  Block keys are distinct and increase with b, so the partner of key
+// +- nw (nw = windows per row) lies within nw places of b: one binary search over those.
 __global__ void k_pair_tile_counts(const uint32_t *blockStart, const uint64_t *blockKey, const uint32_t *pos,
                                    const uint32_t *nblocks, uint32_t ncell, uint32_t ncell0, uint32_t ncell1,
                                    const uint32_t *shape, uint32_t *partner, uint32_t *ntiles) {
@@ -310,14 +311,16 @@ __global__ void k_pair_tile_counts(const uint32_t *blockStart, const uint64_t *b
     const uint32_t r1 = (uint32_t)((key / nw) % ncell1);       // the block's dim-1 coordinate
     if ((r1 & 1u) == 0u) {
       if (r1 + 1u < ncell1) {
-        const uint32_t j = lower_bound_keys(blockKey, b + 1, nb, key + nw);
+        const uint32_t hiB = (uint64_t)nb - b - 1 > nw ? b + 1 + (uint32_t)nw : nb;
+        const uint32_t j = lower_bound_keys(blockKey, b + 1, hiB, key + nw);
         if (j < nb && blockKey[j] == key + nw) {
           pj = j;
           cnt += pos[blockStart[j + 1]] - pos[blockStart[j]];
         }
       }
     } else {
-      const uint32_t j = lower_bound_keys(blockKey, 0, b, key - nw);
+      const uint32_t loB = (uint64_t)b > nw ? b - (uint32_t)nw : 0u;
+      const uint32_t j = lower_bound_keys(blockKey, loB, b, key - nw);
       if (j < b && blockKey[j] == key - nw) cnt = 0;
     }
   }
