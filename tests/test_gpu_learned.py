@@ -46,8 +46,13 @@ def test_mlp_training_matches_oracle(oracle):
     assert losses[-1] < losses[0]
 
 
-@pytest.mark.parametrize("gen,nd,k", [("mixed", 2, 64), ("uniform", 2, 8), ("cube4", 4, 32), ("skew6", 4, 16)])
-def test_features_match_oracle(oracle, gen, nd, k):
+@pytest.mark.parametrize("gen,nd,k,reps", [("mixed", 2, 64, None), ("mixed", 2, 64, "1"), ("mixed", 2, 64, "3"),
+                                            ("uniform", 2, 8, None), ("cube4", 4, 32, None), ("skew6", 4, 16, None)])
+def test_features_match_oracle(oracle, monkeypatch, gen, nd, k, reps):
+    # reps: the point counts over that many replica grids (KNN_FEATURE_REPS; None = the
+    # library's choice, 16 at these sizes), summed before any feature reads them
+    if reps is not None:
+        monkeypatch.setenv("KNN_FEATURE_REPS", reps)
     pts = {"mixed": lambda: synthetic.mixed_box(200000, seed=4),
            "uniform": lambda: synthetic.uniform_box(100000, seed=5),
            "cube4": lambda: synthetic.unit_cube(60000, 4, seed=6),

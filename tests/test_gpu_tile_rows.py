@@ -2,7 +2,8 @@
 queries from two runs of the active list (the block of an even row and the block one row up).
 The join equals the one with single-row blocks (KNN_TILE_ROWS=1) and the oracle's, bit for bit,
 on uniform, clustered and lattice data, in both schedules; the verbose pass lines show that
-the row pairs were chosen."""
+the row pairs were chosen.  The cost model's density basis (per cell holding active queries,
+or per non-empty cell) changes the tiles and not the rows."""
 import ctypes
 import re
 
@@ -61,5 +62,17 @@ def test_row_pairs_learned_schedule_full_size(monkeypatch, capfd):
     monkeypatch.setenv("KNN_TILE_ROWS", "1")
     ids1, dist1, _ = _lib.knnjoin_ex(pts, 64, mode=_lib.KNN_MODE_LEARNED, verbose=1)
     assert set(_pass_rows(capfd)) == {1}
+    assert np.array_equal(ids, ids1)
+    assert np.array_equal(dist.view(np.uint32), dist1.view(np.uint32))
+
+
+@pytest.mark.parametrize("gen,k", [("mixed", 32), ("uniform", 16)])
+def test_density_basis_gives_same_rows(monkeypatch, gen, k):
+    # the tile-block cost model priced per cell holding active queries (default) or per
+    # non-empty cell (KNN_SPAN_DENSITY=cells): other tiles, the same rows
+    pts = _points(gen, 500_000, seed=k + 1)
+    ids, dist, _ = _lib.knnjoin_ex(pts, k, beta=0.025)
+    monkeypatch.setenv("KNN_SPAN_DENSITY", "cells")
+    ids1, dist1, _ = _lib.knnjoin_ex(pts, k, beta=0.025)
     assert np.array_equal(ids, ids1)
     assert np.array_equal(dist.view(np.uint32), dist1.view(np.uint32))
