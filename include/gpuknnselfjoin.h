@@ -142,8 +142,8 @@ int knnjoin_ex(const void *points, int dtype, uint32_t n, uint32_t ndim, uint32_
  * params->device_ptrs = 1 required (points and outputs in device memory, the caller's).  Every
  * working buffer lives in one static device array of KNN_NOALLOC_SCRATCH_BYTES (256 MiB),
  * which each device the library runs on holds until the process ends.  Limits: K+1 <= 85 and
- * N * (8 NDIM + 116) bytes (float; 16 NDIM + 120 for double) plus about 3 MiB must fit in the
- * array (about 2M points of 2 dims); larger inputs return KNN_ERR_ARG / KNN_ERR_NOMEM before
+ * N * (8 NDIM + 129) bytes (float; 16 NDIM + 133 for double) plus about 3 MiB must fit in the
+ * array (about 1.8M points of 2 dims); larger inputs return KNN_ERR_ARG / KNN_ERR_NOMEM before
  * any work.  The dense cell-start table and the learned schedule's feature grid are laid out
  * in the same array when they fit (without the dense table the join searches the cell table
  * instead).  The first call on a device looks up the array's address and creates the
