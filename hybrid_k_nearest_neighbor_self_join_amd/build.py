@@ -31,8 +31,10 @@ CLI = os.path.join(PKG, "knnjoin_main")
 SOURCES = ["prims.hip", "estimator.hip", "join.hip", "range.hip", "learned.hip", "batched.hip",
            "outlier.hip", "brute.hip", "hybrid.hip", "api.hip"]
 # select.hip is compiled once per part (float / double x three ranges of the dimensionality),
-# so its many kernel instantiations build in parallel
+# so its many kernel instantiations build in parallel, and once per bucket count (the learned
+# schedule's and the other passes', KNN_SELECT_BUCKETS_LEARNED / _OTHER in knn_launch.h)
 SELECT_PARTS = 6
+SELECT_BUCKETS = (40, 56)
 ARCH = os.environ.get("KNN_OFFLOAD_ARCH", "gfx950")
 
 HIP_FLAGS = [
@@ -80,9 +82,10 @@ def build(verbose: bool = False, dtype: str = "both") -> str:
     srcs = [s for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
     common = [s for s in srcs if s != "api.hip"]
     jobs = [(s, os.path.join(BUILD, os.path.splitext(s)[0] + ".o"), []) for s in common]
-    jobs += [("select.hip", os.path.join(BUILD, f"select_{p}.o"), [f"-DKNN_SELECT_PART={p}"])
-             for p in range(SELECT_PARTS)]
-    common += [f"select_{p}" for p in range(SELECT_PARTS)]
+    jobs += [("select.hip", os.path.join(BUILD, f"select_{p}_b{nb}.o"),
+              [f"-DKNN_SELECT_PART={p}", f"-DKNN_SELECT_BUCKETS={nb}"])
+             for nb in SELECT_BUCKETS for p in range(SELECT_PARTS)]
+    common += [f"select_{p}_b{nb}" for nb in SELECT_BUCKETS for p in range(SELECT_PARTS)]
     if dtype in ("float", "both"):
         jobs.append(("api.hip", os.path.join(BUILD, "api.o"), []))
     if dtype in ("double", "both"):

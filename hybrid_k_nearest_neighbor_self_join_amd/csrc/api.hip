@@ -486,6 +486,7 @@ struct Pipeline {
   T radiusAll = (T)0;         // one radius for every query (plain schedule); 0 = the grid's eps
   uint32_t lastSpan = 1;
   uint32_t lastRows = 1;
+  uint32_t selBuckets = (uint32_t)SEL_NB_OTHER;  // the select kernel's bucket count for the passes
   uint32_t lastActive = 0;   // queries left in tiles after the pre-check
   double avgReach = 1.0;     // typical search radius in cells (tile-span cost model)
   // persistent range-kernel workgroups: 11 per CU on 256 CUs (the select kernel's LDS,
@@ -813,8 +814,9 @@ PassResult Pipeline<T>::pass(const uint8_t *activeFailed, uint32_t nq, uint8_t *
   a.segCapLog2 = seg_cap_log2();
   // LDS staging slots per query: K+1 and 64 more (the items of the K+1-th neighbour's
   // bucket beyond K+1), at most 512, within the 64 KB a workgroup may allocate
+  a.selBuckets = selBuckets;
   a.smax = std::min<uint32_t>(K1 + 64, 512u);
-  while (a.smax > K1 && range_select_lds_bytes<T>(a.smax) > 65536) a.smax -= 16;
+  while (a.smax > K1 && range_select_lds_bytes<T>(a.smax, selBuckets) > 65536) a.smax -= 16;
   a.overflow = overflow;
   a.overflowCount = overflowCount;
   PassResult r{};
@@ -822,7 +824,7 @@ PassResult Pipeline<T>::pass(const uint8_t *activeFailed, uint32_t nq, uint8_t *
   tK.start();
   if (use_collect(nd, K1)) {
     range_collect<T>(a, g, s);
-  } else if (!(a.smax > K1 && range_select_lds_bytes<T>(a.smax) <= 65536)) {
+  } else if (!(a.smax > K1 && range_select_lds_bytes<T>(a.smax, selBuckets) <= 65536)) {
     // K+1 too large for LDS staging: the collect (or heap) kernel takes every query of the pass
     if (overflow_by_collect(K1)) {
       range_collect<T>(a, g, s);
@@ -833,6 +835,7 @@ PassResult Pipeline<T>::pass(const uint8_t *activeFailed, uint32_t nq, uint8_t *
       range_topk<T>(a, g, s);
     }
   } else {
+    a.ntiles = sel_pass_grid(selBuckets);  // persistent grid: the workgroups its LDS admits
     range_select<T>(a, g, s);
     selectK = true;
   }
@@ -1508,6 +1511,7 @@ static void run_learned(Pipeline<T> &P, const knnjoin_params &p, knnjoin_stats &
   const uint32_t n = P.n, nd = P.nd, K1 = P.K1;
   const RadiusModel &M = radius_model(P.d, nd);
   if (!M.ok) throw Error(KNN_ERR_MODEL, "no learned-radius model registered for this number of indexed dims");
+  P.selBuckets = (uint32_t)SEL_NB_LEARNED;
   // min/max of the indexed dims
   T hmn[KNN_MAX_INDEXED], hmx[KNN_MAX_INDEXED];
   P.index_minmax(hmn, hmx);

@@ -8,6 +8,27 @@
 
 namespace knn {
 
+// Select-kernel bucket counts (knn_range.h, select.hip).
+// The select kernel is built for two bucket counts (select.hip is compiled once per count): fewer
+// buckets take less LDS, so more workgroups per CU, but stage and rank more items per query.
+// The learned schedule's passes run with SEL_NB_LEARNED (40: 14 workgroups per CU), every other
+// pass with SEL_NB_OTHER (56: 12 per CU).  Measured on one MI355X, 10M mixed 2-D points (ms per
+// join, learned K=16 / K=64 / K=128 / plain K=64): 64 buckets 15.1 / 19.4 / 29.4 / 31.7, 56
+// 14.6 / 18.6 / 27.9 / 30.4, 48 14.6 / 18.2 / 27.3 / 31.9, 40 14.1 / 17.5 / 26.9 / 35.3, 32
+// 13.8 / 17.8 / 26.2 / 35.6.
+#ifndef KNN_SELECT_BUCKETS_LEARNED
+#define KNN_SELECT_BUCKETS_LEARNED 40
+#endif
+#ifndef KNN_SELECT_BUCKETS_OTHER
+#define KNN_SELECT_BUCKETS_OTHER 56
+#endif
+constexpr int SEL_NB_LEARNED = KNN_SELECT_BUCKETS_LEARNED;
+constexpr int SEL_NB_OTHER = KNN_SELECT_BUCKETS_OTHER;
+// persistent grid of the select kernel with nb buckets: as many one-wave workgroups per CU (256
+// CUs) as its LDS admits at K = 64 (11 at 64 buckets, 12 at 56, 14 at 40)
+constexpr uint32_t sel_pass_grid(uint32_t nb) { return nb <= 40 ? 3584u : (nb <= 48 ? 3328u : (nb <= 56 ? 3072u : 2816u)); }
+
+
 // ---------------------------------------------------------------- primitives (prims.hip)
 // Exclusive scan of n u32 values into out[0..n); out[n] = total.  `scratch` must hold
 // scan_scratch_words(n) u32 words.  in may alias out.
@@ -168,6 +189,7 @@ struct JoinArgs {
   uint32_t *tileCounter; // TILE_QUEUES dynamic tile queue heads (zeroed before the launch)
   uint32_t tileRanges;   // 1: one queue over all tiles; TILE_QUEUES: one range per XCD
   uint32_t segCapLog2;   // positions per chunk of the segment-start bitmap, log2 (6..11; 11 by default)
+  uint32_t selBuckets;   // select kernel: SEL_NB_LEARNED for the learned schedule's passes, else SEL_NB_OTHER (0 = other)
   void *heapScratch;     // heap kernel: global heap storage when K+1 does not fit in LDS
   uint32_t smax;         // select kernel: LDS staging slots per query
   uint8_t *overflow;     // select kernel: per point, 1 = needs the overflow (collect or heap) kernel
@@ -183,7 +205,7 @@ struct JoinArgs {
 template <typename T>
 void range_select(const JoinArgs &a, const typename GridOf<T>::type &g, hipStream_t s);
 template <typename T>
-size_t range_select_lds_bytes(uint32_t smax);
+size_t range_select_lds_bytes(uint32_t smax, uint32_t nb);
 uint32_t range_select_grid(uint32_t ntiles);
 // Collect kernel (wave = query, one pass over its block into an LDS top-(K+1) collector): for
 // queries with many candidates per neighbour.  K+1 <= BRUTE_MAX_K1.

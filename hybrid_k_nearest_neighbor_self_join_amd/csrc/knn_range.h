@@ -24,7 +24,7 @@ namespace knn {
 // padding against bank conflicts).  Sub-blocks of at most SEL_RMAX rows have their row ranges
 // looked up for the whole tile at once.
 #ifndef KNN_SELECT_BUCKETS
-#define KNN_SELECT_BUCKETS 64
+#define KNN_SELECT_BUCKETS KNN_SELECT_BUCKETS_OTHER
 #endif
 #ifndef KNN_SELECT_RMAX
 #define KNN_SELECT_RMAX 5
@@ -34,16 +34,21 @@ static_assert(SEL_NB % 2 == 0 && SEL_NB >= 8 && SEL_NB <= 64, "select buckets: e
 constexpr int SEL_NBIN = SEL_NB - 1;
 constexpr uint32_t SEL_RMAX = KNN_SELECT_RMAX;
 constexpr uint32_t SEL_RS = SEL_NB / 2 + 1;
-constexpr uint32_t SEL_HWORDS = (uint32_t)SEL_NB * 32 > 64 * SEL_RS ? (uint32_t)SEL_NB * 32 : 64 * SEL_RS;
-// LDS words of the select kernel before its staging arrays: H, rowB/rowX[64],
-// prB/prX[SEL_RMAX][64], prT/prR[64], the collecting query's 32-bit cursors qc[64]
-constexpr uint32_t SEL_FIXED_WORDS = SEL_HWORDS + 128 + SEL_RMAX * 128 + 128 + 64;
+// LDS words of the select kernel with nb buckets before its staging arrays: H (the histogram's
+// nb x 32 words, or 64 rows of nb / 2 + 1 cursor words), rowB/rowX[64], prB/prX[SEL_RMAX][64],
+// prT/prR[64], the collecting query's 32-bit cursors qc[64]
+constexpr uint32_t sel_hwords(uint32_t nb) { return nb * 32 > 64 * (nb / 2 + 1) ? nb * 32 : 64 * (nb / 2 + 1); }
+constexpr uint32_t sel_fixed_words(uint32_t nb) { return sel_hwords(nb) + 128 + SEL_RMAX * 128 + 128 + 64; }
+constexpr uint32_t SEL_HWORDS = sel_hwords(SEL_NB);
+constexpr uint32_t SEL_FIXED_WORDS = sel_fixed_words(SEL_NB);
+
 // persistent grid of the select kernel when the tile count is known on the host: 11 one-wave
 // workgroups per CU on 256 CUs
 constexpr uint32_t SELECT_MAX_GRID = 2816;
 
-// Select kernel launch for D data dims and nd indexed dims (select.hip).
-template <typename T, int D>
+// Select kernel launch for D data dims and nd indexed dims, built with NB buckets (select.hip,
+// compiled once per bucket count).
+template <typename T, int D, int NB>
 void select_launch(const JoinArgs &a, const typename GridOf<T>::type &g, uint32_t nd, hipStream_t s);
 
 namespace {

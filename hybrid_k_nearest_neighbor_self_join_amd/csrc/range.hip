@@ -356,8 +356,8 @@ uint32_t range_select_grid(uint32_t ntiles) {
 }
 
 template <typename T>
-size_t range_select_lds_bytes(uint32_t smax) {
-  return (size_t)SEL_FIXED_WORDS * sizeof(uint32_t) + (size_t)smax * 8 +
+size_t range_select_lds_bytes(uint32_t smax, uint32_t nb) {
+  return (size_t)sel_fixed_words(nb) * sizeof(uint32_t) + (size_t)smax * 8 +
          (size_t)(smax + (smax & 1)) * 4 + (size_t)smax * sizeof(T);
 }
 
@@ -382,27 +382,23 @@ size_t range_select_lds_bytes(uint32_t smax) {
     default: break;                                 \
   }
 
+// the learned schedule's bucket count when a.selBuckets names it, else the other one
 template <typename T>
 void range_select(const JoinArgs &a, const typename GridOf<T>::type &g, hipStream_t s) {
+  const bool learned = a.selBuckets == (uint32_t)SEL_NB_LEARNED;
+#define SEL_CASE(DD)                                                        \
+  case DD:                                                                  \
+    if (learned)                                                            \
+      select_launch<T, DD, SEL_NB_LEARNED>(a, g, g.nd, s);                  \
+    else                                                                    \
+      select_launch<T, DD, SEL_NB_OTHER>(a, g, g.nd, s);                    \
+    break;
   switch (a.d) {
-    case 1: select_launch<T, 1>(a, g, g.nd, s); break;
-    case 2: select_launch<T, 2>(a, g, g.nd, s); break;
-    case 3: select_launch<T, 3>(a, g, g.nd, s); break;
-    case 4: select_launch<T, 4>(a, g, g.nd, s); break;
-    case 5: select_launch<T, 5>(a, g, g.nd, s); break;
-    case 6: select_launch<T, 6>(a, g, g.nd, s); break;
-    case 7: select_launch<T, 7>(a, g, g.nd, s); break;
-    case 8: select_launch<T, 8>(a, g, g.nd, s); break;
-    case 9: select_launch<T, 9>(a, g, g.nd, s); break;
-    case 10: select_launch<T, 10>(a, g, g.nd, s); break;
-    case 11: select_launch<T, 11>(a, g, g.nd, s); break;
-    case 12: select_launch<T, 12>(a, g, g.nd, s); break;
-    case 13: select_launch<T, 13>(a, g, g.nd, s); break;
-    case 14: select_launch<T, 14>(a, g, g.nd, s); break;
-    case 15: select_launch<T, 15>(a, g, g.nd, s); break;
-    case 16: select_launch<T, 16>(a, g, g.nd, s); break;
+    SEL_CASE(1) SEL_CASE(2) SEL_CASE(3) SEL_CASE(4) SEL_CASE(5) SEL_CASE(6) SEL_CASE(7) SEL_CASE(8)
+    SEL_CASE(9) SEL_CASE(10) SEL_CASE(11) SEL_CASE(12) SEL_CASE(13) SEL_CASE(14) SEL_CASE(15) SEL_CASE(16)
     default: break;
   }
+#undef SEL_CASE
 }
 
 template <typename T>
@@ -432,7 +428,7 @@ size_t range_heap_scratch_bytes(uint32_t ntiles, uint32_t k1) {
 
 #define INST(T)                                                                          \
   template void range_select<T>(const JoinArgs &, const GridOf<T>::type &, hipStream_t); \
-  template size_t range_select_lds_bytes<T>(uint32_t);                                   \
+  template size_t range_select_lds_bytes<T>(uint32_t, uint32_t);                                   \
   template void range_topk<T>(const JoinArgs &, const GridOf<T>::type &, hipStream_t);   \
   template void range_collect<T>(const JoinArgs &, const GridOf<T>::type &, hipStream_t); \
   template size_t range_heap_scratch_bytes<T>(uint32_t, uint32_t);

@@ -15,14 +15,16 @@
 // block, so a success is the exact global top-(K+1).  The reference's FP32 test
 // (kernel.cu:130-147) differs from this only for candidates within 2^-20 of eps.
 //
-// Buckets: 63 equal buckets over [0, Tq] of the FP64 key, and bucket 63 for every key beyond
-// (or a NaN).  floor(min(key * scale, 63.5)) with scale = 63 / Tq is monotone in the key (a
-// product by a positive constant, a minimum and a truncation, each monotone under
-// round-to-nearest), so every key of bucket b is <= every key of bucket b + 1, and a key of a
-// bucket <= 62 is below Tq (1 + 2^-51) < r^2: such a point lies inside the block of radius r.
+// Buckets: M = SEL_NB - 1 equal buckets over [0, Tq] of the FP64 key (M = 39 or 55: the unit is
+// compiled for the learned schedule's bucket count and for the other passes'), and bucket M
+// for every key beyond (or a NaN).  floor(min(key * scale, M + 0.5)) with scale = M / Tq is
+// monotone in the key (a product by a positive constant, a minimum and a truncation, each
+// monotone under round-to-nearest), so every key of bucket b is <= every key of bucket b + 1,
+// and a key of a bucket <= M - 1 is below Tq (1 + 2^-51) < r^2: such a point lies inside the
+// block of radius r.
 //
 // Per tile:
-//   pass 1  (lane = query) histogram of the keys of the tile's union of blocks over the 64
+//   pass 1  (lane = query) histogram of the keys of the tile's union of blocks over the SEL_NB
 //           buckets: every candidate of a round is broadcast to all lanes (v_readlane), each
 //           lane adds one to its own counter of the candidate's bucket (LDS, no return);
 //   scan    b* = first bucket whose cumulative count reaches K+1 (b* <= 62: success; the K+1
@@ -749,15 +751,16 @@ __global__ __launch_bounds__(64) void k_select(JoinArgs a, G g0) {
 template <typename T, int D, int ND, typename G>
 void launch_nd(const JoinArgs &a, const G &g, hipStream_t s) {
   const uint32_t grid = a.ntilesDev ? a.ntiles : range_select_grid(a.ntiles);
-  k_select<T, D, ND, G><<<grid, 64, range_select_lds_bytes<T>(a.smax), s>>>(a, g);
+  k_select<T, D, ND, G><<<grid, 64, range_select_lds_bytes<T>(a.smax, (uint32_t)SEL_NB), s>>>(a, g);
 }
 
 }  // namespace
 
 // Launches the select kernel for D data dims and nd indexed dims (nd <= 4 on a kernel built
 // for that count, else the runtime-count kernel).
-template <typename T, int D>
+template <typename T, int D, int NB>
 void select_launch(const JoinArgs &a, const typename GridOf<T>::type &g, uint32_t nd, hipStream_t s) {
+  static_assert(NB == SEL_NB, "select_launch is built for this unit's bucket count");
   using G = typename GridOf<T>::type;
   if (a.ntiles == 0) return;
   if ((uint64_t)a.n * sizeof(T) > 0xffffffffull) {
@@ -776,7 +779,7 @@ void select_launch(const JoinArgs &a, const typename GridOf<T>::type &g, uint32_
 }
 
 #define SEL_INST(T, D) \
-  template void select_launch<T, D>(const JoinArgs &, const GridOf<T>::type &, uint32_t, hipStream_t);
+  template void select_launch<T, D, SEL_NB>(const JoinArgs &, const GridOf<T>::type &, uint32_t, hipStream_t);
 #if KNN_SELECT_PART < 3
 #define KNN_SEL_T float
 #else
