@@ -409,6 +409,7 @@ def main():
         "modes": {m: {"queries_per_s": r["qps"], "ms_per_step": r["ms_per_step"],
                       "ms_reference_total": r["stats"]["ms_reference_total"],
                       "iterations": r["stats"]["iterations"], "reindexes": r["stats"]["reindexes"],
+                      "host_syncs": r["stats"]["host_syncs"],
                       "candidates": r["stats"]["candidates"],
                       "range_kernel_ms": r["stats"]["ms_range_kernel"],
                       "checksum": r["stats"]["checksum"]} for m, r in res.items()},
